@@ -1,0 +1,600 @@
+// Coarsest-level initial partitioner (host C++).
+//
+// The reference hands its coarsest graph to METIS_PartGraphKway
+// (gkway/metis_partition.hpp:46-47).  The METIS it uses exists only as a
+// prebuilt binary, so this build supplies its own deterministic partitioner
+// for that step:
+//   multilevel recursive bisection
+//     - coarsening by sequential heavy-edge matching in a seeded random order,
+//     - initial bisection by greedy graph growing from several seeds,
+//     - 2-way Fiduccia-Mattheyses refinement at every level,
+//   followed by a k-way greedy boundary pass that repairs any part above the
+//   cap and takes positive-gain moves.
+// Every random choice comes from a SplitMix64 stream seeded by (seed, depth,
+// first part id), so the output depends only on the input graph, k, cap and
+// seed -- never on the number of threads.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <future>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "../../include/gkway.h"
+#include "host_graph.hpp"
+
+#ifndef GKIP_NCUTS0
+#define GKIP_NCUTS0 4
+#endif
+#ifndef GKIP_NCUTS1
+#define GKIP_NCUTS1 3
+#endif
+#ifndef GKIP_NCUTS2
+#define GKIP_NCUTS2 2
+#endif
+#ifndef GKIP_FMPASS
+#define GKIP_FMPASS 6
+#endif
+#ifndef GKIP_TRIALS
+#define GKIP_TRIALS 8
+#endif
+
+namespace {
+
+struct Rng {
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  uint32_t below(uint32_t m) { return static_cast<uint32_t>((static_cast<uint64_t>(next() >> 32) * m) >> 32); }
+};
+
+struct G {
+  int32_t n = 0;
+  std::vector<int32_t> xadj, adj, w, vw;
+  int64_t tw = 0;  // total vertex weight
+};
+
+// ------------------------------------------------------------------ heap
+// Indexed binary max-heap over vertices keyed by (gain, -vertex).
+class Heap {
+ public:
+  void reset(int32_t n) {
+    pos_.assign(n, -1);
+    h_.clear();
+  }
+  bool empty() const { return h_.empty(); }
+  bool contains(int32_t v) const { return pos_[v] >= 0; }
+  int32_t top() const { return h_[0].v; }
+  int64_t top_key() const { return h_[0].g; }
+  void push(int32_t v, int64_t g) {
+    pos_[v] = static_cast<int32_t>(h_.size());
+    h_.push_back({g, v});
+    up(pos_[v]);
+  }
+  void update(int32_t v, int64_t g) {
+    int32_t i = pos_[v];
+    int64_t old = h_[i].g;
+    h_[i].g = g;
+    if (g > old) up(i); else down(i);
+  }
+  void remove(int32_t v) {
+    int32_t i = pos_[v];
+    pos_[v] = -1;
+    if (i == static_cast<int32_t>(h_.size()) - 1) { h_.pop_back(); return; }
+    h_[i] = h_.back();
+    h_.pop_back();
+    pos_[h_[i].v] = i;
+    up(i);
+    down(pos_[h_[i].v]);
+  }
+ private:
+  struct E { int64_t g; int32_t v; };
+  static bool better(const E& a, const E& b) { return a.g != b.g ? a.g > b.g : a.v < b.v; }
+  void up(int32_t i) {
+    while (i > 0) {
+      int32_t p = (i - 1) / 2;
+      if (!better(h_[i], h_[p])) break;
+      std::swap(h_[i], h_[p]);
+      pos_[h_[i].v] = i;
+      pos_[h_[p].v] = p;
+      i = p;
+    }
+  }
+  void down(int32_t i) {
+    int32_t n = static_cast<int32_t>(h_.size());
+    while (true) {
+      int32_t l = 2 * i + 1, r = l + 1, b = i;
+      if (l < n && better(h_[l], h_[b])) b = l;
+      if (r < n && better(h_[r], h_[b])) b = r;
+      if (b == i) break;
+      std::swap(h_[i], h_[b]);
+      pos_[h_[i].v] = i;
+      pos_[h_[b].v] = b;
+      i = b;
+    }
+  }
+  std::vector<E> h_;
+  std::vector<int32_t> pos_;
+};
+
+// ------------------------------------------------------------------ coarsening
+// Heavy-edge matching in a seeded random visiting order.  Ties between equal
+// edge weights go to the lighter neighbour, then the lower id.  cmap gets the
+// coarse id of every vertex; returns the coarse graph.
+G coarsen_hem(const G& g, Rng& rng, int64_t maxvw, std::vector<int32_t>& cmap) {
+  const int32_t n = g.n;
+  std::vector<int32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  for (int32_t i = n - 1; i > 0; --i) std::swap(perm[i], perm[rng.below(static_cast<uint32_t>(i + 1))]);
+  std::vector<int32_t> match(n, -1);
+  for (int32_t v : perm) {
+    if (match[v] >= 0) continue;
+    int32_t best = -1;
+    int32_t bw = 0, bvw = 0;
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+      int32_t u = g.adj[j];
+      if (match[u] >= 0) continue;
+      if (int64_t(g.vw[v]) + g.vw[u] > maxvw) continue;
+      int32_t w = g.w[j];
+      if (best < 0 || w > bw || (w == bw && (g.vw[u] < bvw || (g.vw[u] == bvw && u < best)))) {
+        best = u; bw = w; bvw = g.vw[u];
+      }
+    }
+    if (best >= 0) { match[v] = best; match[best] = v; }
+    else match[v] = v;
+  }
+  cmap.assign(n, -1);
+  int32_t nc = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    if (cmap[v] >= 0) continue;
+    cmap[v] = nc;
+    if (match[v] != v) cmap[match[v]] = nc;
+    ++nc;
+  }
+  G c;
+  c.n = nc;
+  c.vw.assign(nc, 0);
+  c.xadj.assign(nc + 1, 0);
+  c.tw = g.tw;
+  std::vector<int32_t> members(2 * nc, -1);
+  for (int32_t v = 0; v < n; ++v) {
+    int32_t cv = cmap[v];
+    c.vw[cv] += g.vw[v];
+    if (members[2 * cv] < 0) members[2 * cv] = v; else members[2 * cv + 1] = v;
+  }
+  std::vector<int32_t> slot(nc, -1);
+  c.adj.reserve(g.adj.size());
+  c.w.reserve(g.adj.size());
+  for (int32_t cv = 0; cv < nc; ++cv) {
+    int32_t start = static_cast<int32_t>(c.adj.size());
+    for (int k = 0; k < 2; ++k) {
+      int32_t v = members[2 * cv + k];
+      if (v < 0) continue;
+      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+        int32_t cu = cmap[g.adj[j]];
+        if (cu == cv) continue;
+        if (slot[cu] >= start) { c.w[slot[cu]] += g.w[j]; }
+        else {
+          slot[cu] = static_cast<int32_t>(c.adj.size());
+          c.adj.push_back(cu);
+          c.w.push_back(g.w[j]);
+        }
+      }
+    }
+    c.xadj[cv + 1] = static_cast<int32_t>(c.adj.size());
+  }
+  return c;
+}
+
+// ------------------------------------------------------------------ 2-way FM
+struct Bal {
+  int64_t max[2];
+};
+
+inline int64_t overload(const int64_t pw[2], const Bal& b) {
+  return std::max<int64_t>(0, pw[0] - b.max[0]) + std::max<int64_t>(0, pw[1] - b.max[1]);
+}
+
+// Cut of a 2-way partition.
+int64_t cut2(const G& g, const std::vector<uint8_t>& side) {
+  int64_t c = 0;
+  for (int32_t v = 0; v < g.n; ++v)
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+      if (side[v] != side[g.adj[j]]) c += g.w[j];
+  return c / 2;
+}
+
+// Boundary FM refinement.  Moves that keep both sides within their maxima
+// (or reduce the overload) are taken in max-gain order; the best prefix of a
+// pass (lowest (overload, cut)) is kept.
+void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int64_t& cut) {
+  const int32_t n = g.n;
+  std::vector<int64_t> id(n), ed(n);
+  int64_t pw[2] = {0, 0};
+  for (int32_t v = 0; v < n; ++v) pw[side[v]] += g.vw[v];
+  Heap q[2];
+  std::vector<uint8_t> locked(n);
+  std::vector<int32_t> moved;
+  const int32_t limit = std::max<int32_t>(25, std::min<int32_t>(n / 20, 300));
+  for (int pass = 0; pass < passes; ++pass) {
+    for (int32_t v = 0; v < n; ++v) {
+      id[v] = ed[v] = 0;
+      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+        (side[g.adj[j]] == side[v] ? id[v] : ed[v]) += g.w[j];
+    }
+    q[0].reset(n);
+    q[1].reset(n);
+    std::fill(locked.begin(), locked.end(), 0);
+    for (int32_t v = 0; v < n; ++v)
+      if (ed[v] > 0) q[side[v]].push(v, ed[v] - id[v]);
+    moved.clear();
+    int64_t cur_cut = cut, best_cut = cut;
+    int64_t best_ol = overload(pw, bal);
+    int32_t best_len = 0, since_best = 0;
+    int64_t cur_pw[2] = {pw[0], pw[1]};
+    while (true) {
+      // Pick the source side: an overloaded side must shed weight; otherwise
+      // take the better top among moves that keep the target within bounds.
+      int from = -1;
+      const bool ol0 = cur_pw[0] > bal.max[0], ol1 = cur_pw[1] > bal.max[1];
+      if (ol0 && !q[0].empty()) from = 0;
+      else if (ol1 && !q[1].empty()) from = 1;
+      else {
+        int64_t bg = INT64_MIN;
+        for (int s = 0; s < 2; ++s) {
+          if (q[s].empty()) continue;
+          int32_t v = q[s].top();
+          if (cur_pw[1 - s] + g.vw[v] > bal.max[1 - s]) continue;
+          if (q[s].top_key() > bg) { bg = q[s].top_key(); from = s; }
+        }
+      }
+      if (from < 0) break;
+      int32_t v = q[from].top();
+      q[from].remove(v);
+      const int to = 1 - from;
+      if (!(ol0 || ol1) && cur_pw[to] + g.vw[v] > bal.max[to]) continue;
+      cur_cut -= ed[v] - id[v];
+      cur_pw[from] -= g.vw[v];
+      cur_pw[to] += g.vw[v];
+      side[v] = static_cast<uint8_t>(to);
+      locked[v] = 1;
+      std::swap(id[v], ed[v]);
+      moved.push_back(v);
+      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+        int32_t u = g.adj[j];
+        int32_t w = g.w[j];
+        if (side[u] == to) { id[u] += w; ed[u] -= w; }
+        else { id[u] -= w; ed[u] += w; }
+        if (locked[u]) continue;
+        int s = side[u];
+        if (q[s].contains(u)) {
+          if (ed[u] > 0) q[s].update(u, ed[u] - id[u]); else q[s].remove(u);
+        } else if (ed[u] > 0) {
+          q[s].push(u, ed[u] - id[u]);
+        }
+      }
+      int64_t ol = overload(cur_pw, bal);
+      if (ol < best_ol || (ol == best_ol && cur_cut < best_cut)) {
+        best_ol = ol;
+        best_cut = cur_cut;
+        best_len = static_cast<int32_t>(moved.size());
+        since_best = 0;
+      } else if (++since_best > limit) {
+        break;
+      }
+    }
+    // Roll back past the best prefix.
+    for (int32_t i = static_cast<int32_t>(moved.size()) - 1; i >= best_len; --i) {
+      int32_t v = moved[i];
+      int to = side[v], from = 1 - to;
+      side[v] = static_cast<uint8_t>(from);
+    }
+    pw[0] = pw[1] = 0;
+    for (int32_t v = 0; v < n; ++v) pw[side[v]] += g.vw[v];
+    bool improved = best_cut < cut || best_len > 0;
+    cut = best_cut;
+    if (!improved || best_len == 0) break;
+  }
+}
+
+// Greedy graph growing from `seedv`: side 0 grows by max gain until it
+// reaches its target weight.
+void grow(const G& g, int32_t seedv, int64_t target0, std::vector<uint8_t>& side) {
+  const int32_t n = g.n;
+  side.assign(n, 1);
+  std::vector<int64_t> conn0(n, 0);
+  Heap q;
+  q.reset(n);
+  int64_t w0 = 0;
+  std::vector<uint8_t> touched(n, 0);
+  int32_t next_unreached = 0;
+  auto add = [&](int32_t v) {
+    side[v] = 0;
+    w0 += g.vw[v];
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+      int32_t u = g.adj[j];
+      if (side[u] == 0) continue;
+      conn0[u] += g.w[j];
+      // gain of moving u to side 0: 2*conn0 - total incident weight
+      int64_t tot = 0;
+      for (int32_t k = g.xadj[u]; k < g.xadj[u + 1]; ++k) tot += g.w[k];
+      int64_t gain = 2 * conn0[u] - tot;
+      if (q.contains(u)) q.update(u, gain); else q.push(u, gain);
+    }
+  };
+  add(seedv);
+  while (w0 < target0) {
+    int32_t v = -1;
+    if (!q.empty()) { v = q.top(); q.remove(v); }
+    else {
+      while (next_unreached < n && side[next_unreached] == 0) ++next_unreached;
+      if (next_unreached >= n) break;
+      v = next_unreached;
+    }
+    if (side[v] == 0) continue;
+    // Stop before overshooting by more than half the vertex weight.
+    if (w0 + g.vw[v] > target0 && (w0 + g.vw[v] - target0) > (target0 - w0)) break;
+    add(v);
+  }
+}
+
+// Multilevel bisection of g.  target0 = desired weight of side 0.
+void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vector<uint8_t>& side) {
+  Rng rng(seed);
+  const int64_t t1 = g.tw - target0;
+  Bal bal;
+  bal.max[0] = static_cast<int64_t>(std::floor(double(target0) * (1.0 + tol)));
+  bal.max[1] = static_cast<int64_t>(std::floor(double(t1) * (1.0 + tol)));
+  // Coarsen.
+  std::vector<G> levels;
+  std::vector<std::vector<int32_t>> cmaps;
+  const G* cur = &g;
+  const int64_t maxvw = std::max<int64_t>(1, static_cast<int64_t>(1.5 * double(g.tw) / 60.0));
+  while (cur->n > 120) {
+    std::vector<int32_t> cm;
+    G c = coarsen_hem(*cur, rng, maxvw, cm);
+    if (c.n > 0.92 * cur->n) break;
+    cmaps.push_back(std::move(cm));
+    levels.push_back(std::move(c));
+    cur = &levels.back();
+  }
+  // Initial bisection: best of several grown regions.
+  const G& cg = *cur;
+  std::vector<uint8_t> best, trial;
+  int64_t best_cut = INT64_MAX, best_ol = INT64_MAX;
+  const int trials = cg.n <= 1 ? 1 : GKIP_TRIALS;
+  for (int t = 0; t < trials; ++t) {
+    int32_t s = static_cast<int32_t>(rng.below(static_cast<uint32_t>(std::max(1, cg.n))));
+    grow(cg, s, target0, trial);
+    int64_t c = cut2(cg, trial);
+    fm2(cg, trial, bal, 4, c);
+    int64_t pw[2] = {0, 0};
+    for (int32_t v = 0; v < cg.n; ++v) pw[trial[v]] += cg.vw[v];
+    int64_t ol = overload(pw, bal);
+    if (ol < best_ol || (ol == best_ol && c < best_cut)) { best_ol = ol; best_cut = c; best = trial; }
+  }
+  // Uncoarsen with refinement.
+  side = std::move(best);
+  for (int32_t l = static_cast<int32_t>(levels.size()) - 1; l >= 0; --l) {
+    const G& fine = (l == 0) ? g : levels[l - 1];
+    const std::vector<int32_t>& cm = cmaps[l];
+    std::vector<uint8_t> fs(fine.n);
+    for (int32_t v = 0; v < fine.n; ++v) fs[v] = side[cm[v]];
+    side = std::move(fs);
+    int64_t c = best_cut;  // projection preserves the cut
+    fm2(fine, side, bal, GKIP_FMPASS, c);
+    best_cut = c;
+  }
+}
+
+// Induced subgraph of the vertices with side[v] == s.
+G induced(const G& g, const std::vector<uint8_t>& side, uint8_t s, std::vector<int32_t>& map_back) {
+  std::vector<int32_t> loc(g.n, -1);
+  map_back.clear();
+  for (int32_t v = 0; v < g.n; ++v)
+    if (side[v] == s) { loc[v] = static_cast<int32_t>(map_back.size()); map_back.push_back(v); }
+  G h;
+  h.n = static_cast<int32_t>(map_back.size());
+  h.xadj.assign(h.n + 1, 0);
+  h.vw.resize(h.n);
+  for (int32_t i = 0; i < h.n; ++i) {
+    int32_t v = map_back[i];
+    h.vw[i] = g.vw[v];
+    h.tw += g.vw[v];
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+      int32_t u = loc[g.adj[j]];
+      if (u < 0) continue;
+      h.adj.push_back(u);
+      h.w.push_back(g.w[j]);
+    }
+    h.xadj[i + 1] = static_cast<int32_t>(h.adj.size());
+  }
+  return h;
+}
+
+struct RBCtx {
+  double tol;
+  int ncuts0, ncuts1, ncuts2;
+  uint64_t seed;
+  std::atomic<int> threads_left;
+};
+
+void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* out,
+        const std::vector<int32_t>& ids) {
+  if (k == 1 || g.n == 0) {
+    for (int32_t i = 0; i < g.n; ++i) out[ids[i]] = part0;
+    return;
+  }
+  if (g.n == 1) {
+    out[ids[0]] = part0;
+    return;
+  }
+  const int32_t k0 = k / 2, k1 = k - k0;
+  const int64_t target0 = g.tw * k0 / k;
+  std::vector<uint8_t> side;
+  const uint64_t s = ctx.seed * 0x9E3779B97F4A7C15ull ^ (uint64_t(depth) << 40) ^ uint64_t(part0) * 0xD1B54A32D192ED03ull;
+  // Several independent multilevel bisections (more near the root, where a
+  // bisection decides the most); keep the lowest (overload, cut, trial).
+  // Trials run on spare threads; the pick does not depend on scheduling.
+  const int ncuts = depth == 0 ? ctx.ncuts0 : (depth < 3 ? ctx.ncuts1 : ctx.ncuts2);
+  std::vector<std::vector<uint8_t>> trial(ncuts);
+  std::vector<int64_t> key_ol(ncuts), key_cut(ncuts);
+  auto run_trial = [&](int c) {
+    ml_bisect(g, target0, ctx.tol, s + 0x51ED2701ull * uint64_t(c), trial[c]);
+    int64_t pw0 = 0;
+    for (int32_t v = 0; v < g.n; ++v) if (trial[c][v] == 0) pw0 += g.vw[v];
+    const int64_t t1 = g.tw - target0;
+    const int64_t m0 = static_cast<int64_t>(std::floor(double(target0) * (1.0 + ctx.tol)));
+    const int64_t m1 = static_cast<int64_t>(std::floor(double(t1) * (1.0 + ctx.tol)));
+    key_ol[c] = std::max<int64_t>(0, pw0 - m0) + std::max<int64_t>(0, (g.tw - pw0) - m1);
+    key_cut[c] = cut2(g, trial[c]);
+  };
+  {
+    std::vector<std::future<void>> futs;
+    int c = 1;
+    for (; c < ncuts && g.n > 2000; ++c) {
+      int left = ctx.threads_left.load();
+      while (left > 0 && !ctx.threads_left.compare_exchange_weak(left, left - 1)) {}
+      if (left <= 0) break;
+      futs.push_back(std::async(std::launch::async, [&, c] { run_trial(c); ctx.threads_left.fetch_add(1); }));
+    }
+    run_trial(0);
+    for (; c < ncuts; ++c) run_trial(c);
+    for (auto& f : futs) f.get();
+  }
+  int bc = 0;
+  for (int c = 1; c < ncuts; ++c)
+    if (key_ol[c] < key_ol[bc] || (key_ol[c] == key_ol[bc] && key_cut[c] < key_cut[bc])) bc = c;
+  side.swap(trial[bc]);
+  std::vector<int32_t> mb0, mb1;
+  G g0 = induced(g, side, 0, mb0);
+  G g1 = induced(g, side, 1, mb1);
+  std::vector<int32_t> ids0(mb0.size()), ids1(mb1.size());
+  for (size_t i = 0; i < mb0.size(); ++i) ids0[i] = ids[mb0[i]];
+  for (size_t i = 0; i < mb1.size(); ++i) ids1[i] = ids[mb1[i]];
+  // Run the two halves concurrently while threads remain; the result does
+  // not depend on which branch runs where.
+  bool spawn = false;
+  if (g.n > 2000) {
+    int left = ctx.threads_left.load();
+    while (left > 0 && !ctx.threads_left.compare_exchange_weak(left, left - 1)) {}
+    spawn = left > 0;
+  }
+  if (spawn) {
+    auto fut = std::async(std::launch::async, [&] { rb(g0, k0, part0, depth + 1, ctx, out, ids0); });
+    rb(g1, k1, part0 + k0, depth + 1, ctx, out, ids1);
+    fut.get();
+    ctx.threads_left.fetch_add(1);
+  } else {
+    rb(g0, k0, part0, depth + 1, ctx, out, ids0);
+    rb(g1, k1, part0 + k0, depth + 1, ctx, out, ids1);
+  }
+}
+
+// k-way greedy boundary pass: repair parts above cap, then positive-gain
+// moves that respect the cap.  Vertices are visited in id order each sweep.
+void kway_polish(const G& g, int32_t k, int64_t cap, int32_t* part, int sweeps) {
+  std::vector<int64_t> pw(k, 0);
+  for (int32_t v = 0; v < g.n; ++v) pw[part[v]] += g.vw[v];
+  std::vector<int64_t> conn(k, 0);
+  std::vector<int32_t> touched;
+  for (int s = 0; s < sweeps; ++s) {
+    int64_t moves = 0;
+    for (int32_t v = 0; v < g.n; ++v) {
+      const int32_t p = part[v];
+      touched.clear();
+      bool boundary = false;
+      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+        int32_t q = part[g.adj[j]];
+        if (conn[q] == 0) touched.push_back(q);
+        conn[q] += g.w[j];
+        if (q != p) boundary = true;
+      }
+      if (boundary) {
+        const bool over = pw[p] > cap;
+        int32_t bestq = -1;
+        int64_t bestg = 0;
+        for (int32_t q : touched) {
+          if (q == p) continue;
+          if (pw[q] + g.vw[v] > cap) continue;
+          int64_t gain = conn[q] - conn[p];
+          bool better;
+          if (over) better = bestq < 0 || gain > bestg || (gain == bestg && (pw[q] < pw[bestq] || (pw[q] == pw[bestq] && q < bestq)));
+          else better = gain > bestg || (gain == bestg && gain > 0 && q < bestq);
+          if (better) { bestq = q; bestg = gain; }
+        }
+        if (bestq >= 0 && (over || bestg > 0)) {
+          pw[p] -= g.vw[v];
+          pw[bestq] += g.vw[v];
+          part[v] = bestq;
+          ++moves;
+        }
+      }
+      for (int32_t q : touched) conn[q] = 0;
+    }
+    if (moves == 0) break;
+  }
+}
+
+}  // namespace
+
+extern "C" int64_t gk_part_cap(int64_t total_vwgt, int32_t k, int32_t ubfactor) {
+  if (k < 1) return total_vwgt;
+  int64_t a = total_vwgt * (1000 + int64_t(ubfactor)) / (1000 * int64_t(k));
+  int64_t b = (total_vwgt + k - 1) / k;
+  return std::max(a, b);
+}
+
+extern "C" int gk_initpart(const gk_csr* hg, int32_t k, int64_t cap, int32_t seed,
+                           int32_t nthreads, int32_t* part, int64_t* cut) {
+  if (!hg || !part || k < 1 || hg->n < 0) return GK_ERR_ARG;
+  G g;
+  g.n = hg->n;
+  g.xadj.assign(hg->xadj, hg->xadj + hg->n + 1);
+  const int32_t nnz = g.xadj[g.n];
+  g.adj.assign(hg->adjncy, hg->adjncy + nnz);
+  if (hg->adjwgt) g.w.assign(hg->adjwgt, hg->adjwgt + nnz); else g.w.assign(nnz, 1);
+  if (hg->vwgt) g.vw.assign(hg->vwgt, hg->vwgt + g.n); else g.vw.assign(g.n, 1);
+  g.tw = 0;
+  for (int32_t v = 0; v < g.n; ++v) g.tw += g.vw[v];
+  if (k == 1) {
+    std::fill(part, part + g.n, 0);
+    if (cut) *cut = 0;
+    return GK_OK;
+  }
+  // Per-bisection tolerance: (cap / (T/k))^(1/levels) - 1, slightly tightened
+  // so that rounding of the recursive targets stays inside the cap.
+  const int levels = static_cast<int>(std::ceil(std::log2(double(k))));
+  const double ratio = double(cap) * k / std::max<double>(1.0, double(g.tw));
+  double tol = std::pow(std::max(1.0, ratio), 1.0 / std::max(1, levels)) - 1.0;
+  tol = std::max(0.0, tol * 0.9);
+  RBCtx ctx;
+  ctx.tol = tol;
+  ctx.ncuts0 = GKIP_NCUTS0;
+  ctx.ncuts1 = GKIP_NCUTS1;
+  ctx.ncuts2 = GKIP_NCUTS2;
+  ctx.seed = static_cast<uint64_t>(seed) + 1;
+  unsigned hw = std::thread::hardware_concurrency();
+  int nt = nthreads > 0 ? nthreads : static_cast<int>(hw ? std::min(hw, 16u) : 1u);
+  ctx.threads_left.store(std::max(0, nt - 1));
+  std::vector<int32_t> ids(g.n);
+  std::iota(ids.begin(), ids.end(), 0);
+  rb(g, k, 0, 0, ctx, part, ids);
+  kway_polish(g, k, cap, part, 4);
+  if (cut) {
+    int64_t c = 0;
+    for (int32_t v = 0; v < g.n; ++v)
+      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+        if (v < g.adj[j] && part[v] != part[g.adj[j]]) c += g.w[j];
+    *cut = c;
+  }
+  return GK_OK;
+}

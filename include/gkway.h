@@ -1,0 +1,210 @@
+/* gkway.h — C ABI of the MI355X (gfx950) multilevel k-way graph partitioner.
+ *
+ * One shared library, libgkway.so, exports everything below.  Signatures use
+ * plain pointers, sizes and opaque handles only.  Device pointers are HIP
+ * device addresses; `stream` arguments are hipStream_t passed as void*.
+ *
+ * What each entry point replaces in the reference (wanluanlee/DAC_g-kway):
+ *   gk_partition_file      graph_partitioner()          gkway/graph_partitioner.hpp:222-365
+ *   gk_partition           the V-cycle inside it         gkway/graph_partitioner.hpp:297-335
+ *   gk_graph_read          gk::Graph::_parse             gkway/graph.hpp:70-152
+ *   gk_write_partition     gk::Graph::dump_result        gkway/graph.hpp:227-240
+ *   gk_initpart            metis_init_partition          gkway/metis_partition.hpp:4-99
+ *   gk_k_match             matching_with_score           gkway/coarsen.hpp:39-78
+ *   gk_k_group             init_group_id + HEC_union_find loop
+ *                                                        gkway/coarsen.hpp:351-359,80-118,461-469
+ *   gk_k_cluster           constraint_group_size + coarse numbering + cal_coarsen_vwgt
+ *                                                        gkway/coarsen.hpp:361-421,120-171,472-483
+ *   gk_k_contract          contraction                   gkway/coarsen.hpp:302-349
+ *   gk_k_project           expand_partition              gkway/uncoarsen.hpp:325-356
+ *   gk_k_gain              update_vertex_info            gkway/uncoarsen.hpp:175-237
+ *   gk_k_select            create_independent_move_buffer + mergesort
+ *                                                        gkway/uncoarsen.hpp:111-173,471-476
+ *   gk_k_balance           find_mv_wgt_sequence + scan + find_balance_sequence
+ *                                                        gkway/uncoarsen.hpp:239-288,37-80
+ *   gk_k_apply             apply_sequence_move           gkway/uncoarsen.hpp:290-323
+ *   gk_k_cut               calculate_cutsize + calculate_partition_wgt
+ *                                                        gkway/uncoarsen.hpp:398-424,
+ *                                                        gkway/cuda_check.hpp:61-73
+ * The reference has no C ABI at all: these stage functions are C++ templates
+ * taking device pointers and a moderngpu context.  Errors are status codes
+ * (never exceptions across the ABI); gk_strerror() names them and
+ * gk_last_error() returns the last detailed message of the calling thread.
+ */
+#ifndef GKWAY_H
+#define GKWAY_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GK_OK 0
+#define GK_ERR_ARG -1        /* invalid argument (null pointer, k < 1, ...) */
+#define GK_ERR_IO -2         /* file could not be opened / written          */
+#define GK_ERR_FORMAT -3     /* malformed METIS file                        */
+#define GK_ERR_GRAPH -4      /* graph violates an input invariant           */
+#define GK_ERR_HIP -5        /* a HIP runtime call failed                   */
+#define GK_ERR_NOMEM -6      /* device or host allocation failed            */
+#define GK_ERR_LIMIT -7      /* size beyond a documented limit              */
+#define GK_ERR_INTERNAL -8   /* internal consistency check failed           */
+#define GK_ERR_NODEVICE -9   /* no HIP device / device code not loaded      */
+
+/* ---- graph description --------------------------------------------------
+ * Symmetric CSR: for every entry (v, u, w) there is an entry (u, v, w).
+ * adjncy is 0-based.  adjwgt / vwgt may be NULL (unit weights).  Limits:
+ * nnz < 2^31, total vertex weight < 2^31, total undirected edge weight < 2^31,
+ * weights >= 1, no self loops.                                                */
+typedef struct gk_csr {
+  int32_t n;               /* vertices                                  */
+  int32_t nnz;             /* directed adjacency entries (2 x edges)    */
+  const int32_t* xadj;     /* n + 1                                     */
+  const int32_t* adjncy;   /* nnz                                       */
+  const int32_t* adjwgt;   /* nnz or NULL                               */
+  const int32_t* vwgt;     /* n or NULL                                 */
+} gk_csr;
+
+/* Coarsening stop policy. */
+#define GK_COARSEN_TARGET 0  /* keep a level only while nc >= max(n0/(20*ceil(log2 k)), 30k) */
+#define GK_COARSEN_REF 1     /* the reference's rule: coarsen while n > 20k
+                                (gkway/graph_partitioner.hpp:22,66,93)   */
+
+typedef struct gk_params {
+  int32_t k;               /* number of parts (>= 1)                          */
+  int32_t ubfactor;        /* allowed imbalance in 1/1000 (30 = 3 %)          */
+  int32_t max_group;       /* coarse vertex size cap G (reference: 6)         */
+  int32_t max_moves;       /* moves applied per refinement round (ref: 1024)  */
+  int32_t coarsen_policy;  /* GK_COARSEN_TARGET or GK_COARSEN_REF             */
+  int32_t coarsen_to;      /* > 0 overrides the TARGET policy's floor         */
+  int32_t max_levels;      /* safety cap on hierarchy depth (0 = 64)          */
+  int32_t seed;            /* seed of the initial partitioner                 */
+  int32_t initpart_threads;/* host threads for the initial partitioner (0=auto)*/
+  int32_t flags;           /* GK_FLAG_* bits                                  */
+} gk_params;
+
+#define GK_FLAG_CHECK 1      /* validate input symmetry on entry             */
+#define GK_FLAG_TRACE 2      /* record per-level hashes in gk_stats.levels   */
+
+/* Fill *p with the defaults: ubfactor 30, max_group 6, max_moves 1024,
+ * TARGET policy, seed 1. k is left as given. */
+void gk_params_default(gk_params* p, int32_t k);
+
+#define GK_MAX_LEVELS 64
+
+typedef struct gk_level_stats {
+  int32_t n;               /* vertices of this level                          */
+  int32_t nnz;             /* directed entries of this level                  */
+  int32_t sweeps;          /* union-find sweeps run to coarsen this level     */
+  int32_t rounds;          /* refinement rounds run at this level             */
+  int64_t moves;           /* vertices moved at this level                    */
+  int64_t cut_after;       /* tracked cut after refining this level           */
+  uint64_t h_cand;         /* FNV-1a of cand[] (GK_FLAG_TRACE)                */
+  uint64_t h_cmap;         /* FNV-1a of cmap[]                                */
+  uint64_t h_coarse;       /* FNV-1a of the coarse CSR (xadj,adjncy,adjwgt,vwgt) */
+  uint64_t h_part;         /* FNV-1a of the partition after refining          */
+} gk_level_stats;
+
+typedef struct gk_stats {
+  int32_t nlevels;         /* levels kept, including level 0                   */
+  int32_t coarsest_n;
+  int64_t cut;             /* recomputed weighted edge cut of the result       */
+  int64_t cut_tracked;     /* initial cut minus applied gains (== cut)         */
+  int64_t init_cut;        /* cut of the initial partition at the coarsest    */
+  int64_t max_part_wgt;
+  int64_t total_vwgt;
+  int64_t cap;             /* maximum allowed part weight                      */
+  double imbalance;        /* max part weight / (total / k)                    */
+  double t_coarsen_ms, t_init_ms, t_refine_ms, t_total_ms;
+  int64_t alg_bytes;       /* algorithmic HBM bytes of the run (see DESIGN)   */
+  gk_level_stats levels[GK_MAX_LEVELS];
+} gk_stats;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct gk_ctx gk_ctx;
+int gk_ctx_create(int device, gk_ctx** out);
+int gk_ctx_destroy(gk_ctx* ctx);
+const char* gk_strerror(int status);
+const char* gk_last_error(void);
+
+/* ---- whole-graph partitioning ---------------------------------------------
+ * gk_partition: host CSR in, host partition out (copies included).
+ * gk_partition_device: CSR already in device memory, partition written to a
+ * device buffer; the work runs on `stream` (NULL = the context's stream).     */
+int gk_partition(gk_ctx* ctx, const gk_csr* host_graph, const gk_params* params,
+                 int32_t* part_out, gk_stats* stats);
+int gk_partition_device(gk_ctx* ctx, const gk_csr* dev_graph, const gk_params* params,
+                        int32_t* dev_part_out, gk_stats* stats, void* stream);
+/* graph_partitioner(GRAPH_FILE, OUT_FILE, k): read, partition, write
+ * "<OUT_FILE>.out" (path taken as given; the C++ wrapper prefixes the cwd). */
+int gk_partition_file(gk_ctx* ctx, const char* graph_file, const char* out_file,
+                      int32_t k, gk_stats* stats);
+
+/* ---- host I/O --------------------------------------------------------------
+ * gk_graph_read allocates the arrays with malloc; free with gk_graph_free.    */
+int gk_graph_read(const char* path, gk_csr* out);
+void gk_graph_free(gk_csr* g);
+int gk_graph_write(const char* path, const gk_csr* g);
+int gk_graph_check(const gk_csr* g, int check_symmetry);
+int gk_write_partition(const char* path, const int32_t* part, int32_t n);
+
+/* ---- coarsest-level initial partitioner (host) -----------------------------
+ * Deterministic multilevel recursive bisection with FM refinement; the role
+ * METIS_PartGraphKway plays in the reference (gkway/metis_partition.hpp:46-47).
+ * cap = maximum part weight.  Writes part[n]; *cut = weighted edge cut.       */
+int gk_initpart(const gk_csr* host_graph, int32_t k, int64_t cap, int32_t seed,
+                int32_t nthreads, int32_t* part, int64_t* cut);
+/* Same computation as gk_params-driven rule: cap for total weight T. */
+int64_t gk_part_cap(int64_t total_vwgt, int32_t k, int32_t ubfactor);
+/* Coarsening floor of the TARGET policy for a level-0 graph of n vertices. */
+int32_t gk_coarsen_floor(int32_t n0, int32_t k, int32_t policy, int32_t override_to);
+
+/* ---- per-stage kernels (device pointers; each is bit-compared with the
+ * oracle's stage of the same name) ------------------------------------------ */
+int gk_k_match(gk_ctx* ctx, const gk_csr* dg, int32_t* cand, void* stream);
+/* labels: max vertex id of the component of {v, cand[v]}; stamps: hop
+ * distance to that vertex; *sweeps: synchronous sweeps executed. */
+int gk_k_group(gk_ctx* ctx, int32_t n, const int32_t* cand, int32_t* label,
+               int32_t* stamp, int32_t* sweeps, void* stream);
+/* order: vertices sorted by (label, stamp, id); cmap: coarse id per vertex;
+ * cvwgt: coarse vertex weights; returns nc in *nc. */
+int gk_k_cluster(gk_ctx* ctx, int32_t n, const int32_t* label, const int32_t* stamp,
+                 const int32_t* vwgt, int32_t max_group, int32_t* order, int32_t* cmap,
+                 int32_t* cvwgt, int32_t* nc, void* stream);
+/* Coarse CSR of the contraction; cxadj needs nc+1 entries, cadjncy/cadjwgt
+ * need dg->nnz entries (upper bound); *cnnz receives the coarse entry count. */
+int gk_k_contract(gk_ctx* ctx, const gk_csr* dg, const int32_t* cmap, int32_t nc,
+                  int32_t* cxadj, int32_t* cadjncy, int32_t* cadjwgt, int32_t* cnnz,
+                  void* stream);
+int gk_k_project(gk_ctx* ctx, int32_t n_fine, const int32_t* cmap,
+                 const int32_t* cpart, const int32_t* cboundary, int32_t* part,
+                 int32_t* boundary, int32_t* gain, int32_t* dest, int32_t* updated,
+                 void* stream);
+/* Recompute boundary/gain/dest for vertices with updated[v] != 0. */
+int gk_k_gain(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part,
+              const int32_t* updated, int32_t* boundary, int32_t* gain, int32_t* dest,
+              void* stream);
+/* Selected moves sorted by (gain desc, id asc), truncated to max_moves.
+ * Clears updated[].  moves[i] = vertex id; *nsel = selected before truncation. */
+int gk_k_select(gk_ctx* ctx, const gk_csr* dg, const int32_t* part, const int32_t* boundary,
+                const int32_t* gain, const int32_t* dest, const int32_t* pw, int64_t cap,
+                int32_t max_moves, int32_t* updated, int32_t* moves, int32_t* nsel,
+                void* stream);
+/* Largest balanced prefix of the nmoves sorted moves: *pos (0-based, -1 when
+ * nmoves == 0); part weights pw[k] are advanced by the prefix. */
+int gk_k_balance(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* moves,
+                 int32_t nmoves, const int32_t* part, const int32_t* dest, int32_t* pw,
+                 int64_t cap, int32_t* pos, void* stream);
+/* Apply moves[0..pos]; marks movers and their neighbours updated; returns the
+ * summed gain of the applied moves in *gain_sum. */
+int gk_k_apply(gk_ctx* ctx, const gk_csr* dg, const int32_t* moves, int32_t pos,
+               const int32_t* gain, const int32_t* dest, int32_t* part, int32_t* updated,
+               int64_t* gain_sum, void* stream);
+/* Weighted cut over entries (v,u), v<u, part differs; part weights pw[k]. */
+int gk_k_cut(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part,
+             int64_t* cut, int64_t* pw, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GKWAY_H */
