@@ -1,0 +1,512 @@
+// Coarsening kernels for gfx950: heavy-edge candidate matching, grouping of
+// the candidate pseudo-forest by pointer jumping, chunking into coarse
+// vertices, and CSR contraction (relabel, per-row sort, merge of parallel
+// edges) -- the work of gkway/coarsen.hpp.  Every kernel is integer-exact and
+// writes results that do not depend on the launch schedule: atomics are used
+// only for commutative integer max/add or for slot reservation whose order is
+// discarded by a later sort.
+#include "kernels.hpp"
+#include "prim.hpp"
+#include "../../include/gkway.h"
+
+namespace gkd {
+
+int lanes_for_degree(double avg) {
+  if (avg <= 3.0) return 4;
+  if (avg <= 6.5) return 8;
+  if (avg <= 14.0) return 16;
+  if (avg <= 28.0) return 32;
+  return 64;
+}
+
+static inline int hip_status() { return hipGetLastError() == hipSuccess ? 0 : GK_ERR_HIP; }
+
+// --------------------------------------------------------------- matching
+// matching_with_score (gkway/coarsen.hpp:39-78) with S lanes per vertex: the
+// lanes stride over the row, then reduce the packed preference
+// (weight desc, neighbour degree asc, neighbour id asc) by xor-shuffles.
+template <int S>
+__global__ void __launch_bounds__(256) k_match(int32_t n, const int32_t* __restrict__ xadj,
+                                               const int32_t* __restrict__ adj,
+                                               const int32_t* __restrict__ w, int32_t* __restrict__ cand) {
+  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int32_t v = static_cast<int32_t>(gt / S);
+  const int sub = static_cast<int>(gt % S);
+  if (v >= n) return;
+  int bw = 0, bd = 0, bu = v;
+  const int s = xadj[v], e = xadj[v + 1];
+  for (int j = s + sub; j < e; j += S) {
+    const int u = adj[j];
+    const int wj = w[j];
+    const int d = xadj[u + 1] - xadj[u];
+    if (wj > bw || (wj == bw && (d < bd || (d == bd && u < bu)))) { bw = wj; bd = d; bu = u; }
+  }
+#pragma unroll
+  for (int o = S / 2; o > 0; o >>= 1) {
+    const int ow = __shfl_xor(bw, o, S), od = __shfl_xor(bd, o, S), ou = __shfl_xor(bu, o, S);
+    if (ow > bw || (ow == bw && (od < bd || (od == bd && ou < bu)))) { bw = ow; bd = od; bu = ou; }
+  }
+  if (sub == 0) cand[v] = bu;
+}
+
+int launch_match(const DGraph& g, int lanes, int32_t* cand, hipStream_t st) {
+  if (g.n <= 0) return 0;
+  const unsigned grid = grid_for(int64_t(g.n) * lanes, 256);
+  switch (lanes) {
+    case 1: k_match<1><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 2: k_match<2><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 4: k_match<4><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 8: k_match<8><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 16: k_match<16><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 32: k_match<32><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    default: k_match<64><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+  }
+  return hip_status();
+}
+
+// --------------------------------------------------------------- grouping
+// The candidate graph {v, cand[v]} is a pseudo-forest whose only cycles are
+// mutual pairs (or a self loop for an isolated vertex).  Pointer jumping
+// finds for every vertex its root pair and the hop distance to it.
+__global__ void __launch_bounds__(256) k_jump_init(int32_t n, const int32_t* __restrict__ cand,
+                                                   int32_t* __restrict__ next, int32_t* __restrict__ dist) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  const int32_t c = cand[v];
+  const bool root = cand[c] == v;
+  next[v] = root ? int32_t(v) : c;
+  dist[v] = root ? 0 : 1;
+}
+
+__global__ void __launch_bounds__(256) k_jump_round(int32_t n, const int32_t* __restrict__ next_in,
+                                                    const int32_t* __restrict__ dist_in,
+                                                    int32_t* __restrict__ next_out,
+                                                    int32_t* __restrict__ dist_out,
+                                                    int32_t* __restrict__ flags, int32_t round) {
+  if (round > 0 && flags[round - 1] == 0) return;  // converged in an earlier round
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  bool moved = false;
+  if (v < n) {
+    const int32_t nx = next_in[v];
+    const int32_t nn = next_in[nx];
+    if (nn == nx) {
+      next_out[v] = nx;
+      dist_out[v] = dist_in[v];
+    } else {
+      next_out[v] = nn;
+      dist_out[v] = dist_in[v] + dist_in[nx];
+      moved = true;
+    }
+  }
+  if (ballot(moved) != 0 && lane_id() == 0) flags[round] = 1;
+}
+
+int launch_jump_init(int32_t n, const int32_t* cand, int32_t* next, int32_t* dist, hipStream_t st) {
+  if (n <= 0) return 0;
+  k_jump_init<<<grid_for(n, 256), 256, 0, st>>>(n, cand, next, dist);
+  return hip_status();
+}
+
+int launch_jump_round(int32_t n, const int32_t* cand, const int32_t* next_in, const int32_t* dist_in,
+                      int32_t* next_out, int32_t* dist_out, int32_t* flags, int32_t round,
+                      hipStream_t st) {
+  (void)cand;
+  if (n <= 0) return 0;
+  k_jump_round<<<grid_for(n, 256), 256, 0, st>>>(n, next_in, dist_in, next_out, dist_out, flags, round);
+  return hip_status();
+}
+
+__global__ void __launch_bounds__(256) k_label_max(int32_t n, const int32_t* __restrict__ cand,
+                                                   const int32_t* __restrict__ next,
+                                                   int32_t* __restrict__ compmax) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  const int32_t r = next[v];
+  const int32_t key = max(r, cand[r]);
+  atomicMax(&compmax[key], int32_t(v));
+}
+
+__global__ void __launch_bounds__(256) k_label_get(int32_t n, const int32_t* __restrict__ cand,
+                                                   const int32_t* __restrict__ next,
+                                                   const int32_t* __restrict__ compmax,
+                                                   int32_t* __restrict__ label) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  const int32_t r = next[v];
+  label[v] = compmax[max(r, cand[r])];
+}
+
+int launch_label(int32_t n, const int32_t* cand, const int32_t* next, int32_t* compmax,
+                 int32_t* label, hipStream_t st) {
+  if (n <= 0) return 0;
+  fill<int32_t>(compmax, 0, n, st);
+  k_label_max<<<grid_for(n, 256), 256, 0, st>>>(n, cand, next, compmax);
+  k_label_get<<<grid_for(n, 256), 256, 0, st>>>(n, cand, next, compmax, label);
+  return hip_status();
+}
+
+__global__ void __launch_bounds__(256) k_make_keys(int32_t n, const int32_t* __restrict__ label,
+                                                   const int32_t* __restrict__ dist, int sbits,
+                                                   uint64_t* __restrict__ keys, int32_t* __restrict__ ids) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  keys[v] = (uint64_t(uint32_t(label[v])) << sbits) | uint64_t(uint32_t(dist[v]));
+  ids[v] = int32_t(v);
+}
+
+int launch_make_keys(int32_t n, const int32_t* label, const int32_t* dist, int sbits,
+                     uint64_t* keys, int32_t* ids, hipStream_t st) {
+  if (n <= 0) return 0;
+  k_make_keys<<<grid_for(n, 256), 256, 0, st>>>(n, label, dist, sbits, keys, ids);
+  return hip_status();
+}
+
+// --------------------------------------------------------------- chunking
+// constraint_group_size + coarse numbering (gkway/coarsen.hpp:361-421,
+// 120-155): in the sorted order, an item starts a chunk when its rank in its
+// group is a multiple of G; coarse ids are the running count of chunk starts.
+__global__ void __launch_bounds__(256) k_group_start(int32_t n, const uint64_t* __restrict__ skeys, int sbits,
+                                                     int32_t* __restrict__ gstart) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t lab = skeys[i] >> sbits;
+  if (i == 0 || (skeys[i - 1] >> sbits) != lab) gstart[lab] = int32_t(i);
+}
+
+struct ChunkFlag {
+  const uint64_t* skeys;
+  const int32_t* gstart;
+  int sbits;
+  int32_t G;
+  __device__ int operator()(int64_t i) const {
+    const int32_t r = int32_t(i) - gstart[skeys[i] >> sbits];
+    return (r % G) == 0 ? 1 : 0;
+  }
+};
+struct StoreExcl {
+  int32_t* out;
+  __device__ void operator()(int64_t i, int excl, int) const { out[i] = excl; }
+};
+
+__global__ void __launch_bounds__(256) k_cmap(int32_t n, const uint64_t* __restrict__ skeys,
+                                              const int32_t* __restrict__ sids, int sbits,
+                                              const int32_t* __restrict__ gstart, int32_t G,
+                                              const int32_t* __restrict__ excl,
+                                              int32_t* __restrict__ cmap, int32_t* __restrict__ cptr) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int32_t r = int32_t(i) - gstart[skeys[i] >> sbits];
+  const int f = (r % G) == 0;
+  const int32_t c = f ? excl[i] : excl[i] - 1;
+  cmap[sids[i]] = c;
+  if (f) cptr[c] = int32_t(i);
+  if (i == n - 1) cptr[c + 1] = n;
+}
+
+int launch_cluster(int32_t n, const uint64_t* skeys, const int32_t* sids, int sbits,
+                   const int32_t* vwgt, int32_t G, int32_t* gstart, int32_t* flagscan,
+                   int32_t* cmap, int32_t* cptr, int32_t* cvwgt, int32_t* nc_dev,
+                   int32_t* scratch, hipStream_t st) {
+  (void)vwgt;
+  (void)cvwgt;
+  if (n <= 0) return 0;
+  k_group_start<<<grid_for(n, 256), 256, 0, st>>>(n, skeys, sbits, gstart);
+  int rc = scan(n, ChunkFlag{skeys, gstart, sbits, G}, StoreExcl{flagscan}, nc_dev, scratch, st);
+  if (rc) return GK_ERR_HIP;
+  k_cmap<<<grid_for(n, 256), 256, 0, st>>>(n, skeys, sids, sbits, gstart, G, flagscan, cmap, cptr);
+  return hip_status();
+}
+
+// cal_coarsen_vwgt (gkway/coarsen.hpp:157-171) as an ordered sum over the
+// (at most G) members of each chunk; no atomics, destination fully written.
+__global__ void __launch_bounds__(256) k_cvwgt(int32_t nc, const int32_t* __restrict__ cptr,
+                                               const int32_t* __restrict__ order,
+                                               const int32_t* __restrict__ vw, int32_t* __restrict__ cvw) {
+  const int64_t c = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (c >= nc) return;
+  int32_t s = 0;
+  for (int32_t j = cptr[c]; j < cptr[c + 1]; ++j) s += vw[order[j]];
+  cvw[c] = s;
+}
+
+int launch_cvwgt(int32_t nc, const int32_t* cptr, const int32_t* order, const int32_t* vw,
+                 int32_t* cvw, hipStream_t st) {
+  if (nc <= 0) return 0;
+  k_cvwgt<<<grid_for(nc, 256), 256, 0, st>>>(nc, cptr, order, vw, cvw);
+  return hip_status();
+}
+
+// --------------------------------------------------------------- contraction
+// Per coarse row: upper bound of entries (sum of member degrees) and its
+// lane class; row ids appended to one list per class.
+__global__ void __launch_bounds__(256) k_row_bounds(int32_t nc, const int32_t* __restrict__ cptr,
+                                                    const int32_t* __restrict__ order,
+                                                    const int32_t* __restrict__ xadj,
+                                                    int32_t* __restrict__ ub, int32_t* __restrict__ lists,
+                                                    int32_t* __restrict__ cnt) {
+  const int64_t c = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  int cls = -1;
+  if (c < nc) {
+    int32_t s = 0;
+    for (int32_t j = cptr[c]; j < cptr[c + 1]; ++j) {
+      const int32_t v = order[j];
+      s += xadj[v + 1] - xadj[v];
+    }
+    ub[c] = s;
+    cls = s <= 16 ? 0 : (s <= 32 ? 1 : (s <= 64 ? 2 : 3));
+  }
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t m = ballot(cls == k);
+    if (m == 0) continue;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(&cnt[k], popc64(m));
+    base = __shfl(base, leader, 64);
+    if (cls == k) lists[int64_t(k) * nc + base + popc64(m & lt)] = int32_t(c);
+  }
+}
+
+struct UbLoad {
+  const int32_t* ub;
+  int32_t nc;
+  __device__ int operator()(int64_t i) const { return i < nc ? ub[i] : 0; }
+};
+
+int launch_contract_bounds(const DGraph& g, int32_t nc, const int32_t* cptr, const int32_t* order,
+                           int32_t* ub, int32_t* toff, int32_t* lists, int32_t* list_cnt,
+                           int32_t* scratch, hipStream_t st) {
+  fill<int32_t>(list_cnt, 0, 4, st);
+  if (nc <= 0) return 0;
+  k_row_bounds<<<grid_for(nc, 256), 256, 0, st>>>(nc, cptr, order, g.xadj, ub, lists, list_cnt);
+  int rc = scan(int64_t(nc) + 1, UbLoad{ub, nc}, StoreExcl{toff}, nullptr, scratch, st);
+  return rc ? GK_ERR_HIP : hip_status();
+}
+
+// Rows whose upper bound fits S lanes: one entry per lane, bitonic sort by
+// coarse neighbour id across the S lanes, segmented sum of equal neighbours,
+// self entries dropped, run tails compacted with a ballot.
+template <int S>
+__global__ void __launch_bounds__(256) k_contract_rows(const int32_t* __restrict__ rows, int32_t nrows,
+                                                       const int32_t* __restrict__ cptr,
+                                                       const int32_t* __restrict__ order,
+                                                       const int32_t* __restrict__ xadj,
+                                                       const int32_t* __restrict__ adj,
+                                                       const int32_t* __restrict__ w,
+                                                       const int32_t* __restrict__ cmap,
+                                                       const int32_t* __restrict__ toff,
+                                                       int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
+                                                       int32_t* __restrict__ len) {
+  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t ri = gt / S;
+  const int sub = static_cast<int>(gt % S);
+  if (ri >= nrows) return;
+  const int32_t c = rows[ri];
+  // Locate this lane's entry in the concatenation of the member rows.
+  int32_t j = -1;
+  {
+    int32_t pre = 0;
+    const int32_t m1 = cptr[c + 1];
+    for (int32_t m = cptr[c]; m < m1; ++m) {
+      const int32_t v = order[m];
+      const int32_t s = xadj[v];
+      const int32_t d = xadj[v + 1] - s;
+      if (sub >= pre && sub < pre + d) j = s + (sub - pre);
+      pre += d;
+    }
+  }
+  int32_t key = INT32_MAX, val = 0;
+  if (j >= 0) { key = cmap[adj[j]]; val = w[j]; }
+#pragma unroll
+  for (int k2 = 2; k2 <= S; k2 <<= 1) {
+#pragma unroll
+    for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
+      const int32_t ok = __shfl_xor(key, j2, S);
+      const int32_t ov = __shfl_xor(val, j2, S);
+      const bool up = (sub & k2) == 0;
+      const bool lower = (sub & j2) == 0;
+      const bool take = lower ? (up ? ok < key : ok > key) : (up ? ok > key : ok < key);
+      if (take) { key = ok; val = ov; }
+    }
+  }
+  int32_t x = val;
+#pragma unroll
+  for (int o = 1; o < S; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, S);
+    const int32_t ky = __shfl_up(key, o, S);
+    if (sub >= o && ky == key) x += y;
+  }
+  const int32_t kn = __shfl_down(key, 1, S);
+  const bool tail = (sub == S - 1) || kn != key;
+  const bool out = tail && key != c && key != INT32_MAX;
+  const uint64_t b = ballot(out);
+  const int base_lane = lane_id() & ~(S - 1);
+  const uint64_t seg = (S == 64) ? b : ((b >> base_lane) & ((1ull << S) - 1));
+  const int pos = popc64(seg & ((1ull << sub) - 1));
+  if (out) {
+    const int32_t o = toff[c] + pos;
+    tadj[o] = key;
+    tw[o] = x;
+  }
+  if (sub == 0) len[c] = popc64(seg);
+}
+
+int launch_contract_rows(const DGraph& g, int lanes, const int32_t* rows, int32_t nrows,
+                         const int32_t* cptr, const int32_t* order, const int32_t* cmap,
+                         const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len,
+                         hipStream_t st) {
+  if (nrows <= 0) return 0;
+  const unsigned grid = grid_for(int64_t(nrows) * lanes, 256);
+  switch (lanes) {
+    case 16: k_contract_rows<16><<<grid, 256, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    case 32: k_contract_rows<32><<<grid, 256, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    default: k_contract_rows<64><<<grid, 256, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+  }
+  return hip_status();
+}
+
+struct BigUbLoad {
+  const int32_t* rows;
+  const int32_t* ub;
+  int32_t nrows;
+  __device__ int operator()(int64_t i) const { return i < nrows ? ub[rows[i]] : 0; }
+};
+
+int launch_big_offsets(const int32_t* rows, int32_t nrows, const int32_t* ub, int32_t* boff,
+                       int32_t* total, int32_t* scratch, hipStream_t st) {
+  int rc = scan(int64_t(nrows) + 1, BigUbLoad{rows, ub, nrows}, StoreExcl{boff}, total, scratch, st);
+  return rc ? GK_ERR_HIP : hip_status();
+}
+
+// Big rows: every entry becomes the key (row rank << 32 | coarse neighbour);
+// one block per row walks the members in order.
+__global__ void __launch_bounds__(256) k_big_gather(const int32_t* __restrict__ rows,
+                                                    const int32_t* __restrict__ boff,
+                                                    const int32_t* __restrict__ cptr,
+                                                    const int32_t* __restrict__ order,
+                                                    const int32_t* __restrict__ xadj,
+                                                    const int32_t* __restrict__ adj,
+                                                    const int32_t* __restrict__ w,
+                                                    const int32_t* __restrict__ cmap,
+                                                    uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int32_t i = blockIdx.x;
+  const int32_t c = rows[i];
+  int32_t o = boff[i];
+  for (int32_t m = cptr[c]; m < cptr[c + 1]; ++m) {
+    const int32_t v = order[m];
+    const int32_t s = xadj[v], d = xadj[v + 1] - s;
+    for (int32_t t = threadIdx.x; t < d; t += 256) {
+      keys[o + t] = (uint64_t(uint32_t(i)) << 32) | uint64_t(uint32_t(cmap[adj[s + t]]));
+      vals[o + t] = w[s + t];
+    }
+    o += d;
+  }
+}
+
+int launch_contract_big_gather(const DGraph& g, const int32_t* rows, int32_t nrows,
+                               const int32_t* boff, const int32_t* cptr, const int32_t* order,
+                               const int32_t* cmap, uint64_t* keys, int32_t* vals, hipStream_t st) {
+  if (nrows <= 0) return 0;
+  k_big_gather<<<nrows, 256, 0, st>>>(rows, boff, cptr, order, g.xadj, g.adj, g.w, cmap, keys, vals);
+  return hip_status();
+}
+
+// One wave per big row walks its sorted entries in chunks of 64 with a
+// segmented sum carried across chunks.
+__global__ void __launch_bounds__(64) k_big_merge(const int32_t* __restrict__ rows,
+                                                  const int32_t* __restrict__ boff,
+                                                  const uint64_t* __restrict__ keys,
+                                                  const int32_t* __restrict__ vals,
+                                                  const int32_t* __restrict__ toff,
+                                                  int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
+                                                  int32_t* __restrict__ len) {
+  const int32_t i = blockIdx.x;
+  const int32_t c = rows[i];
+  const int32_t b0 = boff[i], b1 = boff[i + 1];
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  int32_t carry_key = -1, carry_sum = 0;
+  int32_t count = 0;
+  const int32_t obase = toff[c];
+  for (int32_t base = b0; base < b1; base += 64) {
+    const int32_t e = base + lane;
+    const bool valid = e < b1;
+    const int32_t key = valid ? int32_t(uint32_t(keys[e])) : INT32_MAX;
+    int32_t x = valid ? vals[e] : 0;
+    if (valid && key == carry_key) x += (lane == 0) ? carry_sum : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(x, o, 64);
+      const int32_t ky = __shfl_up(key, o, 64);
+      if (lane >= o && ky == key) x += y;
+    }
+    // Tail: the next element (possibly in the next chunk) has another key.
+    int32_t kn = __shfl_down(key, 1, 64);
+    if (lane == 63) kn = (e + 1 < b1) ? int32_t(uint32_t(keys[e + 1])) : INT32_MAX;
+    const bool tail = valid && (e + 1 == b1 || kn != key);
+    const bool out = tail && key != c;
+    const uint64_t bm = ballot(out);
+    if (out) {
+      const int32_t o = obase + count + popc64(bm & lt);
+      tadj[o] = key;
+      tw[o] = x;
+    }
+    count += popc64(bm);
+    // Carry the last run into the next chunk when it continues.
+    const int32_t last_key = __shfl(key, 63, 64);
+    const int32_t last_x = __shfl(x, 63, 64);
+    const bool last_tail = __shfl(static_cast<int>(tail), 63, 64) != 0;
+    carry_key = last_tail ? -1 : last_key;
+    carry_sum = last_x;
+  }
+  if (lane == 0) len[c] = count;
+}
+
+int launch_contract_big_merge(const int32_t* rows, int32_t nrows, const int32_t* boff,
+                              const uint64_t* keys, const int32_t* vals, const int32_t* toff,
+                              int32_t* tadj, int32_t* tw, int32_t* len, hipStream_t st) {
+  if (nrows <= 0) return 0;
+  k_big_merge<<<nrows, 64, 0, st>>>(rows, boff, keys, vals, toff, tadj, tw, len);
+  return hip_status();
+}
+
+struct LenLoad {
+  const int32_t* len;
+  int32_t nc;
+  __device__ int operator()(int64_t i) const { return i < nc ? len[i] : 0; }
+};
+
+int launch_len_scan(int32_t nc, const int32_t* len, int32_t* cxadj, int32_t* cnnz,
+                    int32_t* scratch, hipStream_t st) {
+  int rc = scan(int64_t(nc) + 1, LenLoad{len, nc}, StoreExcl{cxadj}, cnnz, scratch, st);
+  return rc ? GK_ERR_HIP : hip_status();
+}
+
+// Copy merged rows to the final CSR (8 lanes per row).
+__global__ void __launch_bounds__(256) k_contract_copy(int32_t nc, const int32_t* __restrict__ toff,
+                                                       const int32_t* __restrict__ len,
+                                                       const int32_t* __restrict__ cxadj,
+                                                       const int32_t* __restrict__ tadj,
+                                                       const int32_t* __restrict__ tw,
+                                                       int32_t* __restrict__ cadj, int32_t* __restrict__ cw) {
+  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t c = gt / 8;
+  const int sub = static_cast<int>(gt % 8);
+  if (c >= nc) return;
+  const int32_t s = toff[c], d = cxadj[c], l = len[c];
+  for (int32_t t = sub; t < l; t += 8) {
+    cadj[d + t] = tadj[s + t];
+    cw[d + t] = tw[s + t];
+  }
+}
+
+int launch_contract_copy(int32_t nc, const int32_t* toff, const int32_t* len, const int32_t* cxadj,
+                         const int32_t* tadj, const int32_t* tw, int32_t* cadj, int32_t* cw,
+                         hipStream_t st) {
+  if (nc <= 0) return 0;
+  k_contract_copy<<<grid_for(int64_t(nc) * 8, 256), 256, 0, st>>>(nc, toff, len, cxadj, tadj, tw, cadj, cw);
+  return hip_status();
+}
+
+}  // namespace gkd

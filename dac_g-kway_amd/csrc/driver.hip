@@ -1,0 +1,938 @@
+// Host driver of the MI355X partitioner: context, device memory pool, the
+// multilevel V-cycle, and the C ABI entry points that launch device work.
+//
+// Flow of one gk_partition_device call (the reference's graph_partitioner,
+// gkway/graph_partitioner.hpp:222-365):
+//   coarsening levels  -> match, pointer-jump grouping, radix sort by
+//                         (label, stamp, id), chunking, contraction
+//   coarsest level     -> copy to host, gk_initpart (METIS's role), copy back
+//   uncoarsening       -> per finer level: projection, gain pass, rounds of
+//                         select / balance / apply / gain until no move
+//   final              -> weighted cut and part weights of level 0
+// One stream per context; the host synchronises once per level and once per
+// refinement round to read the counts that size the next launches.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gkway.h"
+#include "host_graph.hpp"
+#include "kernels.hpp"
+#include "prim.hpp"
+
+
+using gkd::DGraph;
+
+namespace {
+
+constexpr int32_t kMaxK = 8192;
+constexpr int32_t kMaxJumpRounds = 40;
+
+// ---------------------------------------------------------------- pool
+// Caching device allocator: blocks return to a size-keyed free list and are
+// reused by later requests of up to twice smaller size, so steady-state
+// partitioning calls perform no hipMalloc.
+class Pool {
+ public:
+  ~Pool() { release_all(); }
+  int get(size_t bytes, void** out) {
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= 2 * bytes) {
+      *out = it->second;
+      free_.erase(it);
+      return GK_OK;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        gkh::set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return GK_ERR_NOMEM;
+      }
+    }
+    size_[p] = bytes;
+    *out = p;
+    return GK_OK;
+  }
+  void put(void* p) {
+    if (!p) return;
+    auto it = size_.find(p);
+    if (it != size_.end()) free_.emplace(it->second, p);
+  }
+  void trim() {
+    for (auto& kv : free_) {
+      (void)hipFree(kv.second);
+      size_.erase(kv.second);
+    }
+    free_.clear();
+  }
+  void release_all() {
+    for (auto& kv : size_) (void)hipFree(kv.first);
+    size_.clear();
+    free_.clear();
+  }
+
+ private:
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> size_;
+};
+
+template <class T>
+struct Buf {
+  Pool* pool = nullptr;
+  T* p = nullptr;
+  Buf() = default;
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  Buf(Buf&& o) noexcept : pool(o.pool), p(o.p) { o.p = nullptr; }
+  Buf& operator=(Buf&& o) noexcept {
+    if (this != &o) { release(); pool = o.pool; p = o.p; o.p = nullptr; }
+    return *this;
+  }
+  ~Buf() { release(); }
+  int alloc(Pool& pl, int64_t n) {
+    release();
+    pool = &pl;
+    void* q = nullptr;
+    int rc = pl.get(sizeof(T) * static_cast<size_t>(n > 0 ? n : 1), &q);
+    if (rc) return rc;
+    p = static_cast<T*>(q);
+    return GK_OK;
+  }
+  void release() {
+    if (pool && p) pool->put(p);
+    p = nullptr;
+  }
+};
+
+int bits_for(int64_t x) {  // bits needed to hold values 0..x
+  int b = 0;
+  while (b < 63 && (int64_t(1) << b) <= x) ++b;
+  return b < 1 ? 1 : b;
+}
+
+uint64_t fnv_host(const std::vector<int32_t>& a, uint64_t h = 0xcbf29ce484222325ull) {
+  for (int32_t x : a) { h ^= uint32_t(x); h *= 0x100000001b3ull; }
+  return h;
+}
+
+}  // namespace
+
+struct gk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Pool pool;
+  char* hpin = nullptr;  // pinned staging for small read-backs
+};
+
+namespace {
+
+// Copy small device values to host variables through pinned staging, then
+// synchronise the stream once.
+struct Fetch {
+  void* host;
+  const void* dev;
+  size_t bytes;
+};
+int fetch(gk_ctx* ctx, hipStream_t st, std::initializer_list<Fetch> items) {
+  size_t off = 0;
+  for (const Fetch& f : items) {
+    GK_HIP(hipMemcpyAsync(ctx->hpin + off, f.dev, f.bytes, hipMemcpyDeviceToHost, st));
+    off += (f.bytes + 7) & ~size_t(7);
+  }
+  GK_HIP(hipStreamSynchronize(st));
+  off = 0;
+  for (const Fetch& f : items) {
+    std::memcpy(f.host, ctx->hpin + off, f.bytes);
+    off += (f.bytes + 7) & ~size_t(7);
+  }
+  return GK_OK;
+}
+
+template <class T>
+int to_host(hipStream_t st, std::vector<T>& out, const T* dev, int64_t n) {
+  out.resize(static_cast<size_t>(n));
+  if (n <= 0) return GK_OK;
+  GK_HIP(hipMemcpyAsync(out.data(), dev, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}
+
+struct Level {
+  DGraph g;
+  Buf<int32_t> xadj, adj, w, vw;  // owned storage (coarse levels, or level-0 fills)
+  Buf<int32_t> cmap;              // fine -> coarse ids of the next level
+};
+
+// ---------------------------------------------------------------- stages
+// Grouping: pointer jumping to the root pair, then the component maximum.
+// Returns label/dist in the given buffers, the jump rounds in *rounds.
+int group_stage(gk_ctx* ctx, hipStream_t st, int32_t n, const int32_t* cand, int32_t* label,
+                Buf<int32_t>& dist_out, int32_t* rounds) {
+  Pool& P = ctx->pool;
+  Buf<int32_t> nx[2], ds[2], flags, compmax;
+  GK_TRY(nx[0].alloc(P, n));
+  GK_TRY(nx[1].alloc(P, n));
+  GK_TRY(ds[0].alloc(P, n));
+  GK_TRY(ds[1].alloc(P, n));
+  GK_TRY(flags.alloc(P, kMaxJumpRounds + 8));
+  GK_TRY(compmax.alloc(P, n));
+  gkd::fill<int32_t>(flags.p, 0, kMaxJumpRounds + 8, st);
+  GK_TRY(gkd::launch_jump_init(n, cand, nx[0].p, ds[0].p, st));
+  int32_t f = -1;
+  int32_t r = 0;
+  constexpr int kBatch = 4;
+  while (f < 0) {
+    if (r >= kMaxJumpRounds) {
+      gkh::set_error("candidate graph did not reduce to pairs (asymmetric input?)");
+      return GK_ERR_GRAPH;
+    }
+    for (int i = 0; i < kBatch && r + i < kMaxJumpRounds; ++i) {
+      const int rr = r + i;
+      GK_TRY(gkd::launch_jump_round(n, cand, nx[rr & 1].p, ds[rr & 1].p, nx[(rr + 1) & 1].p,
+                                    ds[(rr + 1) & 1].p, flags.p, rr, st));
+    }
+    int32_t hf[kBatch];
+    const int cnt = std::min(kBatch, kMaxJumpRounds - r);
+    GK_TRY(fetch(ctx, st, {{hf, flags.p + r, sizeof(int32_t) * cnt}}));
+    for (int i = 0; i < cnt; ++i)
+      if (hf[i] == 0) { f = r + i; break; }
+    r += cnt;
+  }
+  // Round f moved nothing: buffers f&1 and (f+1)&1 agree.
+  int32_t* next = nx[f & 1].p;
+  GK_TRY(gkd::launch_label(n, cand, next, compmax.p, label, st));
+  dist_out = std::move(ds[f & 1]);
+  *rounds = f;
+  return GK_OK;
+}
+
+// Sort vertices by (label, stamp, id) and chunk them; fills order (sorted
+// ids), cmap, cptr (nc+1) and *nc.
+int cluster_stage(gk_ctx* ctx, hipStream_t st, int32_t n, const int32_t* label, const int32_t* dist,
+                  int32_t max_dist_bits, int32_t G, Buf<int32_t>& order, int32_t* cmap,
+                  Buf<int32_t>& cptr, int32_t* nc) {
+  Pool& P = ctx->pool;
+  const int sbits = max_dist_bits;
+  const int lbits = bits_for(n > 0 ? n - 1 : 0);
+  Buf<uint64_t> keys, keys2;
+  Buf<int32_t> ids, ids2, scratch, gstart, flagscan, ncd;
+  GK_TRY(keys.alloc(P, n));
+  GK_TRY(keys2.alloc(P, n));
+  GK_TRY(ids.alloc(P, n));
+  GK_TRY(ids2.alloc(P, n));
+  GK_TRY(scratch.alloc(P, gkd::sort_scratch_ints(n) + gkd::scan_scratch_ints(n)));
+  GK_TRY(gstart.alloc(P, n));
+  GK_TRY(flagscan.alloc(P, n));
+  GK_TRY(ncd.alloc(P, 1));
+  GK_TRY(cptr.alloc(P, int64_t(n) + 1));
+  GK_TRY(gkd::launch_make_keys(n, label, dist, sbits, keys.p, ids.p, st));
+  bool alt = false;
+  if (gkd::radix_sort_pairs<uint64_t>(keys.p, ids.p, keys2.p, ids2.p, n, 0, sbits + lbits, scratch.p, st, &alt))
+    GK_HIP(hipGetLastError());
+  uint64_t* sk = alt ? keys2.p : keys.p;
+  Buf<int32_t>& sid = alt ? ids2 : ids;
+  GK_TRY(gkd::launch_cluster(n, sk, sid.p, sbits, nullptr, G, gstart.p, flagscan.p, cmap, cptr.p,
+                             nullptr, ncd.p, scratch.p, st));
+  GK_TRY(fetch(ctx, st, {{nc, ncd.p, sizeof(int32_t)}}));
+  order = std::move(sid);
+  return GK_OK;
+}
+
+// Contraction of g under (cmap, cptr, order) into the coarse level c.
+int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, const int32_t* cmap,
+                   const int32_t* cptr, const int32_t* order, Level& c) {
+  Pool& P = ctx->pool;
+  Buf<int32_t> ub, toff, lists, lcnt, scratch, tadj, tw, len, cnnzd;
+  GK_TRY(ub.alloc(P, nc));
+  GK_TRY(toff.alloc(P, int64_t(nc) + 1));
+  GK_TRY(lists.alloc(P, int64_t(4) * nc));
+  GK_TRY(lcnt.alloc(P, 4));
+  GK_TRY(scratch.alloc(P, gkd::scan_scratch_ints(int64_t(nc) + 1) + 64));
+  GK_TRY(gkd::launch_contract_bounds(g, nc, cptr, order, ub.p, toff.p, lists.p, lcnt.p, scratch.p, st));
+  int32_t hc[4];
+  GK_TRY(fetch(ctx, st, {{hc, lcnt.p, sizeof hc}}));
+  GK_TRY(tadj.alloc(P, g.nnz));
+  GK_TRY(tw.alloc(P, g.nnz));
+  GK_TRY(len.alloc(P, nc));
+  const int lanes[3] = {16, 32, 64};
+  for (int cls = 0; cls < 3; ++cls)
+    GK_TRY(gkd::launch_contract_rows(g, lanes[cls], lists.p + int64_t(cls) * nc, hc[cls], cptr, order,
+                                     cmap, toff.p, tadj.p, tw.p, len.p, st));
+  if (hc[3] > 0) {
+    const int32_t nbig = hc[3];
+    const int32_t* rows = lists.p + int64_t(3) * nc;
+    Buf<int32_t> boff, btot, bscr;
+    GK_TRY(boff.alloc(P, int64_t(nbig) + 1));
+    GK_TRY(btot.alloc(P, 1));
+    GK_TRY(bscr.alloc(P, gkd::scan_scratch_ints(int64_t(nbig) + 1) + 64));
+    GK_TRY(gkd::launch_big_offsets(rows, nbig, ub.p, boff.p, btot.p, bscr.p, st));
+    int32_t eb = 0;
+    GK_TRY(fetch(ctx, st, {{&eb, btot.p, sizeof eb}}));
+    Buf<uint64_t> k1, k2;
+    Buf<int32_t> v1, v2, sscr;
+    GK_TRY(k1.alloc(P, eb));
+    GK_TRY(k2.alloc(P, eb));
+    GK_TRY(v1.alloc(P, eb));
+    GK_TRY(v2.alloc(P, eb));
+    GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(eb)));
+    GK_TRY(gkd::launch_contract_big_gather(g, rows, nbig, boff.p, cptr, order, cmap, k1.p, v1.p, st));
+    bool alt = false;
+    if (gkd::radix_sort_pairs<uint64_t>(k1.p, v1.p, k2.p, v2.p, eb, 0, 32 + bits_for(nbig - 1), sscr.p, st, &alt))
+      GK_HIP(hipGetLastError());
+    GK_TRY(gkd::launch_contract_big_merge(rows, nbig, boff.p, alt ? k2.p : k1.p, alt ? v2.p : v1.p,
+                                          toff.p, tadj.p, tw.p, len.p, st));
+  }
+  GK_TRY(cnnzd.alloc(P, 1));
+  GK_TRY(c.xadj.alloc(P, int64_t(nc) + 1));
+  GK_TRY(gkd::launch_len_scan(nc, len.p, c.xadj.p, cnnzd.p, scratch.p, st));
+  int32_t cnnz = 0;
+  GK_TRY(fetch(ctx, st, {{&cnnz, cnnzd.p, sizeof cnnz}}));
+  GK_TRY(c.adj.alloc(P, cnnz));
+  GK_TRY(c.w.alloc(P, cnnz));
+  GK_TRY(gkd::launch_contract_copy(nc, toff.p, len.p, c.xadj.p, tadj.p, tw.p, c.adj.p, c.w.p, st));
+  c.g.n = nc;
+  c.g.nnz = cnnz;
+  c.g.xadj = c.xadj.p;
+  c.g.adj = c.adj.p;
+  c.g.w = c.w.p;
+  return GK_OK;
+}
+
+int gain_lanes(const DGraph& g) {
+  const double avg = g.n > 0 ? double(g.nnz) / g.n : 0.0;
+  int s = 4;
+  while (s < 64 && s < 2.0 * avg) s <<= 1;
+  return s;
+}
+
+// ---------------------------------------------------------------- V-cycle
+int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params& prm,
+               int32_t* dpart_out, gk_stats* stats) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  Pool& P = ctx->pool;
+  const int32_t k = prm.k;
+  const bool trace = (prm.flags & GK_FLAG_TRACE) != 0;
+  gk_stats S;
+  std::memset(&S, 0, sizeof S);
+
+  std::vector<Level> L;
+  L.reserve(GK_MAX_LEVELS + 1);
+  L.emplace_back();
+  L[0].g = g0in;
+  const int32_t n0 = g0in.n;
+  if (!g0in.w && g0in.nnz > 0) {
+    GK_TRY(L[0].w.alloc(P, g0in.nnz));
+    gkd::fill<int32_t>(L[0].w.p, 1, g0in.nnz, st);
+    L[0].g.w = L[0].w.p;
+  }
+  if (!g0in.vw) {
+    GK_TRY(L[0].vw.alloc(P, n0));
+    gkd::fill<int32_t>(L[0].vw.p, 1, n0, st);
+    L[0].g.vw = L[0].vw.p;
+  }
+  int64_t T = 0;
+  {
+    Buf<unsigned long long> tw;
+    GK_TRY(tw.alloc(P, 1));
+    GK_TRY(gkd::launch_sum(L[0].g.vw, n0, tw.p, st));
+    unsigned long long ht = 0;
+    GK_TRY(fetch(ctx, st, {{&ht, tw.p, sizeof ht}}));
+    T = static_cast<int64_t>(ht);
+  }
+  if (T >= (int64_t(1) << 31)) { gkh::set_error("total vertex weight >= 2^31"); return GK_ERR_LIMIT; }
+  const int64_t cap = gk_part_cap(T, k, prm.ubfactor);
+  const int32_t floor_n = gk_coarsen_floor(n0, k, prm.coarsen_policy, prm.coarsen_to);
+  const int32_t maxlv = prm.max_levels > 0 ? std::min(prm.max_levels, GK_MAX_LEVELS) : GK_MAX_LEVELS;
+  S.total_vwgt = T;
+  S.cap = cap;
+
+  // ---------------------------------------------------------- coarsening
+  while (static_cast<int32_t>(L.size()) < maxlv) {
+    Level& f = L.back();
+    const int32_t n = f.g.n;
+    if (!(n > floor_n)) break;
+    gk_level_stats& ls = S.levels[L.size() - 1];
+    Buf<int32_t> cand, label, dist, order, cptr, cmap;
+    GK_TRY(cand.alloc(P, n));
+    GK_TRY(label.alloc(P, n));
+    GK_TRY(cmap.alloc(P, n));
+    GK_TRY(gkd::launch_match(f.g, gkd::lanes_for_degree(n ? double(f.g.nnz) / n : 0.0), cand.p, st));
+    int32_t rounds = 0;
+    GK_TRY(group_stage(ctx, st, n, cand.p, label.p, dist, &rounds));
+    int32_t nc = 0;
+    GK_TRY(cluster_stage(ctx, st, n, label.p, dist.p, rounds + 1, prm.max_group, order, cmap.p, cptr, &nc));
+    if (nc == n || nc < k + 1) break;
+    if (prm.coarsen_policy != GK_COARSEN_REF && nc < floor_n) break;
+    Level c;
+    GK_TRY(c.vw.alloc(P, nc));
+    GK_TRY(gkd::launch_cvwgt(nc, cptr.p, order.p, f.g.vw, c.vw.p, st));
+    c.g.vw = c.vw.p;
+    GK_TRY(contract_stage(ctx, st, f.g, nc, cmap.p, cptr.p, order.p, c));
+    ls.sweeps = rounds;
+    if (trace) {
+      std::vector<int32_t> h;
+      GK_TRY(to_host(st, h, cand.p, n));
+      ls.h_cand = fnv_host(h);
+      GK_TRY(to_host(st, h, cmap.p, n));
+      ls.h_cmap = fnv_host(h);
+      std::vector<int32_t> a, b, d, e;
+      GK_TRY(to_host(st, a, c.g.xadj, int64_t(nc) + 1));
+      GK_TRY(to_host(st, b, c.g.adj, c.g.nnz));
+      GK_TRY(to_host(st, d, c.g.w, c.g.nnz));
+      GK_TRY(to_host(st, e, c.g.vw, nc));
+      ls.h_coarse = fnv_host(e, fnv_host(d, fnv_host(b, fnv_host(a))));
+    }
+    f.cmap = std::move(cmap);
+    L.push_back(std::move(c));
+  }
+  const int32_t nl = static_cast<int32_t>(L.size());
+  GK_HIP(hipStreamSynchronize(st));
+  const auto t1 = clk::now();
+
+  // ---------------------------------------------------------- initial partition
+  Level& top = L.back();
+  const int32_t nt = top.g.n;
+  std::vector<int32_t> hx, ha, hw, hv, hp(nt), hb(nt, 0);
+  GK_TRY(to_host(st, hx, top.g.xadj, int64_t(nt) + 1));
+  GK_TRY(to_host(st, ha, top.g.adj, top.g.nnz));
+  GK_TRY(to_host(st, hw, top.g.w, top.g.nnz));
+  GK_TRY(to_host(st, hv, top.g.vw, nt));
+  gk_csr hc;
+  hc.n = nt;
+  hc.nnz = top.g.nnz;
+  hc.xadj = hx.data();
+  hc.adjncy = ha.data();
+  hc.adjwgt = hw.data();
+  hc.vwgt = hv.data();
+  int64_t cut0 = 0;
+  GK_TRY(gk_initpart(&hc, k, cap, prm.seed, prm.initpart_threads, hp.data(), &cut0));
+  std::vector<int32_t> hpw(k, 0);
+  for (int32_t v = 0; v < nt; ++v) {
+    hpw[hp[v]] += hv[v];
+    for (int32_t j = hx[v]; j < hx[v + 1]; ++j)
+      if (hp[ha[j]] != hp[v]) { hb[v] = 1; break; }
+  }
+  S.init_cut = cut0;
+  Buf<int32_t> partA, partB, bndA, bndB, pw;
+  GK_TRY(partA.alloc(P, n0));
+  GK_TRY(partB.alloc(P, n0));
+  GK_TRY(bndA.alloc(P, n0));
+  GK_TRY(bndB.alloc(P, n0));
+  GK_TRY(pw.alloc(P, k));
+  Buf<long long> dcut;
+  GK_TRY(dcut.alloc(P, 1));
+  GK_HIP(hipMemcpyAsync(partA.p, hp.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, st));
+  GK_HIP(hipMemcpyAsync(bndA.p, hb.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, st));
+  GK_HIP(hipMemcpyAsync(pw.p, hpw.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+  GK_HIP(hipMemcpyAsync(dcut.p, &cut0, sizeof(long long), hipMemcpyHostToDevice, st));
+  GK_HIP(hipStreamSynchronize(st));
+  S.levels[nl - 1].cut_after = cut0;
+  if (trace) S.levels[nl - 1].h_part = fnv_host(hp);
+  const auto t2 = clk::now();
+
+  // ---------------------------------------------------------- uncoarsening
+  int32_t* cur_part = partA.p;
+  int32_t* cur_bnd = bndA.p;
+  int32_t* oth_part = partB.p;
+  int32_t* oth_bnd = bndB.p;
+  if (nl > 1) {
+    const int32_t M = prm.max_moves;
+    Buf<int32_t> gain, dest, upd, heavy, hcnt, moves, ctl, sscr;
+    Buf<uint64_t> keys, keys2;
+    GK_TRY(gain.alloc(P, n0));
+    GK_TRY(dest.alloc(P, n0));
+    GK_TRY(upd.alloc(P, n0));
+    GK_TRY(heavy.alloc(P, n0));
+    GK_TRY(hcnt.alloc(P, 1));
+    GK_TRY(moves.alloc(P, M));
+    GK_TRY(ctl.alloc(P, 4));
+    GK_TRY(keys.alloc(P, n0));
+    GK_TRY(keys2.alloc(P, n0));
+    Buf<int32_t> dummy_vals;
+    GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(n0)));
+    gkd::fill<int32_t>(ctl.p, -1, 4, st);
+    for (int32_t l = nl - 2; l >= 0; --l) {
+      Level& f = L[l];
+      const DGraph& g = f.g;
+      gk_level_stats& ls = S.levels[l];
+      GK_TRY(gkd::launch_project(g.n, f.cmap.p, cur_part, cur_bnd, oth_part, oth_bnd, gain.p, dest.p,
+                                 upd.p, st));
+      std::swap(cur_part, oth_part);
+      std::swap(cur_bnd, oth_bnd);
+      const int lanes = gain_lanes(g);
+      GK_TRY(gkd::launch_heavy_list(g, lanes, heavy.p, hcnt.p, st));
+      int32_t nheavy = 0;
+      GK_TRY(fetch(ctx, st, {{&nheavy, hcnt.p, sizeof nheavy}}));
+      GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, st));
+      const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
+      int64_t moved = 0;
+      int32_t rounds = 0;
+      bool pending = false;  // a round whose pos has not been read yet
+      while (true) {
+        GK_TRY(gkd::launch_select(g, cur_bnd, gain.p, dest.p, pw.p, cap, idbits, upd.p, keys.p, ctl.p, st));
+        int32_t hc3[3];
+        GK_TRY(fetch(ctx, st, {{hc3, ctl.p, sizeof hc3}}));
+        if (pending) {
+          if (hc3[2] < 0) { gkh::set_error("balanced prefix is empty"); return GK_ERR_INTERNAL; }
+          moved += hc3[2] + 1;
+          pending = false;
+        }
+        const int32_t count = hc3[0];
+        if (count == 0) break;
+        ++rounds;
+        bool presorted = false;
+        const uint64_t* kk = keys.p;
+        if (count > 1024) {
+          bool alt = false;
+          if (gkd::radix_sort_pairs<uint64_t>(keys.p, nullptr, keys2.p, nullptr, count, 0,
+                                              idbits + bits_for(hc3[1]), sscr.p, st, &alt))
+            GK_HIP(hipGetLastError());
+          kk = alt ? keys2.p : keys.p;
+          presorted = true;
+        }
+        GK_TRY(gkd::launch_balance(g, k, kk, count, M, presorted, idbits, cur_part, dest.p, gain.p, pw.p,
+                                   cap, moves.p, ctl.p, dcut.p, st));
+        GK_TRY(gkd::launch_apply(g, M, moves.p, ctl.p, dest.p, cur_part, upd.p, st));
+        GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, st));
+        pending = true;
+      }
+      long long hcut = 0;
+      GK_TRY(fetch(ctx, st, {{&hcut, dcut.p, sizeof hcut}}));
+      ls.rounds = rounds;
+      ls.moves = moved;
+      ls.cut_after = hcut;
+      if (trace) {
+        std::vector<int32_t> h;
+        GK_TRY(to_host(st, h, cur_part, g.n));
+        ls.h_part = fnv_host(h);
+      }
+    }
+  }
+  GK_HIP(hipStreamSynchronize(st));
+  const auto t3 = clk::now();
+
+  // ---------------------------------------------------------- final metrics
+  {
+    Buf<unsigned long long> cut, pw64;
+    GK_TRY(cut.alloc(P, 1));
+    GK_TRY(pw64.alloc(P, k));
+    GK_TRY(gkd::launch_cut(L[0].g, k, cur_part, cut.p, pw64.p, st));
+    std::vector<unsigned long long> hpw64;
+    unsigned long long hcut = 0;
+    long long tracked = 0;
+    GK_TRY(fetch(ctx, st, {{&hcut, cut.p, sizeof hcut}, {&tracked, dcut.p, sizeof tracked}}));
+    GK_TRY(to_host(st, hpw64, pw64.p, k));
+    S.cut = static_cast<int64_t>(hcut);
+    S.cut_tracked = tracked;
+    unsigned long long mx = 0;
+    for (auto x : hpw64) mx = std::max(mx, x);
+    S.max_part_wgt = static_cast<int64_t>(mx);
+    S.imbalance = T > 0 ? double(mx) * k / double(T) : 1.0;
+  }
+  if (dpart_out != cur_part)
+    GK_HIP(hipMemcpyAsync(dpart_out, cur_part, sizeof(int32_t) * n0, hipMemcpyDeviceToDevice, st));
+  GK_HIP(hipStreamSynchronize(st));
+  S.nlevels = nl;
+  S.coarsest_n = nt;
+  for (int32_t l = 0; l < nl; ++l) {
+    S.levels[l].n = L[l].g.n;
+    S.levels[l].nnz = L[l].g.nnz;
+  }
+  auto ms = [](clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  S.t_coarsen_ms = ms(t0, t1);
+  S.t_init_ms = ms(t1, t2);
+  S.t_refine_ms = ms(t2, t3);
+  S.t_total_ms = ms(t0, clk::now());
+  if (stats) *stats = S;
+  if (S.cut != S.cut_tracked) {
+    gkh::set_error("tracked cut " + std::to_string(S.cut_tracked) + " != recomputed cut " +
+                   std::to_string(S.cut));
+    return GK_ERR_INTERNAL;
+  }
+  return GK_OK;
+}
+
+int check_params(const gk_params* p) {
+  if (!p) { gkh::set_error("null params"); return GK_ERR_ARG; }
+  if (p->k < 1 || p->k > kMaxK) { gkh::set_error("k must be in [1, 8192]"); return GK_ERR_ARG; }
+  if (p->ubfactor < 0) { gkh::set_error("ubfactor < 0"); return GK_ERR_ARG; }
+  if (p->max_group < 1 || p->max_group > 64) { gkh::set_error("max_group must be in [1, 64]"); return GK_ERR_ARG; }
+  if (p->max_moves < 1 || p->max_moves > 1024) { gkh::set_error("max_moves must be in [1, 1024]"); return GK_ERR_ARG; }
+  return GK_OK;
+}
+
+struct DeviceGuard {
+  explicit DeviceGuard(int d) { (void)hipSetDevice(d); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int gk_ctx_create(int device, gk_ctx** out) {
+  if (!out) { gkh::set_error("null out"); return GK_ERR_ARG; }
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    gkh::set_error("no HIP device");
+    return GK_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) { gkh::set_error("device out of range"); return GK_ERR_ARG; }
+  GK_HIP(hipSetDevice(device));
+  gk_ctx* c = new gk_ctx;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    gkh::set_error("hipStreamCreate failed");
+    return GK_ERR_HIP;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->hpin), 4096, hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    gkh::set_error("hipHostMalloc failed");
+    return GK_ERR_NOMEM;
+  }
+  *out = c;
+  return GK_OK;
+}
+
+int gk_ctx_destroy(gk_ctx* c) {
+  if (!c) return GK_OK;
+  DeviceGuard dg(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->pool.release_all();
+  (void)hipHostFree(c->hpin);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GK_OK;
+}
+
+int gk_partition_device(gk_ctx* ctx, const gk_csr* dg, const gk_params* prm, int32_t* dpart,
+                        gk_stats* stats, void* stream) {
+  if (!ctx || !dg || !dpart) { gkh::set_error("null argument"); return GK_ERR_ARG; }
+  GK_TRY(check_params(prm));
+  if (dg->n < 0 || dg->nnz < 0 || !dg->xadj || (dg->nnz > 0 && !dg->adjncy)) {
+    gkh::set_error("bad graph description");
+    return GK_ERR_ARG;
+  }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  if (dg->n == 0) {
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    return GK_OK;
+  }
+  DGraph g;
+  g.n = dg->n;
+  g.nnz = dg->nnz;
+  g.xadj = dg->xadj;
+  g.adj = dg->adjncy;
+  g.w = dg->adjwgt;
+  g.vw = dg->vwgt;
+  return run_vcycle(ctx, st, g, *prm, dpart, stats);
+}
+
+int gk_partition(gk_ctx* ctx, const gk_csr* hg, const gk_params* prm, int32_t* part, gk_stats* stats) {
+  if (!ctx || !hg || !part) { gkh::set_error("null argument"); return GK_ERR_ARG; }
+  GK_TRY(check_params(prm));
+  GK_TRY(gk_graph_check(hg, (prm->flags & GK_FLAG_CHECK) != 0));
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = ctx->stream;
+  const int32_t n = hg->n, nnz = hg->nnz;
+  if (n == 0) {
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    return GK_OK;
+  }
+  Buf<int32_t> x, a, w, v, p;
+  GK_TRY(x.alloc(ctx->pool, int64_t(n) + 1));
+  GK_TRY(a.alloc(ctx->pool, nnz));
+  GK_TRY(p.alloc(ctx->pool, n));
+  GK_HIP(hipMemcpyAsync(x.p, hg->xadj, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+  if (nnz) GK_HIP(hipMemcpyAsync(a.p, hg->adjncy, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+  gk_csr d = *hg;
+  d.xadj = x.p;
+  d.adjncy = a.p;
+  if (hg->adjwgt) {
+    GK_TRY(w.alloc(ctx->pool, nnz));
+    if (nnz) GK_HIP(hipMemcpyAsync(w.p, hg->adjwgt, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st));
+    d.adjwgt = w.p;
+  }
+  if (hg->vwgt) {
+    GK_TRY(v.alloc(ctx->pool, n));
+    GK_HIP(hipMemcpyAsync(v.p, hg->vwgt, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    d.vwgt = v.p;
+  }
+  GK_TRY(gk_partition_device(ctx, &d, prm, p.p, stats, st));
+  GK_HIP(hipMemcpyAsync(part, p.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}
+
+int gk_partition_file(gk_ctx* ctx, const char* graph_file, const char* out_file, int32_t k,
+                      gk_stats* stats) {
+  if (!ctx || !graph_file || !out_file) { gkh::set_error("null argument"); return GK_ERR_ARG; }
+  gk_csr g;
+  GK_TRY(gk_graph_read(graph_file, &g));
+  gk_params p;
+  gk_params_default(&p, k);
+  std::vector<int32_t> part(g.n > 0 ? g.n : 1);
+  int rc = gk_partition(ctx, &g, &p, part.data(), stats);
+  if (rc == GK_OK) rc = gk_write_partition((std::string(out_file) + ".out").c_str(), part.data(), g.n);
+  gk_graph_free(&g);
+  return rc;
+}
+
+// ---------------------------------------------------------------- stage hooks
+static DGraph dview(const gk_csr* g) {
+  DGraph d;
+  d.n = g->n;
+  d.nnz = g->nnz;
+  d.xadj = g->xadj;
+  d.adj = g->adjncy;
+  d.w = g->adjwgt;
+  d.vw = g->vwgt;
+  return d;
+}
+
+static hipStream_t pick(gk_ctx* ctx, void* s) { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
+
+int gk_k_match(gk_ctx* ctx, const gk_csr* dg, int32_t* cand, void* stream) {
+  if (!ctx || !dg || !cand || !dg->adjwgt) { gkh::set_error("null argument (adjwgt required)"); return GK_ERR_ARG; }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  DGraph g = dview(dg);
+  GK_TRY(gkd::launch_match(g, gkd::lanes_for_degree(g.n ? double(g.nnz) / g.n : 0.0), cand, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}
+
+int gk_k_group(gk_ctx* ctx, int32_t n, const int32_t* cand, int32_t* label, int32_t* stamp,
+               int32_t* sweeps, void* stream) {
+  if (!ctx || !cand || !label || !stamp) { gkh::set_error("null argument"); return GK_ERR_ARG; }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  if (n <= 0) { if (sweeps) *sweeps = 0; return GK_OK; }
+  Buf<int32_t> dist;
+  int32_t rounds = 0;
+  GK_TRY(group_stage(ctx, st, n, cand, label, dist, &rounds));
+  GK_HIP(hipMemcpyAsync(stamp, dist.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+  GK_HIP(hipStreamSynchronize(st));
+  if (sweeps) *sweeps = rounds;
+  return GK_OK;
+}
+
+int gk_k_cluster(gk_ctx* ctx, int32_t n, const int32_t* label, const int32_t* stamp, const int32_t* vwgt,
+                 int32_t max_group, int32_t* order, int32_t* cmap, int32_t* cvwgt, int32_t* nc,
+                 void* stream) {
+  if (!ctx || !label || !stamp || !vwgt || !order || !cmap || !cvwgt || !nc || max_group < 1) {
+    gkh::set_error("bad argument");
+    return GK_ERR_ARG;
+  }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  if (n <= 0) { *nc = 0; return GK_OK; }
+  // The stamp bit width is taken from the data (max stamp).
+  std::vector<int32_t> hs;
+  GK_TRY(to_host(st, hs, stamp, n));
+  int32_t mx = 0;
+  for (int32_t x : hs) mx = std::max(mx, x);
+  Buf<int32_t> ord, cptr;
+  GK_TRY(cluster_stage(ctx, st, n, label, stamp, bits_for(mx), max_group, ord, cmap, cptr, nc));
+  GK_TRY(gkd::launch_cvwgt(*nc, cptr.p, ord.p, vwgt, cvwgt, st));
+  GK_HIP(hipMemcpyAsync(order, ord.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}
+
+int gk_k_contract(gk_ctx* ctx, const gk_csr* dg, const int32_t* cmap, int32_t nc, int32_t* cxadj,
+                  int32_t* cadjncy, int32_t* cadjwgt, int32_t* cnnz, void* stream) {
+  if (!ctx || !dg || !cmap || !cxadj || !cadjncy || !cadjwgt || !cnnz || !dg->adjwgt) {
+    gkh::set_error("bad argument");
+    return GK_ERR_ARG;
+  }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  Pool& P = ctx->pool;
+  const int32_t n = dg->n;
+  // Members of each coarse vertex from a stable sort of (cmap[v], v).
+  Buf<int32_t> zero, order, cptr;
+  Buf<int32_t> cm_as_label;
+  GK_TRY(zero.alloc(P, n));
+  gkd::fill<int32_t>(zero.p, 0, n, st);
+  int32_t nc2 = 0;
+  Buf<int32_t> scratch_cmap;
+  GK_TRY(scratch_cmap.alloc(P, n));
+  GK_TRY(cluster_stage(ctx, st, n, cmap, zero.p, 1, INT32_MAX, order, scratch_cmap.p, cptr, &nc2));
+  // With G = INT32_MAX every label is one chunk, so chunks are the coarse
+  // vertices in id order exactly when cmap is onto [0, nc).
+  if (nc2 != nc) { gkh::set_error("cmap is not onto [0, nc)"); return GK_ERR_ARG; }
+  Level c;
+  DGraph g = dview(dg);
+  GK_TRY(contract_stage(ctx, st, g, nc, cmap, cptr.p, order.p, c));
+  GK_HIP(hipMemcpyAsync(cxadj, c.g.xadj, sizeof(int32_t) * (nc + 1), hipMemcpyDeviceToDevice, st));
+  if (c.g.nnz) {
+    GK_HIP(hipMemcpyAsync(cadjncy, c.g.adj, sizeof(int32_t) * c.g.nnz, hipMemcpyDeviceToDevice, st));
+    GK_HIP(hipMemcpyAsync(cadjwgt, c.g.w, sizeof(int32_t) * c.g.nnz, hipMemcpyDeviceToDevice, st));
+  }
+  GK_HIP(hipStreamSynchronize(st));
+  *cnnz = c.g.nnz;
+  return GK_OK;
+}
+
+int gk_k_project(gk_ctx* ctx, int32_t n, const int32_t* cmap, const int32_t* cpart, const int32_t* cbnd,
+                 int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* upd, void* stream) {
+  if (!ctx) return GK_ERR_ARG;
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  GK_TRY(gkd::launch_project(n, cmap, cpart, cbnd, part, bnd, gain, dest, upd, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}
+
+int gk_k_gain(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, const int32_t* upd,
+              int32_t* bnd, int32_t* gain, int32_t* dest, void* stream) {
+  if (!ctx || !dg || !dg->adjwgt || k < 1 || k > kMaxK) { gkh::set_error("bad argument"); return GK_ERR_ARG; }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  DGraph g = dview(dg);
+  const int lanes = gain_lanes(g);
+  Buf<int32_t> heavy, hcnt;
+  GK_TRY(heavy.alloc(ctx->pool, g.n));
+  GK_TRY(hcnt.alloc(ctx->pool, 1));
+  GK_TRY(gkd::launch_heavy_list(g, lanes, heavy.p, hcnt.p, st));
+  int32_t nheavy = 0;
+  GK_TRY(fetch(ctx, st, {{&nheavy, hcnt.p, sizeof nheavy}}));
+  GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, part, upd, bnd, gain, dest, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}
+
+int gk_k_select(gk_ctx* ctx, const gk_csr* dg, const int32_t* part, const int32_t* bnd, const int32_t* gain,
+                const int32_t* dest, const int32_t* pw, int64_t cap, int32_t max_moves, int32_t* upd,
+                int32_t* moves, int32_t* nsel, void* stream) {
+  (void)part;
+  if (!ctx || !dg || !dg->vwgt || !moves || !nsel || max_moves < 1) { gkh::set_error("bad argument"); return GK_ERR_ARG; }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  DGraph g = dview(dg);
+  Pool& P = ctx->pool;
+  Buf<uint64_t> keys, keys2;
+  Buf<int32_t> ctl, sscr;
+  GK_TRY(keys.alloc(P, g.n));
+  GK_TRY(keys2.alloc(P, g.n));
+  GK_TRY(ctl.alloc(P, 4));
+  GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(g.n)));
+  const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
+  GK_TRY(gkd::launch_select(g, bnd, gain, dest, pw, cap, idbits, upd, keys.p, ctl.p, st));
+  int32_t hc[2];
+  GK_TRY(fetch(ctx, st, {{hc, ctl.p, sizeof hc}}));
+  *nsel = hc[0];
+  if (hc[0] == 0) return GK_OK;
+  bool alt = false;
+  if (gkd::radix_sort_pairs<uint64_t>(keys.p, nullptr, keys2.p, nullptr, hc[0], 0, idbits + bits_for(hc[1]),
+                                      sscr.p, st, &alt))
+    GK_HIP(hipGetLastError());
+  std::vector<uint64_t> hk;
+  GK_TRY(to_host(st, hk, alt ? keys2.p : keys.p, hc[0]));
+  const int32_t B = std::min(hc[0], max_moves);
+  std::vector<int32_t> mv(B);
+  const uint64_t mask = (1ull << idbits) - 1;
+  for (int32_t i = 0; i < B; ++i) mv[i] = int32_t(hk[i] & mask);
+  GK_HIP(hipMemcpyAsync(moves, mv.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, st));
+  GK_HIP(hipStreamSynchronize(st));
+  return GK_OK;
+}
+
+int gk_k_balance(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* moves, int32_t nmoves,
+                 const int32_t* part, const int32_t* dest, int32_t* pw, int64_t cap, int32_t* pos,
+                 void* stream) {
+  if (!ctx || !dg || !dg->vwgt || !pos || k < 1 || k > kMaxK || nmoves > 1024) {
+    gkh::set_error("bad argument");
+    return GK_ERR_ARG;
+  }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  if (nmoves <= 0) { *pos = -1; return GK_OK; }
+  DGraph g = dview(dg);
+  Pool& P = ctx->pool;
+  // Keys in the given order: the vertex id with idbits = 32 (presorted).
+  std::vector<int32_t> hm;
+  GK_TRY(to_host(st, hm, moves, nmoves));
+  std::vector<uint64_t> hk(nmoves);
+  for (int32_t i = 0; i < nmoves; ++i) hk[i] = uint64_t(uint32_t(hm[i]));
+  Buf<uint64_t> keys;
+  Buf<int32_t> out, ctl, gz;
+  Buf<long long> cut;
+  GK_TRY(keys.alloc(P, nmoves));
+  GK_TRY(out.alloc(P, nmoves));
+  GK_TRY(ctl.alloc(P, 4));
+  GK_TRY(gz.alloc(P, g.n));
+  GK_TRY(cut.alloc(P, 1));
+  gkd::fill<int32_t>(gz.p, 0, g.n, st);
+  gkd::fill<long long>(cut.p, 0, 1, st);
+  GK_HIP(hipMemcpyAsync(keys.p, hk.data(), sizeof(uint64_t) * nmoves, hipMemcpyHostToDevice, st));
+  GK_TRY(gkd::launch_balance(g, k, keys.p, nmoves, nmoves, true, 32, part, dest, gz.p, pw, cap, out.p,
+                             ctl.p, cut.p, st));
+  int32_t hc[3];
+  GK_TRY(fetch(ctx, st, {{hc, ctl.p, sizeof hc}}));
+  *pos = hc[2];
+  return GK_OK;
+}
+
+int gk_k_apply(gk_ctx* ctx, const gk_csr* dg, const int32_t* moves, int32_t pos, const int32_t* gain,
+               const int32_t* dest, int32_t* part, int32_t* upd, int64_t* gain_sum, void* stream) {
+  if (!ctx || !dg || !moves || !gain_sum) { gkh::set_error("bad argument"); return GK_ERR_ARG; }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  DGraph g = dview(dg);
+  *gain_sum = 0;
+  if (pos < 0) return GK_OK;
+  std::vector<int32_t> hm, hg;
+  GK_TRY(to_host(st, hm, moves, int64_t(pos) + 1));
+  GK_TRY(to_host(st, hg, gain, g.n));
+  int64_t s = 0;
+  for (int32_t v : hm) s += hg[v];
+  Buf<int32_t> ctl;
+  GK_TRY(ctl.alloc(ctx->pool, 4));
+  gkd::fill<int32_t>(ctl.p, pos, 4, st);
+  GK_TRY(gkd::launch_apply(g, pos + 1, moves, ctl.p, dest, part, upd, st));
+  GK_HIP(hipStreamSynchronize(st));
+  *gain_sum = s;
+  return GK_OK;
+}
+
+int gk_k_cut(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, int64_t* cut, int64_t* pw,
+             void* stream) {
+  if (!ctx || !dg || !dg->adjwgt || !dg->vwgt || !cut || k < 1) { gkh::set_error("bad argument"); return GK_ERR_ARG; }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  DGraph g = dview(dg);
+  Buf<unsigned long long> c, p;
+  GK_TRY(c.alloc(ctx->pool, 1));
+  GK_TRY(p.alloc(ctx->pool, k));
+  GK_TRY(gkd::launch_cut(g, k, part, c.p, p.p, st));
+  unsigned long long hc = 0;
+  GK_TRY(fetch(ctx, st, {{&hc, c.p, sizeof hc}}));
+  *cut = static_cast<int64_t>(hc);
+  if (pw) {
+    std::vector<unsigned long long> hp;
+    GK_TRY(to_host(st, hp, p.p, k));
+    for (int32_t q = 0; q < k; ++q) pw[q] = static_cast<int64_t>(hp[q]);
+  }
+  return GK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// Launch wrappers of the gfx950 kernels (coarsen.hip, refine.hip), called by
+// the host driver.  All take device pointers and a stream; all return a gk
+// status code (0 = launched).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gkd {
+
+struct DGraph {
+  int32_t n = 0;
+  int32_t nnz = 0;
+  const int32_t* xadj = nullptr;
+  const int32_t* adj = nullptr;
+  const int32_t* w = nullptr;
+  const int32_t* vw = nullptr;
+};
+
+// Lanes per vertex for row-parallel kernels, from the average degree.
+int lanes_for_degree(double avg_deg);
+
+// ---- coarsening (coarsen.hip)
+int launch_match(const DGraph& g, int lanes, int32_t* cand, hipStream_t st);
+// Pointer jumping over the candidate pseudo-forest.
+int launch_jump_init(int32_t n, const int32_t* cand, int32_t* next, int32_t* dist, hipStream_t st);
+// One Jacobi round; sets flags[round] = 1 when any pointer moved.  Skips all
+// work when round > 0 and flags[round - 1] == 0 (already converged).
+int launch_jump_round(int32_t n, const int32_t* cand, const int32_t* next_in, const int32_t* dist_in,
+                      int32_t* next_out, int32_t* dist_out, int32_t* flags, int32_t round,
+                      hipStream_t st);
+// label[v] = largest id of v's component (root pair found via next[]).
+int launch_label(int32_t n, const int32_t* cand, const int32_t* next, int32_t* compmax,
+                 int32_t* label, hipStream_t st);
+// keys[v] = (label << sbits) | dist, ids[v] = v.
+int launch_make_keys(int32_t n, const int32_t* label, const int32_t* dist, int sbits,
+                     uint64_t* keys, int32_t* ids, hipStream_t st);
+// Chunking of the sorted order: cmap, chunk pointers, coarse weights; nc_dev.
+int launch_cluster(int32_t n, const uint64_t* skeys, const int32_t* sids, int sbits,
+                   const int32_t* vwgt, int32_t G, int32_t* gstart, int32_t* flagscan,
+                   int32_t* cmap, int32_t* cptr, int32_t* cvwgt, int32_t* nc_dev,
+                   int32_t* scratch, hipStream_t st);
+// Contraction, phase 1: per coarse row the upper bound of its entries
+// (sum of member degrees), its class (16/32/64-lane or big) and the
+// exclusive scan of the bounds (toff, nc+1 entries).
+int launch_contract_bounds(const DGraph& g, int32_t nc, const int32_t* cptr, const int32_t* order,
+                           int32_t* ub, int32_t* toff, int32_t* lists, int32_t* list_cnt,
+                           int32_t* scratch, hipStream_t st);
+// Phase 2 for the rows of one class (lanes in {16, 32, 64}).
+int launch_contract_rows(const DGraph& g, int lanes, const int32_t* rows, int32_t nrows,
+                         const int32_t* cptr, const int32_t* order, const int32_t* cmap,
+                         const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len,
+                         hipStream_t st);
+// Phase 2 for big rows: gather to (row-rank, neighbour) keys.
+int launch_contract_big_gather(const DGraph& g, const int32_t* rows, int32_t nrows,
+                               const int32_t* boff, const int32_t* cptr, const int32_t* order,
+                               const int32_t* cmap, uint64_t* keys, int32_t* vals, hipStream_t st);
+// Segmented merge of the sorted big-row entries into the temporary rows.
+int launch_contract_big_merge(const int32_t* rows, int32_t nrows, const int32_t* boff,
+                              const uint64_t* keys, const int32_t* vals, const int32_t* toff,
+                              int32_t* tadj, int32_t* tw, int32_t* len, hipStream_t st);
+// Phase 3: copy the merged rows to their final CSR slots.
+int launch_contract_copy(int32_t nc, const int32_t* toff, const int32_t* len, const int32_t* cxadj,
+                         const int32_t* tadj, const int32_t* tw, int32_t* cadj, int32_t* cw,
+                         hipStream_t st);
+// Big-row prefix (boff = exclusive scan of ub over the big rows).
+int launch_big_offsets(const int32_t* rows, int32_t nrows, const int32_t* ub, int32_t* boff,
+                       int32_t* total, int32_t* scratch, hipStream_t st);
+// cxadj = exclusive scan of len (nc + 1 entries), total into *cnnz.
+int launch_len_scan(int32_t nc, const int32_t* len, int32_t* cxadj, int32_t* cnnz,
+                    int32_t* scratch, hipStream_t st);
+
+// ---- refinement (refine.hip)
+int launch_project(int32_t n, const int32_t* cmap, const int32_t* cpart, const int32_t* cbnd,
+                   int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* upd,
+                   hipStream_t st);
+// Heavy list: vertices whose degree exceeds `lanes` (compacted, any order).
+int launch_heavy_list(const DGraph& g, int lanes, int32_t* list, int32_t* cnt, hipStream_t st);
+int launch_gain(const DGraph& g, int32_t k, int lanes, const int32_t* heavy, int32_t nheavy,
+                const int32_t* part, const int32_t* upd, int32_t* bnd, int32_t* gain,
+                int32_t* dest, hipStream_t st);
+// Selection: clears upd, appends keys ((~gain & 0x7fffffff) << idbits | v)
+// of the selected vertices; ctl[0] = count, ctl[1] = max gain.
+int launch_select(const DGraph& g, const int32_t* bnd, const int32_t* gain, const int32_t* dest,
+                  const int32_t* pw, int64_t cap, int idbits, int32_t* upd, uint64_t* keys,
+                  int32_t* ctl, hipStream_t st);
+// Balance: first min(count, M) keys (sorted when presorted, else sorted in
+// LDS); writes moves[0..pos], ctl[2] = pos; advances pw; cut -= gains.
+int launch_balance(const DGraph& g, int32_t k, const uint64_t* keys, int32_t count, int32_t M,
+                   bool presorted, int idbits, const int32_t* part, const int32_t* dest,
+                   const int32_t* gain, int32_t* pw, int64_t cap, int32_t* moves, int32_t* ctl,
+                   long long* cut, hipStream_t st);
+int launch_apply(const DGraph& g, int32_t M, const int32_t* moves, const int32_t* ctl,
+                 const int32_t* dest, int32_t* part, int32_t* upd, hipStream_t st);
+int launch_boundary(const DGraph& g, const int32_t* part, int32_t* bnd, hipStream_t st);
+// cut (unsigned long long accumulator) and part weights pw64[k].
+int launch_cut(const DGraph& g, int32_t k, const int32_t* part, unsigned long long* cut,
+               unsigned long long* pw64, hipStream_t st);
+
+int launch_sum(const int32_t* a, int64_t n, unsigned long long* out, hipStream_t st);
+int launch_cvwgt(int32_t nc, const int32_t* cptr, const int32_t* order, const int32_t* vw,
+                 int32_t* cvw, hipStream_t st);
+
+}  // namespace gkd

@@ -1,0 +1,461 @@
+// Uncoarsening and refinement kernels for gfx950: projection, boundary/gain
+// computation, independent-set move selection, balanced-prefix search,
+// move application and the final cut/part weights -- the work of
+// gkway/uncoarsen.hpp and gkway/cuda_check.hpp:61-73.  Integer-exact and
+// schedule-independent: selected moves are ordered by a sort on the packed
+// key (gain desc, id asc) before any order-sensitive step.
+#include "kernels.hpp"
+#include "prim.hpp"
+#include "../../include/gkway.h"
+
+namespace gkd {
+
+static inline int hip_status() { return hipGetLastError() == hipSuccess ? 0 : GK_ERR_HIP; }
+
+// expand_partition (gkway/uncoarsen.hpp:325-356), fused with the two
+// device-to-device copies that follow it there (:381-382).
+__global__ void __launch_bounds__(256) k_project(int32_t n, const int32_t* __restrict__ cmap,
+                                                 const int32_t* __restrict__ cpart,
+                                                 const int32_t* __restrict__ cbnd,
+                                                 int32_t* __restrict__ part, int32_t* __restrict__ bnd,
+                                                 int32_t* __restrict__ gain, int32_t* __restrict__ dest,
+                                                 int32_t* __restrict__ upd) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  const int32_t c = cmap[v];
+  const int32_t p = cpart[c];
+  const int32_t b = cbnd[c] ? 1 : 0;
+  part[v] = p;
+  bnd[v] = b;
+  gain[v] = 0;
+  dest[v] = p;
+  upd[v] = b;
+}
+
+int launch_project(int32_t n, const int32_t* cmap, const int32_t* cpart, const int32_t* cbnd,
+                   int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* upd,
+                   hipStream_t st) {
+  if (n <= 0) return 0;
+  k_project<<<grid_for(n, 256), 256, 0, st>>>(n, cmap, cpart, cbnd, part, bnd, gain, dest, upd);
+  return hip_status();
+}
+
+// Vertices with degree above `lanes` go to the heavy (LDS histogram) path.
+__global__ void __launch_bounds__(256) k_heavy_list(int32_t n, const int32_t* __restrict__ xadj, int lanes,
+                                                    int32_t* __restrict__ list, int32_t* __restrict__ cnt) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const bool h = v < n && (xadj[v + 1] - xadj[v]) > lanes;
+  const uint64_t m = ballot(h);
+  if (m == 0) return;
+  const int leader = __ffsll(static_cast<long long>(m)) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(cnt, popc64(m));
+  base = __shfl(base, leader, 64);
+  if (h) list[base + popc64(m & lanemask_lt())] = int32_t(v);
+}
+
+int launch_heavy_list(const DGraph& g, int lanes, int32_t* list, int32_t* cnt, hipStream_t st) {
+  fill<int32_t>(cnt, 0, 1, st);
+  if (g.n <= 0) return 0;
+  k_heavy_list<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, lanes, list, cnt);
+  return hip_status();
+}
+
+// update_vertex_info + calculate_gain (gkway/uncoarsen.hpp:175-237, 82-109)
+// for rows of at most S entries, S lanes per vertex.  Each lane holds one
+// (part, weight) entry and sums the weights of the lanes with the same part
+// (an O(S) shuffle loop), which gives the connectivity to every adjacent
+// part without a per-vertex histogram.  Gain is the signed max over the
+// adjacent parts of conn_q - conn_p, floored at 0, lowest q on ties.
+template <int S>
+__global__ void __launch_bounds__(256) k_gain_small(int32_t n, const int32_t* __restrict__ xadj,
+                                                    const int32_t* __restrict__ adj,
+                                                    const int32_t* __restrict__ w,
+                                                    const int32_t* __restrict__ part,
+                                                    const int32_t* __restrict__ upd,
+                                                    int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
+                                                    int32_t* __restrict__ dest) {
+  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int32_t v = static_cast<int32_t>(gt / S);
+  const int sub = static_cast<int>(gt % S);
+  if (v >= n) return;
+  if (!upd[v]) return;
+  const int32_t s = xadj[v], d = xadj[v + 1] - s;
+  if (d > S) return;  // heavy path
+  const int32_t p = part[v];
+  int32_t q = -1, wj = 0;
+  if (sub < d) {
+    q = part[adj[s + sub]];
+    wj = w[s + sub];
+  }
+  int32_t conn = 0, id = 0;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const int32_t qi = __shfl(q, i, S);
+    const int32_t wi = __shfl(wj, i, S);
+    conn += (qi == q) ? wi : 0;
+    id += (qi == p) ? wi : 0;
+  }
+  const bool ext = q >= 0 && q != p;
+  int32_t g = ext ? conn - id : INT32_MIN;
+  int32_t bq = ext ? q : INT32_MAX;
+  int32_t anyext = ext ? 1 : 0;
+#pragma unroll
+  for (int o = S / 2; o > 0; o >>= 1) {
+    const int32_t og = __shfl_xor(g, o, S), oq = __shfl_xor(bq, o, S);
+    if (og > g || (og == g && oq < bq)) { g = og; bq = oq; }
+    anyext |= __shfl_xor(anyext, o, S);
+  }
+  if (sub == 0) {
+    bnd[v] = anyext;
+    if (anyext && g > 0) {
+      gain[v] = g;
+      dest[v] = bq;
+    } else {
+      gain[v] = 0;
+      dest[v] = p;
+    }
+  }
+}
+
+// Rows above S entries: one wave per vertex with a dense LDS histogram of the
+// k part connectivities (integer LDS atomics, order-independent).
+__global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ heavy, int32_t nheavy,
+                                                   int32_t k, const int32_t* __restrict__ xadj,
+                                                   const int32_t* __restrict__ adj,
+                                                   const int32_t* __restrict__ w,
+                                                   const int32_t* __restrict__ part,
+                                                   const int32_t* __restrict__ upd,
+                                                   int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
+                                                   int32_t* __restrict__ dest) {
+  extern __shared__ int32_t hist[];
+  const int lane = lane_id();
+  for (int32_t hi = blockIdx.x; hi < nheavy; hi += gridDim.x) {
+    const int32_t v = heavy[hi];
+    if (!upd[v]) continue;
+    for (int32_t q = lane; q < k; q += 64) hist[q] = 0;
+    __syncthreads();
+    const int32_t p = part[v];
+    const int32_t s = xadj[v], e = xadj[v + 1];
+    bool ext = false;
+    for (int32_t j = s + lane; j < e; j += 64) {
+      const int32_t q = part[adj[j]];
+      atomicAdd(&hist[q], w[j]);
+      ext |= (q != p);
+    }
+    __syncthreads();
+    const bool any = ballot(ext) != 0;
+    const int32_t id = hist[p];
+    int32_t g = INT32_MIN, bq = INT32_MAX;
+    for (int32_t q = lane; q < k; q += 64) {
+      const int32_t h = hist[q];
+      if (q != p && h > 0) {
+        const int32_t gq = h - id;
+        if (gq > g || (gq == g && q < bq)) { g = gq; bq = q; }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const int32_t og = __shfl_xor(g, o, 64), oq = __shfl_xor(bq, o, 64);
+      if (og > g || (og == g && oq < bq)) { g = og; bq = oq; }
+    }
+    if (lane == 0) {
+      bnd[v] = any ? 1 : 0;
+      if (any && g > 0) { gain[v] = g; dest[v] = bq; }
+      else { gain[v] = 0; dest[v] = p; }
+    }
+    __syncthreads();
+  }
+}
+
+int launch_gain(const DGraph& g, int32_t k, int lanes, const int32_t* heavy, int32_t nheavy,
+                const int32_t* part, const int32_t* upd, int32_t* bnd, int32_t* gain,
+                int32_t* dest, hipStream_t st) {
+  if (g.n <= 0) return 0;
+  const unsigned grid = grid_for(int64_t(g.n) * lanes, 256);
+  switch (lanes) {
+    case 4: k_gain_small<4><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
+    case 8: k_gain_small<8><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
+    case 16: k_gain_small<16><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
+    case 32: k_gain_small<32><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
+    default: k_gain_small<64><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
+  }
+  if (nheavy > 0) {
+    const unsigned hg = static_cast<unsigned>(nheavy < 4096 ? nheavy : 4096);
+    k_gain_heavy<<<hg, 64, sizeof(int32_t) * k, st>>>(heavy, nheavy, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest);
+  }
+  return hip_status();
+}
+
+// create_independent_move_buffer (gkway/uncoarsen.hpp:111-173): clears the
+// update flags, keeps boundary vertices with positive gain whose move fits the
+// cap and that have no smaller-id neighbour with the same property, and
+// appends them with one atomic per wave (ballot + mbcnt instead of the
+// reference's 32-lane __activemask/__popc).
+__global__ void __launch_bounds__(256) k_select(int32_t n, const int32_t* __restrict__ xadj,
+                                                const int32_t* __restrict__ adj,
+                                                const int32_t* __restrict__ vw,
+                                                const int32_t* __restrict__ bnd,
+                                                const int32_t* __restrict__ gain,
+                                                const int32_t* __restrict__ dest,
+                                                const int32_t* __restrict__ pw, long long cap, int idbits,
+                                                int32_t* __restrict__ upd, uint64_t* __restrict__ keys,
+                                                int32_t* __restrict__ ctl) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  bool sel = false;
+  int32_t g = 0;
+  if (v < n) {
+    upd[v] = 0;
+    g = gain[v];
+    if (bnd[v] && g > 0 && (long long)pw[dest[v]] + vw[v] <= cap) {
+      sel = true;
+      for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
+        const int32_t u = adj[j];
+        if (u < v && gain[u] > 0 && (long long)pw[dest[u]] + vw[u] <= cap) { sel = false; break; }
+      }
+    }
+  }
+  const uint64_t m = ballot(sel);
+  if (m == 0) return;
+  const int leader = __ffsll(static_cast<long long>(m)) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(&ctl[0], popc64(m));
+  base = __shfl(base, leader, 64);
+  if (sel) {
+    keys[base + popc64(m & lanemask_lt())] =
+        (uint64_t(uint32_t(~g) & 0x7fffffffu) << idbits) | uint64_t(uint32_t(v));
+    atomicMax(&ctl[1], g);
+  }
+}
+
+int launch_select(const DGraph& g, const int32_t* bnd, const int32_t* gain, const int32_t* dest,
+                  const int32_t* pw, int64_t cap, int idbits, int32_t* upd, uint64_t* keys,
+                  int32_t* ctl, hipStream_t st) {
+  fill<int32_t>(ctl, 0, 2, st);
+  if (g.n <= 0) return 0;
+  k_select<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, bnd, gain, dest, pw, cap,
+                                               idbits, upd, keys, ctl);
+  return hip_status();
+}
+
+// One workgroup: order the (at most 1024) first moves, then find the largest
+// balanced prefix (find_mv_wgt_sequence + scan + find_balance_sequence,
+// gkway/uncoarsen.hpp:239-288, 37-80).  Instead of the reference's dense
+// k x B delta matrix, each wave owns a set of parts and runs a wave64 prefix
+// scan of that part's weight delta along the move sequence; positions where
+// the part would exceed the cap are OR-ed into a bitmap, and the answer is the
+// highest clear bit.  Then the prefix is committed: pw advanced, cut reduced.
+constexpr int kBalThreads = 1024;
+constexpr int kBalMax = 1024;
+
+__global__ void __launch_bounds__(kBalThreads) k_balance(int32_t k, const uint64_t* __restrict__ keys,
+                                                         int32_t count, int32_t M, int presorted,
+                                                         int idbits, const int32_t* __restrict__ part,
+                                                         const int32_t* __restrict__ dest,
+                                                         const int32_t* __restrict__ gain,
+                                                         const int32_t* __restrict__ vw,
+                                                         int32_t* __restrict__ pw, long long cap,
+                                                         int32_t* __restrict__ moves,
+                                                         int32_t* __restrict__ ctl,
+                                                         long long* __restrict__ cut) {
+  __shared__ uint64_t sk[kBalMax];
+  __shared__ int32_t sv[kBalMax], ssrc[kBalMax], sdst[kBalMax], sw[kBalMax], sg[kBalMax];
+  __shared__ unsigned long long bad[kBalMax / 64];
+  __shared__ int32_t spos;
+  __shared__ long long sgain[kBalThreads / 64];
+  extern __shared__ int32_t dpw[];  // k entries
+  const int t = threadIdx.x;
+  const int B = count < M ? count : M;
+  sk[t] = (t < (presorted ? B : count)) ? keys[t] : ~0ull;
+  if (t < kBalMax / 64) bad[t] = 0;
+  for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
+  __syncthreads();
+  if (!presorted) {
+    // Bitonic sort of the 1024 keys (unused slots hold the maximum key).
+    for (int k2 = 2; k2 <= kBalMax; k2 <<= 1) {
+      for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
+        const int ix = t ^ j2;
+        if (ix > t) {
+          const uint64_t a = sk[t], b = sk[ix];
+          const bool up = (t & k2) == 0;
+          if (up ? a > b : a < b) { sk[t] = b; sk[ix] = a; }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  const uint64_t idmask = (idbits >= 64) ? ~0ull : ((1ull << idbits) - 1);
+  if (t < B) {
+    const int32_t v = int32_t(sk[t] & idmask);
+    sv[t] = v;
+    ssrc[t] = part[v];
+    sdst[t] = dest[v];
+    sw[t] = vw[v];
+    sg[t] = gain[v];
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  const int wave = t >> 6;
+  const int nchunk = (B + 63) >> 6;
+  for (int32_t q = wave; q < k; q += kBalThreads / 64) {
+    const long long pq = pw[q];
+    long long carry = 0;
+    for (int c = 0; c < nchunk; ++c) {
+      const int j = c * 64 + lane;
+      int x = 0;
+      if (j < B) {
+        if (sdst[j] == q) x += sw[j];
+        if (ssrc[j] == q) x -= sw[j];
+      }
+      const long long incl = carry + wave_incl_scan(x);
+      const bool b = j < B && incl > 0 && pq + incl > cap;
+      const uint64_t m = ballot(b);
+      if (m != 0 && lane == 0) atomicOr(&bad[c], (unsigned long long)m);
+      carry = __shfl(incl, 63, 64);
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    int pos = -1;
+    for (int i = B - 1; i >= 0; --i) {
+      if (((bad[i >> 6] >> (i & 63)) & 1ull) == 0) { pos = i; break; }
+    }
+    spos = pos;
+  }
+  __syncthreads();
+  const int pos = spos;
+  long long gsum = 0;
+  if (t <= pos) {
+    atomicAdd(&dpw[ssrc[t]], -sw[t]);
+    atomicAdd(&dpw[sdst[t]], sw[t]);
+    moves[t] = sv[t];
+    gsum = sg[t];
+  }
+  gsum = wave_sum64(gsum);
+  if (lane == 0) sgain[wave] = gsum;
+  __syncthreads();
+  for (int32_t q = t; q < k; q += kBalThreads) pw[q] += dpw[q];
+  if (t == 0) {
+    long long s = 0;
+    for (int w2 = 0; w2 < kBalThreads / 64; ++w2) s += sgain[w2];
+    *cut -= s;
+    ctl[2] = pos;
+  }
+}
+
+int launch_balance(const DGraph& g, int32_t k, const uint64_t* keys, int32_t count, int32_t M,
+                   bool presorted, int idbits, const int32_t* part, const int32_t* dest,
+                   const int32_t* gain, int32_t* pw, int64_t cap, int32_t* moves, int32_t* ctl,
+                   long long* cut, hipStream_t st) {
+  if (M > kBalMax || (!presorted && count > kBalMax)) return GK_ERR_LIMIT;
+  k_balance<<<1, kBalThreads, sizeof(int32_t) * k, st>>>(k, keys, count, M, presorted ? 1 : 0, idbits,
+                                                         part, dest, gain, g.vw, pw, cap, moves, ctl, cut);
+  return hip_status();
+}
+
+// apply_sequence_move (gkway/uncoarsen.hpp:290-323) without the pw/cut parts
+// (done in k_balance): move, and mark the mover and its neighbours updated.
+__global__ void __launch_bounds__(256) k_apply(int32_t M, const int32_t* __restrict__ xadj,
+                                               const int32_t* __restrict__ adj,
+                                               const int32_t* __restrict__ moves,
+                                               const int32_t* __restrict__ ctl,
+                                               const int32_t* __restrict__ dest,
+                                               int32_t* __restrict__ part, int32_t* __restrict__ upd) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= M || j > ctl[2]) return;
+  const int32_t v = moves[j];
+  part[v] = dest[v];
+  upd[v] = 1;
+  for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) upd[adj[e]] = 1;
+}
+
+int launch_apply(const DGraph& g, int32_t M, const int32_t* moves, const int32_t* ctl,
+                 const int32_t* dest, int32_t* part, int32_t* upd, hipStream_t st) {
+  k_apply<<<grid_for(M, 256), 256, 0, st>>>(M, g.xadj, g.adj, moves, ctl, dest, part, upd);
+  return hip_status();
+}
+
+__global__ void __launch_bounds__(256) k_boundary(int32_t n, const int32_t* __restrict__ xadj,
+                                                  const int32_t* __restrict__ adj,
+                                                  const int32_t* __restrict__ part, int32_t* __restrict__ bnd) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  const int32_t p = part[v];
+  int32_t b = 0;
+  for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j)
+    if (part[adj[j]] != p) { b = 1; break; }
+  bnd[v] = b;
+}
+
+int launch_boundary(const DGraph& g, const int32_t* part, int32_t* bnd, hipStream_t st) {
+  if (g.n <= 0) return 0;
+  k_boundary<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, part, bnd);
+  return hip_status();
+}
+
+// calculate_cutsize + calculate_partition_wgt (gkway/uncoarsen.hpp:398-424,
+// gkway/cuda_check.hpp:61-73): grid-stride, 8 lanes per vertex; part weights
+// gathered in an LDS histogram per block, then one global atomic per block
+// and part (a block's share of the total weight fits 32 bits, the total
+// being below 2^31); the cut reduced per wave in 64 bits.
+__global__ void __launch_bounds__(256) k_cut(int32_t n, int32_t k, const int32_t* __restrict__ xadj,
+                                             const int32_t* __restrict__ adj,
+                                             const int32_t* __restrict__ w,
+                                             const int32_t* __restrict__ vw,
+                                             const int32_t* __restrict__ part,
+                                             unsigned long long* __restrict__ cut,
+                                             unsigned long long* __restrict__ pw64) {
+  extern __shared__ unsigned int hist32[];
+  for (int32_t q = threadIdx.x; q < k; q += 256) hist32[q] = 0;
+  __syncthreads();
+  long long c = 0;
+  const int sub = threadIdx.x & 7;
+  for (int64_t v = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; v < n; v += int64_t(gridDim.x) * 32) {
+    const int32_t p = part[v];
+    for (int32_t j = xadj[v] + sub; j < xadj[v + 1]; j += 8) {
+      const int32_t u = adj[j];
+      if (v < u && part[u] != p) c += w[j];
+    }
+    if (sub == 0) atomicAdd(&hist32[p], (unsigned int)vw[v]);
+  }
+  c = wave_sum64(c);
+  if (lane_id() == 0 && c != 0) atomicAdd(cut, (unsigned long long)c);
+  __syncthreads();
+  for (int32_t q = threadIdx.x; q < k; q += 256)
+    if (hist32[q]) atomicAdd(&pw64[q], (unsigned long long)hist32[q]);
+}
+
+int launch_cut(const DGraph& g, int32_t k, const int32_t* part, unsigned long long* cut,
+               unsigned long long* pw64, hipStream_t st) {
+  fill<unsigned long long>(cut, 0ull, 1, st);
+  fill<unsigned long long>(pw64, 0ull, k, st);
+  if (g.n <= 0) return 0;
+  unsigned grid = grid_for(int64_t(g.n) * 8, 256);
+  if (grid > 2048) grid = 2048;
+  k_cut<<<grid, 256, sizeof(unsigned int) * k, st>>>(g.n, k, g.xadj, g.adj, g.w, g.vw, part, cut, pw64);
+  return hip_status();
+}
+
+// Sum of an int32 array into a 64-bit accumulator (block reduce, one atomic
+// per block).
+__global__ void __launch_bounds__(256) k_sum(const int32_t* __restrict__ a, int64_t n,
+                                             unsigned long long* __restrict__ out) {
+  __shared__ long long part_s[4];
+  long long s = 0;
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) s += a[i];
+  s = wave_sum64(s);
+  if (lane_id() == 0) part_s[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(part_s[0] + part_s[1] + part_s[2] + part_s[3]));
+}
+
+int launch_sum(const int32_t* a, int64_t n, unsigned long long* out, hipStream_t st) {
+  fill<unsigned long long>(out, 0ull, 1, st);
+  if (n <= 0) return 0;
+  unsigned grid = grid_for(n, 256);
+  if (grid > 1024) grid = 1024;
+  k_sum<<<grid, 256, 0, st>>>(a, n, out);
+  return hip_status();
+}
+
+}  // namespace gkd

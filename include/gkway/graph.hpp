@@ -1,0 +1,60 @@
+// gk::Graph — host C++ mirror of the reference's graph reader
+// (gkway/graph.hpp:17-51 in wanluanlee/DAC_g-kway), implemented on the C ABI
+// (gk_graph_read / gk_write_partition).  Same class name, getters and
+// dump_result; the arrays returned by the getters keep the reference's
+// conventions (adjp = xadj, int arrays) except that adjncy is 0-based (the
+// reference keeps the file's 1-based ids).  Errors throw std::runtime_error,
+// as the reference's check_cuda does.
+#pragma once
+#include <cstdint>
+#include <filesystem>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../gkway.h"
+
+namespace gk {
+
+class Graph {
+ public:
+  explicit Graph(const std::string& input_path) : _input_path(input_path) { _parse(); }
+  Graph(const Graph&) = delete;
+  Graph& operator=(const Graph&) = delete;
+  ~Graph() { gk_graph_free(&_g); }
+
+  const int* get_adjncy() const { return _g.adjncy; }
+  const int* get_adjp() const { return _g.xadj; }
+  const int* get_adjwgt() const { return _g.adjwgt; }
+  const int* get_vwgt() const { return _g.vwgt; }
+  size_t get_num_vertex() const { return static_cast<size_t>(_g.n); }
+  // Directed adjacency entries (2 x edges), as the reference's _num_edge.
+  size_t get_num_edge() const { return static_cast<size_t>(_g.nnz); }
+  int get_total_vertex_weight() const {
+    int64_t t = 0;
+    for (int32_t v = 0; v < _g.n; ++v) t += _g.vwgt ? _g.vwgt[v] : 1;
+    return static_cast<int>(t);
+  }
+  const gk_csr& csr() const { return _g; }
+
+  // Writes "<cwd>/<file_name>.out", one 0-based part id per line.
+  void dump_result(const std::vector<int>& vertex_partition, const std::string& file_name) const {
+    const std::string out = (std::filesystem::current_path() / (file_name + ".out")).string();
+    int rc = gk_write_partition(out.c_str(), vertex_partition.data(),
+                                static_cast<int32_t>(vertex_partition.size()));
+    if (rc != GK_OK) throw std::runtime_error(std::string("dump_result: ") + gk_last_error());
+  }
+
+ private:
+  void _parse() {
+    int rc = gk_graph_read(_input_path.c_str(), &_g);
+    if (rc != GK_OK)
+      throw std::runtime_error(std::string("cannot read ") + _input_path + ": " + gk_strerror(rc) + ": " +
+                               gk_last_error());
+  }
+  std::string _input_path;
+  gk_csr _g{};
+};
+
+}  // namespace gk

@@ -84,31 +84,52 @@ void oracle_match(const gk_csr* g, int32_t* cand) {
 }
 
 // init_group_id + HEC_union_find sweep loop (gkway/coarsen.hpp:351-359,
-// 80-118, 461-469), as synchronous sweeps: L_{t+1}[v] = max of L_t over v,
-// cand[v] and every u with cand[u] == v.  stamp[v] = the sweep in which v's
-// label last changed (0 for the component maximum), i.e. the hop distance to
-// the component's largest id.  Returns the number of sweeps run, the last of
-// which changes nothing.
+// 80-118, 461-469).  The groups are the connected components of the
+// candidate graph {v, cand[v]}; label[v] = the largest id of v's component,
+// the value the reference's max-propagation converges to (:103-112).  The
+// reference's stamp is the sweep in which the label last changed, which
+// depends on the thread schedule; here stamp[v] = the hop distance from v to
+// its component's mutual pair (u, cand[u]) with cand[cand[u]] == u, following
+// cand.  Every component of this graph is a tree hanging off exactly one such
+// pair (an isolated vertex is its own pair).  Returns the number of
+// pointer-jumping rounds that move a pointer: ceil(log2(max stamp)), 0 when
+// the max stamp is <= 1; -1 if some cand chain never reaches a pair.
 int32_t oracle_group(int32_t n, const int32_t* cand, int32_t* label, int32_t* stamp) {
-  std::vector<i32> cur(n), nxt(n);
-  for (i32 v = 0; v < n; ++v) { cur[v] = v; stamp[v] = 0; }
-  int32_t sweeps = 0;
-  while (true) {
-    ++sweeps;
-    nxt = cur;
-    for (i32 v = 0; v < n; ++v) {
-      i32 c = cand[v];
-      nxt[v] = std::max(nxt[v], cur[c]);
-      nxt[c] = std::max(nxt[c], cur[v]);
+  std::vector<i32> depth(n, -1), root(n, -1);
+  std::vector<i32> path;
+  for (i32 v = 0; v < n; ++v) {
+    if (depth[v] >= 0) continue;
+    path.clear();
+    i32 u = v;
+    i64 guard = 0;
+    while (depth[u] < 0 && cand[cand[u]] != u) {
+      path.push_back(u);
+      u = cand[u];
+      if (++guard > n) return -1;
     }
-    bool changed = false;
-    for (i32 v = 0; v < n; ++v)
-      if (nxt[v] != cur[v]) { changed = true; stamp[v] = sweeps; }
-    cur.swap(nxt);
-    if (!changed) break;
+    if (depth[u] < 0) { depth[u] = 0; root[u] = u; }
+    for (i32 i = i32(path.size()) - 1; i >= 0; --i) {
+      const i32 x = path[i];
+      depth[x] = depth[cand[x]] + 1;
+      root[x] = root[cand[x]];
+    }
   }
-  std::copy(cur.begin(), cur.end(), label);
-  return sweeps;
+  std::vector<i32> compmax(n, -1);
+  i32 maxd = 0;
+  for (i32 v = 0; v < n; ++v) {
+    const i32 r = root[v];
+    const i32 key = std::max(r, cand[r]);
+    compmax[key] = std::max(compmax[key], v);
+    maxd = std::max(maxd, depth[v]);
+  }
+  for (i32 v = 0; v < n; ++v) {
+    const i32 r = root[v];
+    label[v] = compmax[std::max(r, cand[r])];
+    stamp[v] = depth[v];
+  }
+  int32_t rounds = 0;
+  while ((i64(1) << rounds) < maxd) ++rounds;
+  return rounds;
 }
 
 // constraint_group_size + coarse numbering + cal_coarsen_vwgt
@@ -348,6 +369,7 @@ int oracle_partition(const gk_csr* g0, const gk_params* p, oracle_initpart_fn in
     std::vector<i32> cand(f.n), label(f.n), stamp(f.n), order(f.n), cmap(f.n), cvw(f.n);
     oracle_match(&fv, cand.data());
     i32 sw = oracle_group(f.n, cand.data(), label.data(), stamp.data());
+    if (sw < 0) return GK_ERR_GRAPH;
     i32 nc = oracle_cluster(f.n, label.data(), stamp.data(), f.vw.data(), p->max_group,
                             order.data(), cmap.data(), cvw.data());
     if (nc == f.n || nc < k + 1) break;

@@ -1,0 +1,66 @@
+"""End-to-end parity: the HIP V-cycle against the CPU oracle, bit for bit.
+
+Every level's candidate array, clustering map and coarse graph (FNV-1a
+hashes), every level's refinement trace (rounds, moves, cut after the level,
+partition hash) and the final partition must agree exactly.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def compare(gk, oracle, ctx, g, k, **kw):
+    print(f"case n={g.n} nnz={g.nnz} k={k} {kw}", flush=True)
+    part, st = ctx.partition(g, k, flags=gk.GK_FLAG_TRACE, **kw)
+    opart, ost = oracle.partition(g, k, flags=gk.GK_FLAG_TRACE, **kw)
+    assert st.nlevels == ost.nlevels
+    for l in range(st.nlevels):
+        a, b = st.levels[l], ost.levels[l]
+        got = (a.n, a.nnz, a.sweeps, a.h_cand, a.h_cmap, a.h_coarse)
+        exp = (b.n, b.nnz, b.sweeps, b.h_cand, b.h_cmap, b.h_coarse)
+        assert got == exp, f"coarsening level {l}: {got} != {exp}"
+    assert st.init_cut == ost.init_cut
+    for l in reversed(range(st.nlevels)):
+        a, b = st.levels[l], ost.levels[l]
+        got = (a.rounds, a.moves, a.cut_after, a.h_part)
+        exp = (b.rounds, b.moves, b.cut_after, b.h_part)
+        assert got == exp, f"refinement level {l}: {got} != {exp}"
+    assert np.array_equal(part, opart)
+    assert st.cut == ost.cut == st.cut_tracked
+    c, pw = oracle.cut(g, k, part)
+    assert c == st.cut
+    assert pw.max() == st.max_part_wgt
+    return st
+
+
+@pytest.mark.parametrize("k", [2, 4, 8, 64])
+def test_delaunay_n11(gk, oracle, ctx, k):
+    g = gk.read_metis(os.path.join(GOLDEN, "delaunay_n11.graph"))
+    st = compare(gk, oracle, ctx, g, k)
+    assert st.imbalance <= 1.03 + 1e-12
+
+
+@pytest.mark.parametrize("k", [4, 16])
+def test_delaunay_n11_ref_policy(gk, oracle, ctx, k):
+    g = gk.read_metis(os.path.join(GOLDEN, "delaunay_n11.graph"))
+    compare(gk, oracle, ctx, g, k, coarsen_policy=gk.GK_COARSEN_REF)
+
+
+@pytest.mark.parametrize("n,k,seed", [(2048, 4, 1), (20000, 16, 2), (60000, 64, 3)])
+def test_delaunay_generated(gk, oracle, ctx, n, k, seed):
+    compare(gk, oracle, ctx, gk.delaunay(n, seed), k)
+
+
+@pytest.mark.parametrize("n,layers,k", [(20000, 40, 32), (50000, 100, 256)])
+def test_dag_weighted(gk, oracle, ctx, n, layers, k):
+    compare(gk, oracle, ctx, gk.dag(n, layers, 3), k)
+
+
+@pytest.mark.parametrize("n,k", [(30000, 8), (60000, 256)])
+def test_road(gk, oracle, ctx, n, k):
+    compare(gk, oracle, ctx, gk.road(n, 2), k)
