@@ -1,0 +1,217 @@
+"""Benchmark: partitioned edges/s of the MI355X multilevel k-way partitioner.
+
+Default workload (the headline configuration): 2-D Delaunay triangulation of
+2^22 uniform random points (seed 1, ~12.6M undirected edges), unit weights,
+k = 64, imbalance 3%.  One step = one full V-cycle (coarsening, initial
+partition of the coarsest graph on the host, refinement of every level, final
+cut) from a device-resident CSR to a device-resident partition.
+
+The result of every timed step must be bit-identical to the CPU oracle's
+partition of the same graph (checked once, the oracle also being the CPU
+baseline); otherwise `value` is reported as 0.
+
+  python bench.py [--steps K] [--warmup W] [--workload delaunay|dag|road]
+                  [--n N] [--k K] [--no-cpu] [--calibrate-copy]
+"""
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("gkway_amd", os.path.join(ROOT, "dac_g-kway_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gkway_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_graph(gk, workload, n, seed):
+    if workload == "delaunay":
+        return gk.delaunay(n, seed)
+    if workload == "dag":
+        return gk.dag(n, 200, seed)
+    if workload == "road":
+        return gk.road(n, seed)
+    raise SystemExit(f"unknown workload {workload}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="delaunay")
+    ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline + bit-exact check)")
+    ap.add_argument("--no-stage-time", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain device copy of 1 GiB (PMC calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(graph-level sharding across GPUs is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    gk = load_pkg()
+    import torch
+
+    t = time.time()
+    g = make_graph(gk, args.workload, args.n, args.seed)
+    log(f"generated {args.workload} n={g.n} m={g.m} in {time.time() - t:.1f}s")
+
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).cuda()
+    dx, da, dw, dv = dev(g.xadj), dev(g.adjncy), dev(g.adjwgt), dev(g.vwgt)
+    dpart = torch.zeros(g.n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    I32P = gk.I32P
+    ptr = lambda t_: ctypes.cast(t_.data_ptr(), I32P)
+    csr = gk.Csr()
+    csr.n, csr.nnz = g.n, g.nnz
+    csr.xadj, csr.adjncy, csr.adjwgt, csr.vwgt = ptr(dx), ptr(da), ptr(dw), ptr(dv)
+    L = gk.lib()
+
+    ctx = gk.Context(0)
+    flags = 0 if args.no_stage_time else gk.GK_FLAG_STAGE_TIME
+    p = gk.params(args.k, flags=flags)
+
+    def step():
+        st = gk.Stats()
+        rc = L.gk_partition_device(ctx.handle, ctypes.byref(csr), ctypes.byref(p), ptr(dpart),
+                                   ctypes.byref(st), None)
+        if rc != 0:
+            raise RuntimeError(f"gk_partition_device: {L.gk_strerror(rc).decode()}: {L.gk_last_error().decode()}")
+        return st
+
+    for i in range(args.warmup):
+        st = step()
+        log(f"warmup {i}: {st.t_total_ms:.1f} ms cut={st.cut}")
+
+    torch.cuda.synchronize()
+    times, stats = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        a = time.perf_counter()
+        st = step()
+        times.append(time.perf_counter() - a)
+        stats.append(st)
+        log(f"step {i}: {times[-1] * 1e3:.1f} ms cut={st.cut} imb={st.imbalance:.4f}")
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    part = dpart.cpu().numpy()
+    cuts = {s.cut for s in stats}
+    deterministic = len(cuts) == 1
+    st = stats[-1]
+
+    # Per-stage device time (HIP events on the work stream) over the timed steps.
+    names = gk.STAGE_NAMES
+    stage = {}
+    for i, nm in enumerate(names):
+        ms = sum(s.stage_ms[i] for s in stats)
+        cnt = sum(s.stage_launches[i] for s in stats)
+        by = sum(s.stage_bytes[i] for s in stats)
+        stage[nm] = (ms, cnt, by)
+    roofline = None
+    if not args.no_stage_time:
+        dom = max(stage, key=lambda k_: stage[k_][0])
+        ms, cnt, by = stage[dom]
+        if ms > 0:
+            gbs = by / (ms / 1e3) / 1e9
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                        "alg_bytes_per_call": int(by / max(cnt, 1)),
+                        "avg_call_us": round(ms * 1e3 / max(cnt, 1), 2)}
+            tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.workload}_n{args.n}_k{args.k}.json")
+            if os.path.exists(tf):
+                with open(tf) as f:
+                    roofline["traffic"] = json.load(f).get("hbm_bytes_per_call")
+        for nm in names:
+            ms, cnt, by = stage[nm]
+            if cnt:
+                log(f"stage {nm:13s} {ms / args.steps:9.3f} ms/step {cnt // args.steps:6d} calls/step "
+                    f"{by / args.steps / 1e6:10.1f} MB/step {by / max(ms, 1e-9) / 1e6:8.1f} GB/s")
+
+    m = g.m
+    ms_step = total / args.steps * 1e3
+    value = m / (total / args.steps)
+    p50 = statistics.median(times) * 1e3
+    p99 = float(np.percentile(np.array(times) * 1e3, 99))
+
+    cpu = None
+    bit_exact = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle  # the checker, and the CPU baseline
+        t = time.perf_counter()
+        opart, ost = _oracle.partition(g, args.k, initpart_threads=1)
+        tcpu = time.perf_counter() - t
+        bit_exact = bool(np.array_equal(opart, part)) and ost.cut == st.cut
+        cpu = {"value": round(m / tcpu, 1), "unit": "edges/s", "cores": 1, "kind": "port",
+               "sample": f"one full partition of the same graph by the single-threaded oracle "
+                         f"({tcpu:.1f} s, initial partitioner on 1 thread)",
+               "cut": int(ost.cut)}
+        log(f"oracle: {tcpu:.1f}s cut={ost.cut} bit_exact={bit_exact}")
+
+    if args.calibrate_copy:
+        import torch as _t
+        a = _t.empty(1 << 28, dtype=_t.int32, device="cuda")
+        b = _t.empty_like(a)
+        a.fill_(1)
+        _t.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        _t.cuda.synchronize()
+        log("calibration copy: 3 x 1 GiB device-to-device")
+
+    ok = deterministic and (bit_exact is not False) and st.imbalance <= 1.03 + 1e-9
+    out = {
+        "metric": "partitioned edges/s at bit-exact cut, eps=3% (plain path, one graph per call)",
+        "value": round(value, 1) if ok else 0.0,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "p50_ms": round(p50, 3),
+        "p99_ms": round(p99, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"{args.workload} n={g.n} m={m} seed={args.seed}", "k": args.k,
+                   "eps": 0.03, "window": "device CSR -> device partition"},
+        "cut": int(st.cut),
+        "imbalance": round(st.imbalance, 5),
+        "levels": int(st.nlevels),
+        "coarsest_n": int(st.coarsest_n),
+        "phase_ms": {"coarsen": round(st.t_coarsen_ms, 3), "init": round(st.t_init_ms, 3),
+                     "refine": round(st.t_refine_ms, 3)},
+        "bit_exact_vs_oracle": bit_exact,
+        "deterministic": deterministic,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

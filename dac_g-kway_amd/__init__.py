@@ -51,13 +51,25 @@ class LevelStats(Structure):
                 ("h_cmap", c_uint64), ("h_coarse", c_uint64), ("h_part", c_uint64)]
 
 
+GK_FLAG_STAGE_TIME = 4
+GK_ST_COUNT = 16
+STAGE_NAMES = ["match", "group", "sort", "cluster", "contract", "contract_big", "project", "gain",
+               "select", "msort", "balance", "apply", "cut"]
+
+
 class Stats(Structure):
     _fields_ = [("nlevels", c_int32), ("coarsest_n", c_int32), ("cut", c_int64),
                 ("cut_tracked", c_int64), ("init_cut", c_int64), ("max_part_wgt", c_int64),
                 ("total_vwgt", c_int64), ("cap", c_int64), ("imbalance", c_double),
                 ("t_coarsen_ms", c_double), ("t_init_ms", c_double), ("t_refine_ms", c_double),
                 ("t_total_ms", c_double), ("alg_bytes", c_int64),
-                ("levels", LevelStats * GK_MAX_LEVELS)]
+                ("levels", LevelStats * GK_MAX_LEVELS),
+                ("stage_ms", c_double * GK_ST_COUNT), ("stage_launches", c_int32 * GK_ST_COUNT),
+                ("stage_bytes", c_int64 * GK_ST_COUNT)]
+
+    def stages(self):
+        return {name: (self.stage_ms[i], self.stage_launches[i], self.stage_bytes[i])
+                for i, name in enumerate(STAGE_NAMES)}
 
     def level_list(self):
         return [self.levels[i] for i in range(self.nlevels)]
@@ -140,6 +152,26 @@ def lib():
                                           POINTER(Stats), c_void_p]
         L.gk_initpart.argtypes = [POINTER(Csr), c_int32, c_int64, c_int32, c_int32, I32P,
                                   POINTER(c_int64)]
+        VP = c_void_p
+        L.gk_k_match.argtypes = [VP, POINTER(Csr), I32P, VP]
+        L.gk_k_group.argtypes = [VP, c_int32, I32P, I32P, I32P, I32P, VP]
+        L.gk_k_cluster.argtypes = [VP, c_int32, I32P, I32P, I32P, c_int32, I32P, I32P, I32P, I32P, VP]
+        L.gk_k_contract.argtypes = [VP, POINTER(Csr), I32P, c_int32, I32P, I32P, I32P, I32P, VP]
+        L.gk_k_project.argtypes = [VP, c_int32, I32P, I32P, I32P, I32P, I32P, I32P, I32P, I32P, VP]
+        L.gk_k_gain.argtypes = [VP, POINTER(Csr), c_int32, I32P, I32P, I32P, I32P, I32P, VP]
+        L.gk_k_select.argtypes = [VP, POINTER(Csr), I32P, I32P, I32P, I32P, I32P, c_int64, c_int32,
+                                  I32P, I32P, I32P, VP]
+        L.gk_k_balance.argtypes = [VP, POINTER(Csr), c_int32, I32P, c_int32, I32P, I32P, I32P, c_int64,
+                                   I32P, VP]
+        L.gk_k_apply.argtypes = [VP, POINTER(Csr), I32P, c_int32, I32P, I32P, I32P, I32P,
+                                 POINTER(c_int64), VP]
+        L.gk_k_cut.argtypes = [VP, POINTER(Csr), c_int32, I32P, POINTER(c_int64), POINTER(c_int64), VP]
+        L.gk_graph_read.argtypes = [c_char_p, POINTER(Csr)]
+        L.gk_graph_write.argtypes = [c_char_p, POINTER(Csr)]
+        L.gk_graph_check.argtypes = [POINTER(Csr), c_int]
+        L.gk_graph_free.argtypes = [POINTER(Csr)]
+        L.gk_write_partition.argtypes = [c_char_p, I32P, c_int32]
+        L.gk_partition_file.argtypes = [VP, c_char_p, c_char_p, c_int32, POINTER(Stats)]
         _LIB = L
     return _LIB
 

@@ -127,14 +127,89 @@ uint64_t fnv_host(const std::vector<int32_t>& a, uint64_t h = 0xcbf29ce484222325
 
 }  // namespace
 
+// Per-stage device timing: each timed call is bracketed by two HIP events
+// recorded on the work stream; elapsed times are summed at the end of a run.
+class StageClock {
+ public:
+  ~StageClock() {
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
+  void start(bool on, hipStream_t st) {
+    on_ = on;
+    st_ = st;
+    for (auto& v : recs_) v.clear();
+    used_ = 0;
+  }
+  void begin(int id) {
+    if (!on_) return;
+    cur_[id] = take();
+    (void)hipEventRecord(cur_[id], st_);
+  }
+  void end(int id) {
+    if (!on_) return;
+    hipEvent_t e = take();
+    (void)hipEventRecord(e, st_);
+    recs_[id].emplace_back(cur_[id], e);
+  }
+  int collect(gk_stats& S) {
+    if (!on_) return GK_OK;
+    GK_HIP(hipStreamSynchronize(st_));
+    for (int id = 0; id < GK_ST_COUNT; ++id) {
+      double ms = 0;
+      for (auto& pr : recs_[id]) {
+        float t = 0;
+        GK_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
+        ms += t;
+      }
+      S.stage_ms[id] = ms;
+    }
+    return GK_OK;
+  }
+
+ private:
+  hipEvent_t take() {
+    if (used_ == pool_.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool_.push_back(e);
+    }
+    return pool_[used_++];
+  }
+  bool on_ = false;
+  hipStream_t st_ = nullptr;
+  std::vector<hipEvent_t> pool_;
+  size_t used_ = 0;
+  hipEvent_t cur_[GK_ST_COUNT] = {};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> recs_[GK_ST_COUNT];
+};
+
 struct gk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   Pool pool;
   char* hpin = nullptr;  // pinned staging for small read-backs
+  StageClock clock;
+  gk_stats* cur = nullptr;  // stats of the running partition (stage counters)
 };
 
 namespace {
+
+// Stage bracket: times `expr` when stage timing is on, counts the call and
+// adds the stage's algorithmic bytes.
+#define GK_STAGE(ctx, id, bytes, expr)                \
+  do {                                                \
+    (ctx)->clock.begin(id);                           \
+    int rc_s_ = (expr);                               \
+    (ctx)->clock.end(id);                             \
+    if (rc_s_) return rc_s_;                          \
+    count_stage(ctx, id, (int64_t)(bytes));           \
+  } while (0)
+
+inline void count_stage(gk_ctx* ctx, int id, int64_t bytes) {
+  if (!ctx->cur) return;
+  ctx->cur->stage_launches[id] += 1;
+  ctx->cur->stage_bytes[id] += bytes;
+}
 
 // Copy small device values to host variables through pinned staging, then
 // synchronise the stream once.
@@ -187,7 +262,7 @@ int group_stage(gk_ctx* ctx, hipStream_t st, int32_t n, const int32_t* cand, int
   GK_TRY(flags.alloc(P, kMaxJumpRounds + 8));
   GK_TRY(compmax.alloc(P, n));
   gkd::fill<int32_t>(flags.p, 0, kMaxJumpRounds + 8, st);
-  GK_TRY(gkd::launch_jump_init(n, cand, nx[0].p, ds[0].p, st));
+  GK_STAGE(ctx, GK_ST_GROUP, 16LL * n, gkd::launch_jump_init(n, cand, nx[0].p, ds[0].p, st));
   int32_t f = -1;
   int32_t r = 0;
   constexpr int kBatch = 4;
@@ -196,11 +271,13 @@ int group_stage(gk_ctx* ctx, hipStream_t st, int32_t n, const int32_t* cand, int
       gkh::set_error("candidate graph did not reduce to pairs (asymmetric input?)");
       return GK_ERR_GRAPH;
     }
+    ctx->clock.begin(GK_ST_GROUP);
     for (int i = 0; i < kBatch && r + i < kMaxJumpRounds; ++i) {
       const int rr = r + i;
       GK_TRY(gkd::launch_jump_round(n, cand, nx[rr & 1].p, ds[rr & 1].p, nx[(rr + 1) & 1].p,
                                     ds[(rr + 1) & 1].p, flags.p, rr, st));
     }
+    ctx->clock.end(GK_ST_GROUP);
     int32_t hf[kBatch];
     const int cnt = std::min(kBatch, kMaxJumpRounds - r);
     GK_TRY(fetch(ctx, st, {{hf, flags.p + r, sizeof(int32_t) * cnt}}));
@@ -210,7 +287,9 @@ int group_stage(gk_ctx* ctx, hipStream_t st, int32_t n, const int32_t* cand, int
   }
   // Round f moved nothing: buffers f&1 and (f+1)&1 agree.
   int32_t* next = nx[f & 1].p;
-  GK_TRY(gkd::launch_label(n, cand, next, compmax.p, label, st));
+  // Rounds 0..f ran in full (24 bytes per vertex each); later ones exit early.
+  count_stage(ctx, GK_ST_GROUP, 24LL * n * (f + 1));
+  GK_STAGE(ctx, GK_ST_GROUP, 32LL * n, gkd::launch_label(n, cand, next, compmax.p, label, st));
   dist_out = std::move(ds[f & 1]);
   *rounds = f;
   return GK_OK;
@@ -235,14 +314,18 @@ int cluster_stage(gk_ctx* ctx, hipStream_t st, int32_t n, const int32_t* label, 
   GK_TRY(flagscan.alloc(P, n));
   GK_TRY(ncd.alloc(P, 1));
   GK_TRY(cptr.alloc(P, int64_t(n) + 1));
-  GK_TRY(gkd::launch_make_keys(n, label, dist, sbits, keys.p, ids.p, st));
   bool alt = false;
-  if (gkd::radix_sort_pairs<uint64_t>(keys.p, ids.p, keys2.p, ids2.p, n, 0, sbits + lbits, scratch.p, st, &alt))
-    GK_HIP(hipGetLastError());
+  const int passes = (sbits + lbits + 7) / 8;
+  GK_STAGE(ctx, GK_ST_SORT, 20LL * n + 32LL * n * passes,
+           (gkd::launch_make_keys(n, label, dist, sbits, keys.p, ids.p, st) ||
+                    gkd::radix_sort_pairs<uint64_t>(keys.p, ids.p, keys2.p, ids2.p, n, 0, sbits + lbits, scratch.p, st, &alt)
+                ? GK_ERR_HIP
+                : GK_OK));
   uint64_t* sk = alt ? keys2.p : keys.p;
   Buf<int32_t>& sid = alt ? ids2 : ids;
-  GK_TRY(gkd::launch_cluster(n, sk, sid.p, sbits, nullptr, G, gstart.p, flagscan.p, cmap, cptr.p,
-                             nullptr, ncd.p, scratch.p, st));
+  GK_STAGE(ctx, GK_ST_CLUSTER, 64LL * n,
+           gkd::launch_cluster(n, sk, sid.p, sbits, nullptr, G, gstart.p, flagscan.p, cmap, cptr.p,
+                               nullptr, ncd.p, scratch.p, st));
   GK_TRY(fetch(ctx, st, {{nc, ncd.p, sizeof(int32_t)}}));
   order = std::move(sid);
   return GK_OK;
@@ -258,16 +341,19 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
   GK_TRY(lists.alloc(P, int64_t(4) * nc));
   GK_TRY(lcnt.alloc(P, 4));
   GK_TRY(scratch.alloc(P, gkd::scan_scratch_ints(int64_t(nc) + 1) + 64));
-  GK_TRY(gkd::launch_contract_bounds(g, nc, cptr, order, ub.p, toff.p, lists.p, lcnt.p, scratch.p, st));
+  GK_STAGE(ctx, GK_ST_CONTRACT, 12LL * g.n + 28LL * nc,
+           gkd::launch_contract_bounds(g, nc, cptr, order, ub.p, toff.p, lists.p, lcnt.p, scratch.p, st));
   int32_t hc[4];
   GK_TRY(fetch(ctx, st, {{hc, lcnt.p, sizeof hc}}));
   GK_TRY(tadj.alloc(P, g.nnz));
   GK_TRY(tw.alloc(P, g.nnz));
   GK_TRY(len.alloc(P, nc));
   const int lanes[3] = {16, 32, 64};
+  ctx->clock.begin(GK_ST_CONTRACT);
   for (int cls = 0; cls < 3; ++cls)
     GK_TRY(gkd::launch_contract_rows(g, lanes[cls], lists.p + int64_t(cls) * nc, hc[cls], cptr, order,
                                      cmap, toff.p, tadj.p, tw.p, len.p, st));
+  ctx->clock.end(GK_ST_CONTRACT);
   if (hc[3] > 0) {
     const int32_t nbig = hc[3];
     const int32_t* rows = lists.p + int64_t(3) * nc;
@@ -275,7 +361,8 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
     GK_TRY(boff.alloc(P, int64_t(nbig) + 1));
     GK_TRY(btot.alloc(P, 1));
     GK_TRY(bscr.alloc(P, gkd::scan_scratch_ints(int64_t(nbig) + 1) + 64));
-    GK_TRY(gkd::launch_big_offsets(rows, nbig, ub.p, boff.p, btot.p, bscr.p, st));
+    GK_STAGE(ctx, GK_ST_CONTRACT_BIG, 16LL * nbig,
+             gkd::launch_big_offsets(rows, nbig, ub.p, boff.p, btot.p, bscr.p, st));
     int32_t eb = 0;
     GK_TRY(fetch(ctx, st, {{&eb, btot.p, sizeof eb}}));
     Buf<uint64_t> k1, k2;
@@ -285,21 +372,28 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
     GK_TRY(v1.alloc(P, eb));
     GK_TRY(v2.alloc(P, eb));
     GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(eb)));
-    GK_TRY(gkd::launch_contract_big_gather(g, rows, nbig, boff.p, cptr, order, cmap, k1.p, v1.p, st));
     bool alt = false;
-    if (gkd::radix_sort_pairs<uint64_t>(k1.p, v1.p, k2.p, v2.p, eb, 0, 32 + bits_for(nbig - 1), sscr.p, st, &alt))
-      GK_HIP(hipGetLastError());
-    GK_TRY(gkd::launch_contract_big_merge(rows, nbig, boff.p, alt ? k2.p : k1.p, alt ? v2.p : v1.p,
-                                          toff.p, tadj.p, tw.p, len.p, st));
+    const int bpasses = (32 + bits_for(nbig - 1) + 7) / 8;
+    GK_STAGE(ctx, GK_ST_CONTRACT_BIG, 24LL * eb + 36LL * eb * bpasses + 24LL * eb,
+             (gkd::launch_contract_big_gather(g, rows, nbig, boff.p, cptr, order, cmap, k1.p, v1.p, st) ||
+                      gkd::radix_sort_pairs<uint64_t>(k1.p, v1.p, k2.p, v2.p, eb, 0, 32 + bits_for(nbig - 1),
+                                                      sscr.p, st, &alt) ||
+                      gkd::launch_contract_big_merge(rows, nbig, boff.p, alt ? k2.p : k1.p, alt ? v2.p : v1.p,
+                                                     toff.p, tadj.p, tw.p, len.p, st)
+                  ? GK_ERR_HIP
+                  : GK_OK));
   }
   GK_TRY(cnnzd.alloc(P, 1));
   GK_TRY(c.xadj.alloc(P, int64_t(nc) + 1));
-  GK_TRY(gkd::launch_len_scan(nc, len.p, c.xadj.p, cnnzd.p, scratch.p, st));
+  GK_STAGE(ctx, GK_ST_CONTRACT, 12LL * nc, gkd::launch_len_scan(nc, len.p, c.xadj.p, cnnzd.p, scratch.p, st));
   int32_t cnnz = 0;
   GK_TRY(fetch(ctx, st, {{&cnnz, cnnzd.p, sizeof cnnz}}));
   GK_TRY(c.adj.alloc(P, cnnz));
   GK_TRY(c.w.alloc(P, cnnz));
-  GK_TRY(gkd::launch_contract_copy(nc, toff.p, len.p, c.xadj.p, tadj.p, tw.p, c.adj.p, c.w.p, st));
+  // Row bytes (member rows read once, merged rows written twice, read once)
+  // are counted with the copy, when the coarse entry count is known.
+  GK_STAGE(ctx, GK_ST_CONTRACT, 12LL * nc + 16LL * cnnz + 12LL * g.n + 20LL * nc + 12LL * g.nnz + 8LL * cnnz,
+           gkd::launch_contract_copy(nc, toff.p, len.p, c.xadj.p, tadj.p, tw.p, c.adj.p, c.w.p, st));
   c.g.n = nc;
   c.g.nnz = cnnz;
   c.g.xadj = c.xadj.p;
@@ -325,6 +419,12 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   const bool trace = (prm.flags & GK_FLAG_TRACE) != 0;
   gk_stats S;
   std::memset(&S, 0, sizeof S);
+  ctx->cur = &S;
+  ctx->clock.start((prm.flags & GK_FLAG_STAGE_TIME) != 0, st);
+  struct Reset {
+    gk_ctx* c;
+    ~Reset() { c->cur = nullptr; c->clock.start(false, nullptr); }
+  } reset{ctx};
 
   std::vector<Level> L;
   L.reserve(GK_MAX_LEVELS + 1);
@@ -367,7 +467,8 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(cand.alloc(P, n));
     GK_TRY(label.alloc(P, n));
     GK_TRY(cmap.alloc(P, n));
-    GK_TRY(gkd::launch_match(f.g, gkd::lanes_for_degree(n ? double(f.g.nnz) / n : 0.0), cand.p, st));
+    GK_STAGE(ctx, GK_ST_MATCH, 16LL * f.g.nnz + 8LL * n + 4,
+             gkd::launch_match(f.g, gkd::lanes_for_degree(n ? double(f.g.nnz) / n : 0.0), cand.p, st));
     int32_t rounds = 0;
     GK_TRY(group_stage(ctx, st, n, cand.p, label.p, dist, &rounds));
     int32_t nc = 0;
@@ -376,7 +477,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     if (prm.coarsen_policy != GK_COARSEN_REF && nc < floor_n) break;
     Level c;
     GK_TRY(c.vw.alloc(P, nc));
-    GK_TRY(gkd::launch_cvwgt(nc, cptr.p, order.p, f.g.vw, c.vw.p, st));
+    GK_STAGE(ctx, GK_ST_CLUSTER, 8LL * n + 12LL * nc, gkd::launch_cvwgt(nc, cptr.p, order.p, f.g.vw, c.vw.p, st));
     c.g.vw = c.vw.p;
     GK_TRY(contract_stage(ctx, st, f.g, nc, cmap.p, cptr.p, order.p, c));
     ls.sweeps = rounds;
@@ -449,6 +550,9 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   if (nl > 1) {
     const int32_t M = prm.max_moves;
     Buf<int32_t> gain, dest, upd, heavy, hcnt, moves, ctl, sscr;
+    Buf<unsigned long long> work;
+    GK_TRY(work.alloc(P, 8));
+    gkd::fill<unsigned long long>(work.p, 0ull, 8, st);
     Buf<uint64_t> keys, keys2;
     GK_TRY(gain.alloc(P, n0));
     GK_TRY(dest.alloc(P, n0));
@@ -466,21 +570,24 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       Level& f = L[l];
       const DGraph& g = f.g;
       gk_level_stats& ls = S.levels[l];
-      GK_TRY(gkd::launch_project(g.n, f.cmap.p, cur_part, cur_bnd, oth_part, oth_bnd, gain.p, dest.p,
-                                 upd.p, st));
+      GK_STAGE(ctx, GK_ST_PROJECT, 32LL * g.n,
+               gkd::launch_project(g.n, f.cmap.p, cur_part, cur_bnd, oth_part, oth_bnd, gain.p, dest.p,
+                                   upd.p, st));
       std::swap(cur_part, oth_part);
       std::swap(cur_bnd, oth_bnd);
       const int lanes = gain_lanes(g);
-      GK_TRY(gkd::launch_heavy_list(g, lanes, heavy.p, hcnt.p, st));
+      GK_STAGE(ctx, GK_ST_GAIN, 8LL * g.n, gkd::launch_heavy_list(g, lanes, heavy.p, hcnt.p, st));
       int32_t nheavy = 0;
       GK_TRY(fetch(ctx, st, {{&nheavy, hcnt.p, sizeof nheavy}}));
-      GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, st));
+      GK_STAGE(ctx, GK_ST_GAIN, 4LL * g.n + 4LL * nheavy,
+               gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
       const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
       int64_t moved = 0;
       int32_t rounds = 0;
       bool pending = false;  // a round whose pos has not been read yet
       while (true) {
-        GK_TRY(gkd::launch_select(g, cur_bnd, gain.p, dest.p, pw.p, cap, idbits, upd.p, keys.p, ctl.p, st));
+        GK_STAGE(ctx, GK_ST_SELECT, 12LL * g.n,
+                 gkd::launch_select(g, cur_bnd, gain.p, dest.p, pw.p, cap, idbits, upd.p, keys.p, ctl.p, work.p, st));
         int32_t hc3[3];
         GK_TRY(fetch(ctx, st, {{hc3, ctl.p, sizeof hc3}}));
         if (pending) {
@@ -489,22 +596,30 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
           pending = false;
         }
         const int32_t count = hc3[0];
+        S.stage_bytes[GK_ST_SELECT] += 8LL * count;  // keys written by the selection
         if (count == 0) break;
         ++rounds;
         bool presorted = false;
         const uint64_t* kk = keys.p;
         if (count > 1024) {
           bool alt = false;
-          if (gkd::radix_sort_pairs<uint64_t>(keys.p, nullptr, keys2.p, nullptr, count, 0,
-                                              idbits + bits_for(hc3[1]), sscr.p, st, &alt))
-            GK_HIP(hipGetLastError());
+          const int mp = (idbits + bits_for(hc3[1]) + 7) / 8;
+          GK_STAGE(ctx, GK_ST_MSORT, 24LL * count * mp,
+                   gkd::radix_sort_pairs<uint64_t>(keys.p, nullptr, keys2.p, nullptr, count, 0,
+                                                   idbits + bits_for(hc3[1]), sscr.p, st, &alt)
+                       ? GK_ERR_HIP
+                       : GK_OK);
           kk = alt ? keys2.p : keys.p;
           presorted = true;
         }
-        GK_TRY(gkd::launch_balance(g, k, kk, count, M, presorted, idbits, cur_part, dest.p, gain.p, pw.p,
-                                   cap, moves.p, ctl.p, dcut.p, st));
-        GK_TRY(gkd::launch_apply(g, M, moves.p, ctl.p, dest.p, cur_part, upd.p, st));
-        GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, st));
+        const int32_t B = std::min(count, M);
+        GK_STAGE(ctx, GK_ST_BALANCE, 24LL * B + 12LL * k,
+                 gkd::launch_balance(g, k, kk, count, M, presorted, idbits, cur_part, dest.p, gain.p, pw.p,
+                                     cap, moves.p, ctl.p, dcut.p, st));
+        GK_STAGE(ctx, GK_ST_APPLY, (24LL + 8LL * (g.nnz / std::max(1, g.n))) * B,
+                 gkd::launch_apply(g, M, moves.p, ctl.p, dest.p, cur_part, upd.p, st));
+        GK_STAGE(ctx, GK_ST_GAIN, 4LL * g.n + 4LL * nheavy,
+                 gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
         pending = true;
       }
       long long hcut = 0;
@@ -518,6 +633,11 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
         ls.h_part = fnv_host(h);
       }
     }
+    // Data-dependent bytes of gain and select from the device work counters.
+    unsigned long long wk[8];
+    GK_TRY(fetch(ctx, st, {{wk, work.p, sizeof wk}}));
+    S.stage_bytes[GK_ST_GAIN] += int64_t(24 * wk[0] + 12 * wk[1]);
+    S.stage_bytes[GK_ST_SELECT] += int64_t(12 * wk[2] + 8 * wk[3] + 20 * wk[4]);
   }
   GK_HIP(hipStreamSynchronize(st));
   const auto t3 = clk::now();
@@ -527,7 +647,8 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     Buf<unsigned long long> cut, pw64;
     GK_TRY(cut.alloc(P, 1));
     GK_TRY(pw64.alloc(P, k));
-    GK_TRY(gkd::launch_cut(L[0].g, k, cur_part, cut.p, pw64.p, st));
+    GK_STAGE(ctx, GK_ST_CUT, 16LL * n0 + 12LL * L[0].g.nnz,
+             gkd::launch_cut(L[0].g, k, cur_part, cut.p, pw64.p, st));
     std::vector<unsigned long long> hpw64;
     unsigned long long hcut = 0;
     long long tracked = 0;
@@ -555,7 +676,9 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   S.t_coarsen_ms = ms(t0, t1);
   S.t_init_ms = ms(t1, t2);
   S.t_refine_ms = ms(t2, t3);
+  GK_TRY(ctx->clock.collect(S));
   S.t_total_ms = ms(t0, clk::now());
+  for (int i = 0; i < GK_ST_COUNT; ++i) S.alg_bytes += S.stage_bytes[i];
   if (stats) *stats = S;
   if (S.cut != S.cut_tracked) {
     gkh::set_error("tracked cut " + std::to_string(S.cut_tracked) + " != recomputed cut " +
@@ -815,7 +938,7 @@ int gk_k_gain(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, con
   GK_TRY(gkd::launch_heavy_list(g, lanes, heavy.p, hcnt.p, st));
   int32_t nheavy = 0;
   GK_TRY(fetch(ctx, st, {{&nheavy, hcnt.p, sizeof nheavy}}));
-  GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, part, upd, bnd, gain, dest, st));
+  GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, part, upd, bnd, gain, dest, nullptr, st));
   GK_HIP(hipStreamSynchronize(st));
   return GK_OK;
 }
@@ -836,7 +959,7 @@ int gk_k_select(gk_ctx* ctx, const gk_csr* dg, const int32_t* part, const int32_
   GK_TRY(ctl.alloc(P, 4));
   GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(g.n)));
   const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
-  GK_TRY(gkd::launch_select(g, bnd, gain, dest, pw, cap, idbits, upd, keys.p, ctl.p, st));
+  GK_TRY(gkd::launch_select(g, bnd, gain, dest, pw, cap, idbits, upd, keys.p, ctl.p, nullptr, st));
   int32_t hc[2];
   GK_TRY(fetch(ctx, st, {{hc, ctl.p, sizeof hc}}));
   *nsel = hc[0];

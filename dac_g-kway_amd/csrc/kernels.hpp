@@ -75,14 +75,16 @@ int launch_project(int32_t n, const int32_t* cmap, const int32_t* cpart, const i
                    hipStream_t st);
 // Heavy list: vertices whose degree exceeds `lanes` (compacted, any order).
 int launch_heavy_list(const DGraph& g, int lanes, int32_t* list, int32_t* cnt, hipStream_t st);
+// work (may be null) accumulates [0] vertices recomputed, [1] their entries.
 int launch_gain(const DGraph& g, int32_t k, int lanes, const int32_t* heavy, int32_t nheavy,
                 const int32_t* part, const int32_t* upd, int32_t* bnd, int32_t* gain,
-                int32_t* dest, hipStream_t st);
+                int32_t* dest, unsigned long long* work, hipStream_t st);
 // Selection: clears upd, appends keys ((~gain & 0x7fffffff) << idbits | v)
-// of the selected vertices; ctl[0] = count, ctl[1] = max gain.
+// of the selected vertices; ctl[0] = count, ctl[1] = max gain.  work (may be
+// null) accumulates [2] candidates, [3] rows walked, [4] entries scanned.
 int launch_select(const DGraph& g, const int32_t* bnd, const int32_t* gain, const int32_t* dest,
                   const int32_t* pw, int64_t cap, int idbits, int32_t* upd, uint64_t* keys,
-                  int32_t* ctl, hipStream_t st);
+                  int32_t* ctl, unsigned long long* work, hipStream_t st);
 // Balance: first min(count, M) keys (sorted when presorted, else sorted in
 // LDS); writes moves[0..pos], ctl[2] = pos; advances pw; cut -= gains.
 int launch_balance(const DGraph& g, int32_t k, const uint64_t* keys, int32_t count, int32_t M,

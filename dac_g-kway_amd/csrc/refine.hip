@@ -74,46 +74,63 @@ __global__ void __launch_bounds__(256) k_gain_small(int32_t n, const int32_t* __
                                                     const int32_t* __restrict__ part,
                                                     const int32_t* __restrict__ upd,
                                                     int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
-                                                    int32_t* __restrict__ dest) {
+                                                    int32_t* __restrict__ dest,
+                                                    unsigned long long* __restrict__ work) {
   const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
   const int32_t v = static_cast<int32_t>(gt / S);
   const int sub = static_cast<int>(gt % S);
-  if (v >= n) return;
-  if (!upd[v]) return;
-  const int32_t s = xadj[v], d = xadj[v + 1] - s;
-  if (d > S) return;  // heavy path
-  const int32_t p = part[v];
-  int32_t q = -1, wj = 0;
-  if (sub < d) {
-    q = part[adj[s + sub]];
-    wj = w[s + sub];
+  // No early return: every lane reaches the wave-wide work count below.
+  int32_t d = 0;
+  bool act = v < n && upd[v] != 0;
+  int32_t s = 0;
+  if (act) {
+    s = xadj[v];
+    d = xadj[v + 1] - s;
+    act = d <= S;  // longer rows take the heavy path
   }
-  int32_t conn = 0, id = 0;
+  if (act) {
+    const int32_t p = part[v];
+    int32_t q = -1, wj = 0;
+    if (sub < d) {
+      q = part[adj[s + sub]];
+      wj = w[s + sub];
+    }
+    int32_t conn = 0, id = 0;
 #pragma unroll
-  for (int i = 0; i < S; ++i) {
-    const int32_t qi = __shfl(q, i, S);
-    const int32_t wi = __shfl(wj, i, S);
-    conn += (qi == q) ? wi : 0;
-    id += (qi == p) ? wi : 0;
-  }
-  const bool ext = q >= 0 && q != p;
-  int32_t g = ext ? conn - id : INT32_MIN;
-  int32_t bq = ext ? q : INT32_MAX;
-  int32_t anyext = ext ? 1 : 0;
+    for (int i = 0; i < S; ++i) {
+      const int32_t qi = __shfl(q, i, S);
+      const int32_t wi = __shfl(wj, i, S);
+      conn += (qi == q) ? wi : 0;
+      id += (qi == p) ? wi : 0;
+    }
+    const bool ext = q >= 0 && q != p;
+    int32_t g = ext ? conn - id : INT32_MIN;
+    int32_t bq = ext ? q : INT32_MAX;
+    int32_t anyext = ext ? 1 : 0;
 #pragma unroll
-  for (int o = S / 2; o > 0; o >>= 1) {
-    const int32_t og = __shfl_xor(g, o, S), oq = __shfl_xor(bq, o, S);
-    if (og > g || (og == g && oq < bq)) { g = og; bq = oq; }
-    anyext |= __shfl_xor(anyext, o, S);
+    for (int o = S / 2; o > 0; o >>= 1) {
+      const int32_t og = __shfl_xor(g, o, S), oq = __shfl_xor(bq, o, S);
+      if (og > g || (og == g && oq < bq)) { g = og; bq = oq; }
+      anyext |= __shfl_xor(anyext, o, S);
+    }
+    if (sub == 0) {
+      bnd[v] = anyext;
+      if (anyext && g > 0) {
+        gain[v] = g;
+        dest[v] = bq;
+      } else {
+        gain[v] = 0;
+        dest[v] = p;
+      }
+    }
   }
-  if (sub == 0) {
-    bnd[v] = anyext;
-    if (anyext && g > 0) {
-      gain[v] = g;
-      dest[v] = bq;
-    } else {
-      gain[v] = 0;
-      dest[v] = p;
+  if (work) {
+    const bool head = act && sub == 0;
+    const uint64_t m = ballot(head);
+    const int dsum = wave_sum(head ? d : 0);
+    if (m && lane_id() == 0) {
+      atomicAdd(&work[0], (unsigned long long)popc64(m));
+      atomicAdd(&work[1], (unsigned long long)dsum);
     }
   }
 }
@@ -127,7 +144,8 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
                                                    const int32_t* __restrict__ part,
                                                    const int32_t* __restrict__ upd,
                                                    int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
-                                                   int32_t* __restrict__ dest) {
+                                                   int32_t* __restrict__ dest,
+                                                   unsigned long long* __restrict__ work) {
   extern __shared__ int32_t hist[];
   const int lane = lane_id();
   for (int32_t hi = blockIdx.x; hi < nheavy; hi += gridDim.x) {
@@ -163,6 +181,10 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
       bnd[v] = any ? 1 : 0;
       if (any && g > 0) { gain[v] = g; dest[v] = bq; }
       else { gain[v] = 0; dest[v] = p; }
+      if (work) {
+        atomicAdd(&work[0], 1ull);
+        atomicAdd(&work[1], (unsigned long long)(e - s));
+      }
     }
     __syncthreads();
   }
@@ -170,19 +192,19 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
 
 int launch_gain(const DGraph& g, int32_t k, int lanes, const int32_t* heavy, int32_t nheavy,
                 const int32_t* part, const int32_t* upd, int32_t* bnd, int32_t* gain,
-                int32_t* dest, hipStream_t st) {
+                int32_t* dest, unsigned long long* work, hipStream_t st) {
   if (g.n <= 0) return 0;
   const unsigned grid = grid_for(int64_t(g.n) * lanes, 256);
   switch (lanes) {
-    case 4: k_gain_small<4><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
-    case 8: k_gain_small<8><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
-    case 16: k_gain_small<16><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
-    case 32: k_gain_small<32><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
-    default: k_gain_small<64><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest); break;
+    case 4: k_gain_small<4><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
+    case 8: k_gain_small<8><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
+    case 16: k_gain_small<16><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
+    case 32: k_gain_small<32><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
+    default: k_gain_small<64><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
   }
   if (nheavy > 0) {
     const unsigned hg = static_cast<unsigned>(nheavy < 4096 ? nheavy : 4096);
-    k_gain_heavy<<<hg, 64, sizeof(int32_t) * k, st>>>(heavy, nheavy, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest);
+    k_gain_heavy<<<hg, 64, sizeof(int32_t) * k, st>>>(heavy, nheavy, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work);
   }
   return hip_status();
 }
@@ -200,41 +222,56 @@ __global__ void __launch_bounds__(256) k_select(int32_t n, const int32_t* __rest
                                                 const int32_t* __restrict__ dest,
                                                 const int32_t* __restrict__ pw, long long cap, int idbits,
                                                 int32_t* __restrict__ upd, uint64_t* __restrict__ keys,
-                                                int32_t* __restrict__ ctl) {
+                                                int32_t* __restrict__ ctl,
+                                                unsigned long long* __restrict__ work) {
   const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  bool sel = false;
-  int32_t g = 0;
+  bool sel = false, cand = false, walked = false;
+  int32_t g = 0, scanned = 0;
   if (v < n) {
     upd[v] = 0;
     g = gain[v];
-    if (bnd[v] && g > 0 && (long long)pw[dest[v]] + vw[v] <= cap) {
+    cand = bnd[v] && g > 0;
+    if (cand && (long long)pw[dest[v]] + vw[v] <= cap) {
       sel = true;
+      walked = true;
       for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
+        ++scanned;
         const int32_t u = adj[j];
         if (u < v && gain[u] > 0 && (long long)pw[dest[u]] + vw[u] <= cap) { sel = false; break; }
       }
     }
   }
+  if (work) {
+    const int c0 = popc64(ballot(cand)), c1 = popc64(ballot(walked));
+    const int c2 = wave_sum(scanned);
+    if (lane_id() == 0 && (c0 | c1 | c2)) {
+      atomicAdd(&work[2], (unsigned long long)c0);
+      atomicAdd(&work[3], (unsigned long long)c1);
+      atomicAdd(&work[4], (unsigned long long)c2);
+    }
+  }
   const uint64_t m = ballot(sel);
   if (m == 0) return;
   const int leader = __ffsll(static_cast<long long>(m)) - 1;
+  const int gmax = wave_max(sel ? g : 0);
   int base = 0;
-  if (lane_id() == leader) base = atomicAdd(&ctl[0], popc64(m));
+  if (lane_id() == leader) {
+    base = atomicAdd(&ctl[0], popc64(m));
+    atomicMax(&ctl[1], gmax);
+  }
   base = __shfl(base, leader, 64);
-  if (sel) {
+  if (sel)
     keys[base + popc64(m & lanemask_lt())] =
         (uint64_t(uint32_t(~g) & 0x7fffffffu) << idbits) | uint64_t(uint32_t(v));
-    atomicMax(&ctl[1], g);
-  }
 }
 
 int launch_select(const DGraph& g, const int32_t* bnd, const int32_t* gain, const int32_t* dest,
                   const int32_t* pw, int64_t cap, int idbits, int32_t* upd, uint64_t* keys,
-                  int32_t* ctl, hipStream_t st) {
+                  int32_t* ctl, unsigned long long* work, hipStream_t st) {
   fill<int32_t>(ctl, 0, 2, st);
   if (g.n <= 0) return 0;
   k_select<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, bnd, gain, dest, pw, cap,
-                                               idbits, upd, keys, ctl);
+                                               idbits, upd, keys, ctl, work);
   return hip_status();
 }
 

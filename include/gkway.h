@@ -85,6 +85,24 @@ typedef struct gk_params {
 
 #define GK_FLAG_CHECK 1      /* validate input symmetry on entry             */
 #define GK_FLAG_TRACE 2      /* record per-level hashes in gk_stats.levels   */
+#define GK_FLAG_STAGE_TIME 4 /* bracket every launch with HIP events on the
+                                work stream; fill gk_stats.stage_*          */
+
+/* Stage ids of gk_stats.stage_ms / stage_launches / stage_bytes. */
+#define GK_ST_MATCH 0      /* candidate matching                           */
+#define GK_ST_GROUP 1      /* pointer jumping + component labels           */
+#define GK_ST_SORT 2       /* radix sort by (label, stamp, id)             */
+#define GK_ST_CLUSTER 3    /* chunking, coarse ids, coarse vertex weights  */
+#define GK_ST_CONTRACT 4   /* row bounds, row merge, copy                  */
+#define GK_ST_CONTRACT_BIG 5 /* gather + sort + merge of rows > 64 entries */
+#define GK_ST_PROJECT 6
+#define GK_ST_GAIN 7
+#define GK_ST_SELECT 8
+#define GK_ST_MSORT 9      /* radix sort of > 1024 selected moves          */
+#define GK_ST_BALANCE 10
+#define GK_ST_APPLY 11
+#define GK_ST_CUT 12       /* final cut and part weights                   */
+#define GK_ST_COUNT 16
 
 /* Fill *p with the defaults: ubfactor 30, max_group 6, max_moves 1024,
  * TARGET policy, seed 1. k is left as given. */
@@ -116,8 +134,13 @@ typedef struct gk_stats {
   int64_t cap;             /* maximum allowed part weight                      */
   double imbalance;        /* max part weight / (total / k)                    */
   double t_coarsen_ms, t_init_ms, t_refine_ms, t_total_ms;
-  int64_t alg_bytes;       /* algorithmic HBM bytes of the run (see DESIGN)   */
+  int64_t alg_bytes;       /* algorithmic HBM bytes of all stages             */
   gk_level_stats levels[GK_MAX_LEVELS];
+  /* Per-stage device time (GK_FLAG_STAGE_TIME), launches and algorithmic
+   * bytes (compulsory 4-byte element traffic of the stage's kernels). */
+  double stage_ms[GK_ST_COUNT];
+  int32_t stage_launches[GK_ST_COUNT];
+  int64_t stage_bytes[GK_ST_COUNT];
 } gk_stats;
 
 /* ---- context ------------------------------------------------------------ */
