@@ -255,16 +255,30 @@ __global__ void __launch_bounds__(256) k_row_bounds(int32_t nc, const int32_t* _
     ub[c] = s;
     cls = s <= 16 ? 0 : (s <= 32 ? 1 : (s <= 64 ? 2 : 3));
   }
+  // Block-aggregated append: per class, wave counts in LDS, one global
+  // atomic per block and class.
+  __shared__ int wcnt[4][4];
+  __shared__ int bbase[4];
+  const int wave = threadIdx.x >> 6;
   const uint64_t lt = lanemask_lt();
+  uint64_t m[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const uint64_t m = ballot(cls == k);
-    if (m == 0) continue;
-    const int leader = __ffsll(static_cast<long long>(m)) - 1;
-    int base = 0;
-    if (lane_id() == leader) base = atomicAdd(&cnt[k], popc64(m));
-    base = __shfl(base, leader, 64);
-    if (cls == k) lists[int64_t(k) * nc + base + popc64(m & lt)] = int32_t(c);
+    m[k] = ballot(cls == k);
+    if (lane_id() == 0) wcnt[k][wave] = popc64(m[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    const int t = wcnt[k][0] + wcnt[k][1] + wcnt[k][2] + wcnt[k][3];
+    bbase[k] = t ? atomicAdd(&cnt[k], t) : 0;
+  }
+  __syncthreads();
+  if (cls >= 0) {
+    int off = bbase[cls];
+    for (int w = 0; w < wave; ++w) off += wcnt[cls][w];
+    const uint64_t mine = cls == 0 ? m[0] : (cls == 1 ? m[1] : (cls == 2 ? m[2] : m[3]));
+    lists[int64_t(cls) * nc + off + popc64(mine & lt)] = int32_t(c);
   }
 }
 

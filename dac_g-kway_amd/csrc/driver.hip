@@ -558,12 +558,13 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(dest.alloc(P, n0));
     GK_TRY(upd.alloc(P, n0));
     GK_TRY(heavy.alloc(P, n0));
-    GK_TRY(hcnt.alloc(P, 1));
+    GK_TRY(hcnt.alloc(P, 2));  // [0] gain worklist count, [1] heavy-row count
+    Buf<int32_t> wl;
+    GK_TRY(wl.alloc(P, n0));
     GK_TRY(moves.alloc(P, M));
     GK_TRY(ctl.alloc(P, 4));
     GK_TRY(keys.alloc(P, n0));
     GK_TRY(keys2.alloc(P, n0));
-    Buf<int32_t> dummy_vals;
     GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(n0)));
     gkd::fill<int32_t>(ctl.p, -1, 4, st);
     for (int32_t l = nl - 2; l >= 0; --l) {
@@ -576,18 +577,20 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       std::swap(cur_part, oth_part);
       std::swap(cur_bnd, oth_bnd);
       const int lanes = gain_lanes(g);
-      GK_STAGE(ctx, GK_ST_GAIN, 8LL * g.n, gkd::launch_heavy_list(g, lanes, heavy.p, hcnt.p, st));
-      int32_t nheavy = 0;
-      GK_TRY(fetch(ctx, st, {{&nheavy, hcnt.p, sizeof nheavy}}));
-      GK_STAGE(ctx, GK_ST_GAIN, 4LL * g.n + 4LL * nheavy,
-               gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
+      // The projected boundary becomes the first gain worklist.
+      gkd::fill<int32_t>(hcnt.p, 0, 2, st);
+      GK_STAGE(ctx, GK_ST_PROJECT, 4LL * g.n, gkd::launch_compact(g.n, upd.p, wl.p, hcnt.p, st));
+      GK_STAGE(ctx, GK_ST_GAIN, 0,
+               gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
+                                   work.p, st));
       const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
       int64_t moved = 0;
       int32_t rounds = 0;
       bool pending = false;  // a round whose pos has not been read yet
       while (true) {
-        GK_STAGE(ctx, GK_ST_SELECT, 12LL * g.n,
-                 gkd::launch_select(g, cur_bnd, gain.p, dest.p, pw.p, cap, idbits, upd.p, keys.p, ctl.p, work.p, st));
+        GK_STAGE(ctx, GK_ST_SELECT, 8LL * g.n,
+                 gkd::launch_select(g, cur_bnd, gain.p, dest.p, pw.p, cap, idbits, upd.p, keys.p, ctl.p, work.p,
+                                    0, hcnt.p, st));
         int32_t hc3[3];
         GK_TRY(fetch(ctx, st, {{hc3, ctl.p, sizeof hc3}}));
         if (pending) {
@@ -617,9 +620,10 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
                  gkd::launch_balance(g, k, kk, count, M, presorted, idbits, cur_part, dest.p, gain.p, pw.p,
                                      cap, moves.p, ctl.p, dcut.p, st));
         GK_STAGE(ctx, GK_ST_APPLY, (24LL + 8LL * (g.nnz / std::max(1, g.n))) * B,
-                 gkd::launch_apply(g, M, moves.p, ctl.p, dest.p, cur_part, upd.p, st));
-        GK_STAGE(ctx, GK_ST_GAIN, 4LL * g.n + 4LL * nheavy,
-                 gkd::launch_gain(g, k, lanes, heavy.p, nheavy, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
+                 gkd::launch_apply(g, M, moves.p, ctl.p, dest.p, cur_part, upd.p, wl.p, hcnt.p, st));
+        GK_STAGE(ctx, GK_ST_GAIN, 0,
+                 gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
+                                     work.p, st));
         pending = true;
       }
       long long hcut = 0;
@@ -636,7 +640,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     // Data-dependent bytes of gain and select from the device work counters.
     unsigned long long wk[8];
     GK_TRY(fetch(ctx, st, {{wk, work.p, sizeof wk}}));
-    S.stage_bytes[GK_ST_GAIN] += int64_t(24 * wk[0] + 12 * wk[1]);
+    S.stage_bytes[GK_ST_GAIN] += int64_t(28 * wk[0] + 12 * wk[1]);
     S.stage_bytes[GK_ST_SELECT] += int64_t(12 * wk[2] + 8 * wk[3] + 20 * wk[4]);
   }
   GK_HIP(hipStreamSynchronize(st));
@@ -932,13 +936,16 @@ int gk_k_gain(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, con
   hipStream_t st = pick(ctx, stream);
   DGraph g = dview(dg);
   const int lanes = gain_lanes(g);
-  Buf<int32_t> heavy, hcnt;
+  Buf<int32_t> wl, heavy, cnt, flags;
+  GK_TRY(wl.alloc(ctx->pool, g.n));
   GK_TRY(heavy.alloc(ctx->pool, g.n));
-  GK_TRY(hcnt.alloc(ctx->pool, 1));
-  GK_TRY(gkd::launch_heavy_list(g, lanes, heavy.p, hcnt.p, st));
-  int32_t nheavy = 0;
-  GK_TRY(fetch(ctx, st, {{&nheavy, hcnt.p, sizeof nheavy}}));
-  GK_TRY(gkd::launch_gain(g, k, lanes, heavy.p, nheavy, part, upd, bnd, gain, dest, nullptr, st));
+  GK_TRY(cnt.alloc(ctx->pool, 2));
+  GK_TRY(flags.alloc(ctx->pool, g.n));
+  // The kernels clear the flags they consume: work on a copy.
+  if (g.n) GK_HIP(hipMemcpyAsync(flags.p, upd, sizeof(int32_t) * g.n, hipMemcpyDeviceToDevice, st));
+  gkd::fill<int32_t>(cnt.p, 0, 2, st);
+  GK_TRY(gkd::launch_compact(g.n, flags.p, wl.p, cnt.p, st));
+  GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, cnt.p, heavy.p, part, flags.p, bnd, gain, dest, nullptr, st));
   GK_HIP(hipStreamSynchronize(st));
   return GK_OK;
 }
@@ -959,7 +966,7 @@ int gk_k_select(gk_ctx* ctx, const gk_csr* dg, const int32_t* part, const int32_
   GK_TRY(ctl.alloc(P, 4));
   GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(g.n)));
   const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
-  GK_TRY(gkd::launch_select(g, bnd, gain, dest, pw, cap, idbits, upd, keys.p, ctl.p, nullptr, st));
+  GK_TRY(gkd::launch_select(g, bnd, gain, dest, pw, cap, idbits, upd, keys.p, ctl.p, nullptr, 1, nullptr, st));
   int32_t hc[2];
   GK_TRY(fetch(ctx, st, {{hc, ctl.p, sizeof hc}}));
   *nsel = hc[0];
@@ -1028,10 +1035,13 @@ int gk_k_apply(gk_ctx* ctx, const gk_csr* dg, const int32_t* moves, int32_t pos,
   GK_TRY(to_host(st, hg, gain, g.n));
   int64_t s = 0;
   for (int32_t v : hm) s += hg[v];
-  Buf<int32_t> ctl;
+  Buf<int32_t> ctl, wl, wlc;
   GK_TRY(ctl.alloc(ctx->pool, 4));
+  GK_TRY(wl.alloc(ctx->pool, g.n));
+  GK_TRY(wlc.alloc(ctx->pool, 2));
   gkd::fill<int32_t>(ctl.p, pos, 4, st);
-  GK_TRY(gkd::launch_apply(g, pos + 1, moves, ctl.p, dest, part, upd, st));
+  gkd::fill<int32_t>(wlc.p, 0, 2, st);
+  GK_TRY(gkd::launch_apply(g, pos + 1, moves, ctl.p, dest, part, upd, wl.p, wlc.p, st));
   GK_HIP(hipStreamSynchronize(st));
   *gain_sum = s;
   return GK_OK;

@@ -73,26 +73,34 @@ int launch_len_scan(int32_t nc, const int32_t* len, int32_t* cxadj, int32_t* cnn
 int launch_project(int32_t n, const int32_t* cmap, const int32_t* cpart, const int32_t* cbnd,
                    int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* upd,
                    hipStream_t st);
-// Heavy list: vertices whose degree exceeds `lanes` (compacted, any order).
-int launch_heavy_list(const DGraph& g, int lanes, int32_t* list, int32_t* cnt, hipStream_t st);
-// work (may be null) accumulates [0] vertices recomputed, [1] their entries.
-int launch_gain(const DGraph& g, int32_t k, int lanes, const int32_t* heavy, int32_t nheavy,
-                const int32_t* part, const int32_t* upd, int32_t* bnd, int32_t* gain,
-                int32_t* dest, unsigned long long* work, hipStream_t st);
-// Selection: clears upd, appends keys ((~gain & 0x7fffffff) << idbits | v)
-// of the selected vertices; ctl[0] = count, ctl[1] = max gain.  work (may be
-// null) accumulates [2] candidates, [3] rows walked, [4] entries scanned.
+// Worklist of the v with flags[v] != 0 appended at list[*cnt...].
+int launch_compact(int32_t n, const int32_t* flags, int32_t* list, int32_t* cnt, hipStream_t st);
+// Gain pass over the worklist wl[0..wlc[0]); rows longer than `lanes` go
+// through heavy[] (count wlc[1]) to the LDS-histogram kernel.  Clears upd of
+// every processed vertex.  work (may be null) accumulates [0] vertices
+// recomputed, [1] their entries.
+int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
+                   const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
+                   unsigned long long* work, hipStream_t st);
+// Selection: appends keys ((~gain & 0x7fffffff) << idbits | v) of the
+// selected vertices; ctl[0] = count, ctl[1] = max gain.  clear_upd zeroes
+// upd[] (the reference's behaviour); reset (may be null) gets reset[0] =
+// reset[1] = 0 (worklist counters).  work (may be null) accumulates
+// [2] candidates, [3] rows walked, [4] entries scanned.
 int launch_select(const DGraph& g, const int32_t* bnd, const int32_t* gain, const int32_t* dest,
                   const int32_t* pw, int64_t cap, int idbits, int32_t* upd, uint64_t* keys,
-                  int32_t* ctl, unsigned long long* work, hipStream_t st);
+                  int32_t* ctl, unsigned long long* work, int clear_upd, int32_t* reset, hipStream_t st);
 // Balance: first min(count, M) keys (sorted when presorted, else sorted in
 // LDS); writes moves[0..pos], ctl[2] = pos; advances pw; cut -= gains.
 int launch_balance(const DGraph& g, int32_t k, const uint64_t* keys, int32_t count, int32_t M,
                    bool presorted, int idbits, const int32_t* part, const int32_t* dest,
                    const int32_t* gain, int32_t* pw, int64_t cap, int32_t* moves, int32_t* ctl,
                    long long* cut, hipStream_t st);
+// Applies moves[0..ctl[2]]; movers and neighbours whose upd flag goes 0 -> 1
+// are appended to wl (count wlc[0]).
 int launch_apply(const DGraph& g, int32_t M, const int32_t* moves, const int32_t* ctl,
-                 const int32_t* dest, int32_t* part, int32_t* upd, hipStream_t st);
+                 const int32_t* dest, int32_t* part, int32_t* upd, int32_t* wl, int32_t* wlc,
+                 hipStream_t st);
 int launch_boundary(const DGraph& g, const int32_t* part, int32_t* bnd, hipStream_t st);
 // cut (unsigned long long accumulator) and part weights pw64[k].
 int launch_cut(const DGraph& g, int32_t k, const int32_t* part, unsigned long long* cut,

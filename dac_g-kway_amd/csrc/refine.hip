@@ -40,55 +40,64 @@ int launch_project(int32_t n, const int32_t* cmap, const int32_t* cpart, const i
   return hip_status();
 }
 
-// Vertices with degree above `lanes` go to the heavy (LDS histogram) path.
-__global__ void __launch_bounds__(256) k_heavy_list(int32_t n, const int32_t* __restrict__ xadj, int lanes,
-                                                    int32_t* __restrict__ list, int32_t* __restrict__ cnt) {
+// Worklist compaction: the indices v with flags[v] != 0, in any order (the
+// gain pass is per vertex); one global atomic per block.
+__global__ void __launch_bounds__(256) k_compact(int32_t n, const int32_t* __restrict__ flags,
+                                                 int32_t* __restrict__ list, int32_t* __restrict__ cnt) {
+  __shared__ int wsum[4];
+  __shared__ int base_s;
   const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  const bool h = v < n && (xadj[v + 1] - xadj[v]) > lanes;
-  const uint64_t m = ballot(h);
-  if (m == 0) return;
-  const int leader = __ffsll(static_cast<long long>(m)) - 1;
-  int base = 0;
-  if (lane_id() == leader) base = atomicAdd(cnt, popc64(m));
-  base = __shfl(base, leader, 64);
-  if (h) list[base + popc64(m & lanemask_lt())] = int32_t(v);
+  const bool f = v < n && flags[v] != 0;
+  const uint64_t m = ballot(f);
+  const int wave = threadIdx.x >> 6;
+  if (lane_id() == 0) wsum[wave] = popc64(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    base_s = t ? atomicAdd(cnt, t) : 0;
+  }
+  __syncthreads();
+  int off = base_s;
+  for (int w = 0; w < wave; ++w) off += wsum[w];
+  if (f) list[off + popc64(m & lanemask_lt())] = int32_t(v);
 }
 
-int launch_heavy_list(const DGraph& g, int lanes, int32_t* list, int32_t* cnt, hipStream_t st) {
-  fill<int32_t>(cnt, 0, 1, st);
-  if (g.n <= 0) return 0;
-  k_heavy_list<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, lanes, list, cnt);
+int launch_compact(int32_t n, const int32_t* flags, int32_t* list, int32_t* cnt, hipStream_t st) {
+  if (n <= 0) return 0;
+  k_compact<<<grid_for(n, 256), 256, 0, st>>>(n, flags, list, cnt);
   return hip_status();
 }
 
 // update_vertex_info + calculate_gain (gkway/uncoarsen.hpp:175-237, 82-109)
-// for rows of at most S entries, S lanes per vertex.  Each lane holds one
-// (part, weight) entry and sums the weights of the lanes with the same part
-// (an O(S) shuffle loop), which gives the connectivity to every adjacent
-// part without a per-vertex histogram.  Gain is the signed max over the
-// adjacent parts of conn_q - conn_p, floored at 0, lowest q on ties.
+// over a worklist of updated vertices (wlc[0] entries), S lanes per vertex,
+// grid-stride.  Each lane holds one (part, weight) entry of the row and sums
+// the weights of the lanes with the same part (an O(S) shuffle loop), which
+// gives the connectivity to every adjacent part without a histogram.  Gain is
+// the signed max over the adjacent parts of conn_q - conn_p, floored at 0,
+// lowest q on ties.  Rows longer than S are appended to the heavy list
+// (wlc[1] entries) for k_gain_heavy.  Processed vertices get upd[v] = 0.
 template <int S>
-__global__ void __launch_bounds__(256) k_gain_small(int32_t n, const int32_t* __restrict__ xadj,
-                                                    const int32_t* __restrict__ adj,
-                                                    const int32_t* __restrict__ w,
-                                                    const int32_t* __restrict__ part,
-                                                    const int32_t* __restrict__ upd,
-                                                    int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
-                                                    int32_t* __restrict__ dest,
-                                                    unsigned long long* __restrict__ work) {
-  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  const int32_t v = static_cast<int32_t>(gt / S);
-  const int sub = static_cast<int>(gt % S);
-  // No early return: every lane reaches the wave-wide work count below.
-  int32_t d = 0;
-  bool act = v < n && upd[v] != 0;
-  int32_t s = 0;
-  if (act) {
-    s = xadj[v];
-    d = xadj[v + 1] - s;
-    act = d <= S;  // longer rows take the heavy path
-  }
-  if (act) {
+__global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
+                                                 const int32_t* __restrict__ xadj,
+                                                 const int32_t* __restrict__ adj,
+                                                 const int32_t* __restrict__ w,
+                                                 const int32_t* __restrict__ part,
+                                                 int32_t* __restrict__ upd, int32_t* __restrict__ bnd,
+                                                 int32_t* __restrict__ gain, int32_t* __restrict__ dest,
+                                                 int32_t* __restrict__ heavy,
+                                                 unsigned long long* __restrict__ work) {
+  const int32_t cnt = wlc[0];
+  const int64_t sg0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) / S;
+  const int64_t nsg = int64_t(gridDim.x) * (256 / S);
+  const int sub = threadIdx.x & (S - 1);
+  int done_v = 0, done_e = 0;
+  for (int64_t e = sg0; e < cnt; e += nsg) {
+    const int32_t v = wl[e];
+    const int32_t s = xadj[v], d = xadj[v + 1] - s;
+    if (d > S) {
+      if (sub == 0) heavy[atomicAdd(&wlc[1], 1)] = v;
+      continue;
+    }
     const int32_t p = part[v];
     int32_t q = -1, wj = 0;
     if (sub < d) {
@@ -122,35 +131,37 @@ __global__ void __launch_bounds__(256) k_gain_small(int32_t n, const int32_t* __
         gain[v] = 0;
         dest[v] = p;
       }
+      upd[v] = 0;
+      ++done_v;
+      done_e += d;
     }
   }
   if (work) {
-    const bool head = act && sub == 0;
-    const uint64_t m = ballot(head);
-    const int dsum = wave_sum(head ? d : 0);
-    if (m && lane_id() == 0) {
-      atomicAdd(&work[0], (unsigned long long)popc64(m));
-      atomicAdd(&work[1], (unsigned long long)dsum);
+    const int a = wave_sum(done_v), b = wave_sum(done_e);
+    if (lane_id() == 0 && a) {
+      atomicAdd(&work[0], (unsigned long long)a);
+      atomicAdd(&work[1], (unsigned long long)b);
     }
   }
 }
 
 // Rows above S entries: one wave per vertex with a dense LDS histogram of the
 // k part connectivities (integer LDS atomics, order-independent).
-__global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ heavy, int32_t nheavy,
-                                                   int32_t k, const int32_t* __restrict__ xadj,
+__global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ heavy,
+                                                   const int32_t* __restrict__ wlc, int32_t k,
+                                                   const int32_t* __restrict__ xadj,
                                                    const int32_t* __restrict__ adj,
                                                    const int32_t* __restrict__ w,
                                                    const int32_t* __restrict__ part,
-                                                   const int32_t* __restrict__ upd,
+                                                   int32_t* __restrict__ upd,
                                                    int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
                                                    int32_t* __restrict__ dest,
                                                    unsigned long long* __restrict__ work) {
   extern __shared__ int32_t hist[];
   const int lane = lane_id();
-  for (int32_t hi = blockIdx.x; hi < nheavy; hi += gridDim.x) {
+  const int32_t cnt = wlc[1];
+  for (int32_t hi = blockIdx.x; hi < cnt; hi += gridDim.x) {
     const int32_t v = heavy[hi];
-    if (!upd[v]) continue;
     for (int32_t q = lane; q < k; q += 64) hist[q] = 0;
     __syncthreads();
     const int32_t p = part[v];
@@ -181,6 +192,7 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
       bnd[v] = any ? 1 : 0;
       if (any && g > 0) { gain[v] = g; dest[v] = bq; }
       else { gain[v] = 0; dest[v] = p; }
+      upd[v] = 0;
       if (work) {
         atomicAdd(&work[0], 1ull);
         atomicAdd(&work[1], (unsigned long long)(e - s));
@@ -190,22 +202,28 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
   }
 }
 
-int launch_gain(const DGraph& g, int32_t k, int lanes, const int32_t* heavy, int32_t nheavy,
-                const int32_t* part, const int32_t* upd, int32_t* bnd, int32_t* gain,
-                int32_t* dest, unsigned long long* work, hipStream_t st) {
+int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
+                   const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
+                   unsigned long long* work, hipStream_t st) {
   if (g.n <= 0) return 0;
-  const unsigned grid = grid_for(int64_t(g.n) * lanes, 256);
+  unsigned grid = grid_for(int64_t(g.n) * lanes, 256);
+  if (grid > 2048) grid = 2048;
+#define GK_GAIN_CASE(S)                                                                            \
+  case S:                                                                                           \
+    k_gain_wl<S><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work); \
+    break;
   switch (lanes) {
-    case 4: k_gain_small<4><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
-    case 8: k_gain_small<8><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
-    case 16: k_gain_small<16><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
-    case 32: k_gain_small<32><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
-    default: k_gain_small<64><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work); break;
+    GK_GAIN_CASE(4)
+    GK_GAIN_CASE(8)
+    GK_GAIN_CASE(16)
+    GK_GAIN_CASE(32)
+    default:
+      k_gain_wl<64><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work);
+      break;
   }
-  if (nheavy > 0) {
-    const unsigned hg = static_cast<unsigned>(nheavy < 4096 ? nheavy : 4096);
-    k_gain_heavy<<<hg, 64, sizeof(int32_t) * k, st>>>(heavy, nheavy, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, work);
-  }
+#undef GK_GAIN_CASE
+  k_gain_heavy<<<256, 64, sizeof(int32_t) * k, st>>>(heavy, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain,
+                                                     dest, work);
   return hip_status();
 }
 
@@ -223,12 +241,14 @@ __global__ void __launch_bounds__(256) k_select(int32_t n, const int32_t* __rest
                                                 const int32_t* __restrict__ pw, long long cap, int idbits,
                                                 int32_t* __restrict__ upd, uint64_t* __restrict__ keys,
                                                 int32_t* __restrict__ ctl,
-                                                unsigned long long* __restrict__ work) {
+                                                unsigned long long* __restrict__ work, int clear_upd,
+                                                int32_t* __restrict__ reset) {
   const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (reset && v == 0) { reset[0] = 0; reset[1] = 0; }
   bool sel = false, cand = false, walked = false;
   int32_t g = 0, scanned = 0;
   if (v < n) {
-    upd[v] = 0;
+    if (clear_upd) upd[v] = 0;
     g = gain[v];
     cand = bnd[v] && g > 0;
     if (cand && (long long)pw[dest[v]] + vw[v] <= cap) {
@@ -267,11 +287,11 @@ __global__ void __launch_bounds__(256) k_select(int32_t n, const int32_t* __rest
 
 int launch_select(const DGraph& g, const int32_t* bnd, const int32_t* gain, const int32_t* dest,
                   const int32_t* pw, int64_t cap, int idbits, int32_t* upd, uint64_t* keys,
-                  int32_t* ctl, unsigned long long* work, hipStream_t st) {
+                  int32_t* ctl, unsigned long long* work, int clear_upd, int32_t* reset, hipStream_t st) {
   fill<int32_t>(ctl, 0, 2, st);
   if (g.n <= 0) return 0;
   k_select<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, bnd, gain, dest, pw, cap,
-                                               idbits, upd, keys, ctl, work);
+                                               idbits, upd, keys, ctl, work, clear_upd, reset);
   return hip_status();
 }
 
@@ -392,23 +412,30 @@ int launch_balance(const DGraph& g, int32_t k, const uint64_t* keys, int32_t cou
 
 // apply_sequence_move (gkway/uncoarsen.hpp:290-323) without the pw/cut parts
 // (done in k_balance): move, and mark the mover and its neighbours updated.
+// A vertex whose flag goes 0 -> 1 is appended to the gain worklist, so each
+// updated vertex enters it once.
 __global__ void __launch_bounds__(256) k_apply(int32_t M, const int32_t* __restrict__ xadj,
                                                const int32_t* __restrict__ adj,
                                                const int32_t* __restrict__ moves,
                                                const int32_t* __restrict__ ctl,
                                                const int32_t* __restrict__ dest,
-                                               int32_t* __restrict__ part, int32_t* __restrict__ upd) {
+                                               int32_t* __restrict__ part, int32_t* __restrict__ upd,
+                                               int32_t* __restrict__ wl, int32_t* __restrict__ wlc) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= M || j > ctl[2]) return;
   const int32_t v = moves[j];
   part[v] = dest[v];
-  upd[v] = 1;
-  for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) upd[adj[e]] = 1;
+  if (atomicExch(&upd[v], 1) == 0) wl[atomicAdd(&wlc[0], 1)] = v;
+  for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) {
+    const int32_t u = adj[e];
+    if (atomicExch(&upd[u], 1) == 0) wl[atomicAdd(&wlc[0], 1)] = u;
+  }
 }
 
 int launch_apply(const DGraph& g, int32_t M, const int32_t* moves, const int32_t* ctl,
-                 const int32_t* dest, int32_t* part, int32_t* upd, hipStream_t st) {
-  k_apply<<<grid_for(M, 256), 256, 0, st>>>(M, g.xadj, g.adj, moves, ctl, dest, part, upd);
+                 const int32_t* dest, int32_t* part, int32_t* upd, int32_t* wl, int32_t* wlc,
+                 hipStream_t st) {
+  k_apply<<<grid_for(M, 256), 256, 0, st>>>(M, g.xadj, g.adj, moves, ctl, dest, part, upd, wl, wlc);
   return hip_status();
 }
 
