@@ -170,15 +170,16 @@ def main():
         log(f"oracle: {tcpu:.1f}s cut={ost.cut} bit_exact={bit_exact}")
 
     if args.calibrate_copy:
+        # Elementwise kernel of known traffic: reads 1 GiB, writes 1 GiB per
+        # dispatch; FETCH_SIZE / WRITE_SIZE of it give the counter scale.
         import torch as _t
-        a = _t.empty(1 << 28, dtype=_t.int32, device="cuda")
+        a = _t.ones(1 << 28, dtype=_t.int32, device="cuda")
         b = _t.empty_like(a)
-        a.fill_(1)
         _t.cuda.synchronize()
         for _ in range(3):
-            b.copy_(a)
+            _t.mul(a, 3, out=b)
         _t.cuda.synchronize()
-        log("calibration copy: 3 x 1 GiB device-to-device")
+        log("calibration: 3 x elementwise multiply, 1 GiB in + 1 GiB out each")
 
     ok = deterministic and (bit_exact is not False) and st.imbalance <= 1.03 + 1e-9
     out = {

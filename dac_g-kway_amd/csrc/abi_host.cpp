@@ -55,7 +55,11 @@ int32_t gk_coarsen_floor(int32_t n0, int32_t k, int32_t policy, int32_t override
   int lg = 0;
   while ((int64_t(1) << lg) < k) ++lg;
   if (lg < 1) lg = 1;
+  // METIS-style n0 / (20 log2 k), capped at 240 coarse vertices per part
+  // (larger coarsest graphs cost host time without a better final cut), and
+  // at least 30 per part.
   int64_t a = int64_t(n0) / (20 * int64_t(lg));
+  if (a > 240 * int64_t(k)) a = 240 * int64_t(k);
   int64_t b = 30 * int64_t(k);
   int64_t r = a > b ? a : b;
   return static_cast<int32_t>(r > INT32_MAX ? INT32_MAX : r);

@@ -4,6 +4,8 @@
 // gkway/uncoarsen.hpp and gkway/cuda_check.hpp:61-73.  Integer-exact and
 // schedule-independent: selected moves are ordered by a sort on the packed
 // key (gain desc, id asc) before any order-sensitive step.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "prim.hpp"
 #include "../../include/gkway.h"
@@ -41,16 +43,27 @@ int launch_project(int32_t n, const int32_t* cmap, const int32_t* cpart, const i
 }
 
 // Worklist compaction: the indices v with flags[v] != 0, in any order (the
-// gain pass is per vertex); one global atomic per block.
+// gain pass is per vertex).  A block covers kCompactTile flags (16 per
+// thread, wave-contiguous slots): it counts them, takes one global atomic,
+// then writes -- few enough atomics that the counter is never contended.
+constexpr int kCompactIPT = 16;
+constexpr int kCompactTile = 256 * kCompactIPT;
+
 __global__ void __launch_bounds__(256) k_compact(int32_t n, const int32_t* __restrict__ flags,
                                                  int32_t* __restrict__ list, int32_t* __restrict__ cnt) {
   __shared__ int wsum[4];
   __shared__ int base_s;
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  const bool f = v < n && flags[v] != 0;
-  const uint64_t m = ballot(f);
   const int wave = threadIdx.x >> 6;
-  if (lane_id() == 0) wsum[wave] = popc64(m);
+  const int64_t wbase = int64_t(blockIdx.x) * kCompactTile + int64_t(wave) * 64 * kCompactIPT;
+  uint64_t m[kCompactIPT];
+  int tot = 0;
+#pragma unroll
+  for (int j = 0; j < kCompactIPT; ++j) {
+    const int64_t v = wbase + int64_t(j) * 64 + lane_id();
+    m[j] = ballot(v < n && flags[v] != 0);
+    tot += popc64(m[j]);
+  }
+  if (lane_id() == 0) wsum[wave] = tot;
   __syncthreads();
   if (threadIdx.x == 0) {
     const int t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
@@ -59,12 +72,17 @@ __global__ void __launch_bounds__(256) k_compact(int32_t n, const int32_t* __res
   __syncthreads();
   int off = base_s;
   for (int w = 0; w < wave; ++w) off += wsum[w];
-  if (f) list[off + popc64(m & lanemask_lt())] = int32_t(v);
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < kCompactIPT; ++j) {
+    if ((m[j] >> lane_id()) & 1ull) list[off + popc64(m[j] & lt)] = int32_t(wbase + int64_t(j) * 64 + lane_id());
+    off += popc64(m[j]);
+  }
 }
 
 int launch_compact(int32_t n, const int32_t* flags, int32_t* list, int32_t* cnt, hipStream_t st) {
   if (n <= 0) return 0;
-  k_compact<<<grid_for(n, 256), 256, 0, st>>>(n, flags, list, cnt);
+  k_compact<<<grid_for(n, kCompactTile), 256, 0, st>>>(n, flags, list, cnt);
   return hip_status();
 }
 
@@ -204,9 +222,12 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
 
 int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
                    const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
-                   unsigned long long* work, hipStream_t st) {
+                   unsigned long long* work, int64_t max_items, hipStream_t st) {
   if (g.n <= 0) return 0;
-  unsigned grid = grid_for(int64_t(g.n) * lanes, 256);
+  // The grid covers at most max_items worklist entries (a bound the caller
+  // knows without reading the count); the kernel grid-strides beyond it.
+  const int64_t items = std::min<int64_t>(g.n, std::max<int64_t>(max_items, 1));
+  unsigned grid = grid_for(items * lanes, 256);
   if (grid > 2048) grid = 2048;
 #define GK_GAIN_CASE(S)                                                                            \
   case S:                                                                                           \
@@ -328,11 +349,14 @@ __global__ void __launch_bounds__(kBalThreads) k_balance(int32_t k, const uint64
   for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
   __syncthreads();
   if (!presorted) {
-    // Bitonic sort of the 1024 keys (unused slots hold the maximum key).
-    for (int k2 = 2; k2 <= kBalMax; k2 <<= 1) {
+    // Bitonic sort of the first P = next_pow2(count) keys (unused slots hold
+    // the maximum key); P is uniform across the block.
+    int P = 2;
+    while (P < count) P <<= 1;
+    for (int k2 = 2; k2 <= P; k2 <<= 1) {
       for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
         const int ix = t ^ j2;
-        if (ix > t) {
+        if (t < P && ix > t) {
           const uint64_t a = sk[t], b = sk[ix];
           const bool up = (t & k2) == 0;
           if (up ? a > b : a < b) { sk[t] = b; sk[ix] = a; }

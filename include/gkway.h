@@ -66,7 +66,7 @@ typedef struct gk_csr {
 } gk_csr;
 
 /* Coarsening stop policy. */
-#define GK_COARSEN_TARGET 0  /* keep a level only while nc >= max(n0/(20*ceil(log2 k)), 30k) */
+#define GK_COARSEN_TARGET 0  /* keep a level only while nc >= max(min(n0/(20*ceil(log2 k)), 240k), 30k) */
 #define GK_COARSEN_REF 1     /* the reference's rule: coarsen while n > 20k
                                 (gkway/graph_partitioner.hpp:22,66,93)   */
 

@@ -60,7 +60,7 @@ int32_t oracle_coarsen_floor(int32_t n0, int32_t k, int32_t policy, int32_t over
   int lg = 0;
   while ((int64_t(1) << lg) < k) ++lg;
   if (lg < 1) lg = 1;
-  int64_t a = int64_t(n0) / (20 * int64_t(lg));
+  int64_t a = std::min<int64_t>(int64_t(n0) / (20 * int64_t(lg)), 240 * int64_t(k));
   int64_t b = 30 * int64_t(k);
   return int32_t(std::min<int64_t>(INT32_MAX, std::max(a, b)));
 }

@@ -1,0 +1,233 @@
+"""Host-side components on CPU: the C ABI library (symbols, status strings,
+parameters), METIS file I/O and validation, the initial partitioner, and the
+workload generators.  No GPU calls."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+# ------------------------------------------------------------------ C ABI
+def declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:gk|gkgen)_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.mark.parametrize("header,lib", [("gkway.h", "libgkway.so"), ("gkgen.h", "libgkgen.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    path = os.path.join(ROOT, "dac_g-kway_amd", lib)
+    ctypes.CDLL(path)  # loads without a GPU
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (\w+)", out))
+    names = declared(header)
+    assert names, header
+    missing = [s for s in names if s not in exported]
+    assert not missing, missing
+
+
+def test_status_strings_and_defaults(gk):
+    L = gk.lib()
+    for code in range(0, -10, -1):
+        assert L.gk_strerror(code)
+    p = gk.Params()
+    L.gk_params_default(ctypes.byref(p), 64)
+    assert (p.k, p.ubfactor, p.max_group, p.max_moves, p.coarsen_policy, p.seed) == (64, 30, 6, 1024, 0, 1)
+
+
+def test_cap_and_floor_match_oracle(gk, oracle):
+    L, O = gk.lib(), oracle.lib()
+    for T, k, ub in [(2048, 4, 30), (4194304, 64, 30), (10, 3, 30), (7, 8, 30), (10 ** 9, 256, 50)]:
+        assert L.gk_part_cap(T, k, ub) == O.oracle_part_cap(T, k, ub)
+    # cap is at least ceil(T/k) and exactly floor(T*1.03/k) when that is larger
+    assert L.gk_part_cap(2048, 4, 30) == 527
+    assert L.gk_part_cap(7, 8, 30) == 1
+    for n0, k, pol, ov in [(4194304, 64, 0, 0), (2048, 4, 0, 0), (10 ** 6, 256, 0, 0), (5000, 4, 1, 0), (100, 2, 0, 77)]:
+        assert L.gk_coarsen_floor(n0, k, pol, ov) == O.oracle_coarsen_floor(n0, k, pol, ov)
+    assert L.gk_coarsen_floor(4194304, 64, 0, 0) == 240 * 64
+    assert L.gk_coarsen_floor(1 << 20, 64, 0, 0) == (1 << 20) // 120
+
+
+def test_no_device_is_a_status_not_a_crash(gk):
+    L = gk.lib()
+    h = ctypes.c_void_p()
+    rc = L.gk_ctx_create(0, ctypes.byref(h))
+    if rc == 0:
+        L.gk_ctx_destroy(h)
+        pytest.skip("a HIP device is present")
+    assert rc == -9  # GK_ERR_NODEVICE
+    assert b"device" in L.gk_last_error()
+
+
+# ------------------------------------------------------------------ METIS I/O
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_read_sample_graph(gk):
+    g = gk.read_metis(os.path.join(GOLDEN, "delaunay_n11.graph"))
+    assert (g.n, g.nnz) == (2048, 12254)
+    d = g.degrees()
+    assert d.min() == 3 and d.max() == 13
+    assert gk.lib().gk_graph_check(ctypes.byref(g.csr()), 1) == 0
+
+
+def test_read_formats(gk, tmp_path):
+    # triangle 1-2-3 plus isolated vertex 4, with comments and both weight kinds
+    plain = "% comment\n4 3\n2 3\n1 3\n1 2\n\n"
+    g = gk.read_metis(write(tmp_path, "a.graph", plain))
+    assert list(g.xadj) == [0, 2, 4, 6, 6] and list(g.adjncy) == [1, 2, 0, 2, 0, 1]
+    ew = "4 3 1\n2 5 3 7\n1 5 3 2\n1 7 2 2\n\n"
+    g = gk.read_metis(write(tmp_path, "b.graph", ew))
+    assert list(g.adjwgt) == [5, 7, 5, 2, 7, 2]
+    vwew = "4 3 011\n3 2 5 3 7\n1 1 5 3 2\n2 1 7 2 2\n4\n"
+    g = gk.read_metis(write(tmp_path, "c.graph", vwew))
+    assert list(g.vwgt) == [3, 1, 2, 4] and list(g.adjwgt) == [5, 7, 5, 2, 7, 2]
+    vsize = "3 2 100\n9 2\n9 1 3\n9 2\n"
+    g = gk.read_metis(write(tmp_path, "d.graph", vsize))
+    assert list(g.adjncy) == [1, 0, 2, 1]
+
+
+@pytest.mark.parametrize("text", [
+    "",                          # empty
+    "3\n",                       # header without edge count
+    "3 2\n2\n1 3\n",             # fewer lines than vertices
+    "2 1\n2\n1\n1\n",            # extra non-blank line
+    "2 1\n3\n1\n",               # neighbour out of range
+    "2 2\n2\n1\n",               # entry count != 2m
+    "2 1 1\n2\n1 1\n",           # missing edge weight
+    "2 1 2\n2\n1\n",             # bad fmt digit
+    "2 1\n2 x\n1\n",             # junk token
+])
+def test_read_rejects_malformed(gk, tmp_path, text):
+    with pytest.raises(RuntimeError):
+        gk.read_metis(write(tmp_path, "bad.graph", text))
+
+
+def test_check_rejects_invalid_graphs(gk):
+    L = gk.lib()
+
+    def check(xadj, adj, w=None, vw=None, sym=1):
+        return L.gk_graph_check(ctypes.byref(gk.Graph(xadj, adj, w, vw).csr()), sym)
+
+    assert check([0, 1, 2], [1, 0]) == 0
+    assert check([0, 1, 1], [1]) == -4          # asymmetric
+    assert check([0, 1, 2], [0, 1]) == -4       # self loops
+    assert check([0, 1, 2], [1, 0], [2, 3]) == -4  # unequal weights
+    assert check([0, 1, 2], [1, 0], [0, 0]) == -4  # zero weight
+    assert check([0, 2, 2], [1, 1], sym=0) == 0    # duplicates only caught with symmetry check
+    assert check([0, 2, 2], [1, 1]) == -4
+
+
+def test_write_read_round_trip(gk, tmp_path):
+    g = gk.dag(500, 10, 4)
+    path = str(tmp_path / "rt.graph")
+    assert gk.lib().gk_graph_write(path.encode(), ctypes.byref(g.csr())) == 0
+    h = gk.read_metis(path)
+    for a in ("xadj", "adjncy", "adjwgt", "vwgt"):
+        assert np.array_equal(getattr(g, a), getattr(h, a))
+
+
+def test_write_partition_format(gk, tmp_path):
+    part = np.array([3, 0, 2, 1], np.int32)
+    path = str(tmp_path / "x.out")
+    assert gk.lib().gk_write_partition(path.encode(), part.ctypes.data_as(gk.I32P), 4) == 0
+    assert open(path).read() == "3\n0\n2\n1\n"
+
+
+# ------------------------------------------------------------------ initial partitioner
+@pytest.mark.parametrize("k", [2, 3, 7, 16, 64])
+def test_initpart_valid_balanced_deterministic(gk, k):
+    g = gk.delaunay(4000, 3)
+    cap = gk.lib().gk_part_cap(int(g.vwgt.sum()), k, 30)
+    p1, c1 = gk.initpart(g, k, cap, threads=1)
+    p8, c8 = gk.initpart(g, k, cap, threads=8)
+    assert np.array_equal(p1, p8) and c1 == c8
+    assert p1.min() == 0 and p1.max() == k - 1
+    pw = np.bincount(p1, weights=g.vwgt, minlength=k)
+    assert pw.max() <= cap
+    src = np.repeat(np.arange(g.n), g.degrees())
+    assert c1 == int(g.adjwgt[(p1[src] != p1[g.adjncy]) & (src < g.adjncy)].sum())
+
+
+def test_initpart_weighted(gk):
+    g = gk.dag(6000, 30, 8)
+    k = 32
+    cap = gk.lib().gk_part_cap(int(g.vwgt.sum()), k, 30)
+    p, c = gk.initpart(g, k, cap)
+    assert np.bincount(p, weights=g.vwgt, minlength=k).max() <= cap
+
+
+def test_initpart_k1_and_tiny(gk):
+    g = gk.Graph([0, 1, 2], [1, 0])
+    p, c = gk.initpart(g, 1)
+    assert list(p) == [0, 0] and c == 0
+    p, c = gk.initpart(g, 2)
+    assert sorted(p) == [0, 1] and c == 1
+
+
+# Recorded METIS 5.2.1 cuts on the sample graph (see test_cpu_oracle.py).
+@pytest.mark.parametrize("k,metis", [(2, 91), (4, 205), (8, 349), (64, 2083)])
+def test_initpart_quality_on_sample_graph(gk, k, metis):
+    g = gk.read_metis(os.path.join(GOLDEN, "delaunay_n11.graph"))
+    _, c = gk.initpart(g, k)
+    assert c <= 1.05 * metis
+
+
+# ------------------------------------------------------------------ generators
+def test_delaunay_matches_scipy(gk):
+    spatial = pytest.importorskip("scipy.spatial")
+    G = gk.gen()
+    for n, seed in [(50, 1), (2048, 1), (30000, 2)]:
+        x = np.zeros(n, np.int32)
+        y = np.zeros(n, np.int32)
+        G.gkgen_points(n, seed, x.ctypes.data_as(gk.I32P), y.ctypes.data_as(gk.I32P))
+        tri = spatial.Delaunay(np.stack([x, y], 1).astype(np.float64))
+        ref = set()
+        for s in tri.simplices:
+            for a, b in ((s[0], s[1]), (s[1], s[2]), (s[0], s[2])):
+                ref.add((min(a, b), max(a, b)))
+        g = gk.delaunay(n, seed)
+        src = np.repeat(np.arange(g.n), g.degrees())
+        m = src < g.adjncy
+        assert set(zip(src[m].tolist(), g.adjncy[m].tolist())) == ref
+
+
+def test_generators_structure(gk):
+    L = gk.lib()
+    d = gk.delaunay(5000, 4)
+    assert L.gk_graph_check(ctypes.byref(d.csr()), 1) == 0
+    assert abs(d.m - 3 * 5000) < 200  # planar triangulation: m ~ 3n - hull
+    r = gk.road(20000, 2)
+    assert L.gk_graph_check(ctypes.byref(r.csr()), 1) == 0
+    assert r.m == round(1.05 * 20000)
+    # connected (the EMST spans every point)
+    seen = np.zeros(r.n, bool)
+    stack = [0]
+    seen[0] = True
+    while stack:
+        v = stack.pop()
+        for u in r.adjncy[r.xadj[v]:r.xadj[v + 1]]:
+            if not seen[u]:
+                seen[u] = True
+                stack.append(u)
+    assert seen.all()
+    g = gk.dag(20000, 100, 3)
+    assert L.gk_graph_check(ctypes.byref(g.csr()), 1) == 0
+    assert g.adjwgt.min() >= 1 and g.adjwgt.max() <= 100
+    assert g.vwgt.min() >= 1 and g.vwgt.max() <= 4
+    layer = np.arange(g.n) * 100 // g.n
+    src = np.repeat(np.arange(g.n), g.degrees())
+    gap = np.abs(layer[src] - layer[g.adjncy])
+    assert gap.min() >= 1 and gap.max() <= 3
+    # same seed, same graph
+    g2 = gk.dag(20000, 100, 3)
+    assert np.array_equal(g.adjncy, g2.adjncy) and np.array_equal(g.adjwgt, g2.adjwgt)
