@@ -54,7 +54,7 @@ class LevelStats(Structure):
 GK_FLAG_STAGE_TIME = 4
 GK_ST_COUNT = 16
 STAGE_NAMES = ["match", "group", "sort", "cluster", "contract", "contract_big", "project", "gain",
-               "select", "msort", "balance", "apply", "cut"]
+               "select", "msort", "balance", "apply", "cut", "gain_heavy", "contract_aux", "pack"]
 
 
 class Stats(Structure):
@@ -73,6 +73,24 @@ class Stats(Structure):
 
     def level_list(self):
         return [self.levels[i] for i in range(self.nlevels)]
+
+
+class PackedCsr(Structure):
+    _fields_ = [("ngraphs", c_int32), ("n", c_int32), ("nnz", c_int32), ("xadj", I32P),
+                ("adjncy", I32P), ("adjwgt", I32P), ("vwgt", I32P), ("v_off", I32P),
+                ("e_off", I32P)]
+
+
+class ServeConfig(Structure):
+    _fields_ = [("hbm_budget", c_int64), ("max_inflight", c_int32), ("workers", c_int32)]
+
+
+class ServeStats(Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("failed", c_int64),
+                ("queued", c_int64), ("inflight", c_int64), ("edges_done", c_int64),
+                ("steps", c_int64), ("footprint_now", c_int64), ("footprint_peak", c_int64),
+                ("device_peak", c_int64), ("p50_ms", c_double), ("p99_ms", c_double),
+                ("max_ms", c_double), ("edges_per_s", c_double)]
 
 
 class GenGraph(Structure):
@@ -172,6 +190,18 @@ def lib():
         L.gk_graph_free.argtypes = [POINTER(Csr)]
         L.gk_write_partition.argtypes = [c_char_p, I32P, c_int32]
         L.gk_partition_file.argtypes = [VP, c_char_p, c_char_p, c_int32, POINTER(Stats)]
+        L.gk_partition_packed.argtypes = [VP, POINTER(PackedCsr), POINTER(Params), I32P,
+                                          POINTER(Stats), VP]
+        L.gk_partition_batch.argtypes = [VP, c_int32, POINTER(Csr), POINTER(Params),
+                                         POINTER(I32P), POINTER(Stats)]
+        L.gk_serve_open.argtypes = [VP, POINTER(ServeConfig), POINTER(c_void_p)]
+        L.gk_serve_submit.argtypes = [VP, POINTER(Csr), POINTER(Params), POINTER(c_int64)]
+        L.gk_serve_poll.argtypes = [VP, c_int64, I32P, POINTER(Stats)]
+        L.gk_serve_wait.argtypes = [VP, c_int64, I32P, POINTER(Stats)]
+        L.gk_serve_query.argtypes = [VP, POINTER(ServeStats)]
+        L.gk_serve_close.argtypes = [VP]
+        L.gk_serve_footprint.restype = c_int64
+        L.gk_serve_footprint.argtypes = [c_int32, c_int32, POINTER(Params)]
         _LIB = L
     return _LIB
 
@@ -297,6 +327,104 @@ class Context:
             raise RuntimeError(f"gk_partition: {lib().gk_strerror(rc).decode()}: "
                                f"{lib().gk_last_error().decode()}")
         return part, st
+
+    def partition_batch(self, graphs, ks, **kw):
+        """gk_partition_batch over host graphs; ks[i] is graph i's part count.
+        Returns (list of partitions, list of Stats)."""
+        n = len(graphs)
+        csrs = (Csr * n)(*[g.csr() for g in graphs])
+        prm = (Params * n)(*[params(int(k), **kw) for k in ks])
+        parts = [np.zeros(max(g.n, 1), np.int32) for g in graphs]
+        outs = (I32P * n)(*[p.ctypes.data_as(I32P) for p in parts])
+        st = (Stats * n)()
+        rc = lib().gk_partition_batch(self._h, n, csrs, prm, outs, st)
+        if rc != GK_OK:
+            raise RuntimeError(f"gk_partition_batch: {lib().gk_strerror(rc).decode()}: "
+                               f"{lib().gk_last_error().decode()}")
+        return [p[:g.n] for p, g in zip(parts, graphs)], list(st)
+
+
+class Server:
+    """gk_serve_* request loop on its own gk_ctx (the server owns it)."""
+
+    def __init__(self, device=0, hbm_budget=0, max_inflight=0, workers=0):
+        self._ctx = Context(device)
+        cfg = ServeConfig(hbm_budget, max_inflight, workers)
+        self._h = c_void_p()
+        rc = lib().gk_serve_open(self._ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h))
+        if rc != GK_OK:
+            self._ctx.close()
+            raise RuntimeError(f"gk_serve_open: {lib().gk_strerror(rc).decode()}")
+        self._n = {}
+
+    def submit(self, g: Graph, k, **kw) -> int:
+        p = params(k, **kw)
+        rid = c_int64(0)
+        rc = lib().gk_serve_submit(self._h, ctypes.byref(g.csr()), ctypes.byref(p), ctypes.byref(rid))
+        if rc != GK_OK:
+            raise RuntimeError(f"gk_serve_submit: {lib().gk_strerror(rc).decode()}: "
+                               f"{lib().gk_last_error().decode()}")
+        self._n[rid.value] = g.n
+        return rid.value
+
+    def poll(self, rid):
+        """None while pending, else (part, Stats)."""
+        n = self._n[rid]
+        part = np.zeros(max(n, 1), np.int32)
+        st = Stats()
+        rc = lib().gk_serve_poll(self._h, rid, part.ctypes.data_as(I32P), ctypes.byref(st))
+        if rc == 0:
+            return None
+        del self._n[rid]
+        if rc < 0:
+            raise RuntimeError(f"request {rid}: {lib().gk_strerror(rc).decode()}: "
+                               f"{lib().gk_last_error().decode()}")
+        return part[:n], st
+
+    def wait(self, rid):
+        n = self._n.pop(rid)
+        part = np.zeros(max(n, 1), np.int32)
+        st = Stats()
+        rc = lib().gk_serve_wait(self._h, rid, part.ctypes.data_as(I32P), ctypes.byref(st))
+        if rc != GK_OK:
+            raise RuntimeError(f"request {rid}: {lib().gk_strerror(rc).decode()}: "
+                               f"{lib().gk_last_error().decode()}")
+        return part[:n], st
+
+    def stats(self) -> ServeStats:
+        s = ServeStats()
+        lib().gk_serve_query(self._h, ctypes.byref(s))
+        return s
+
+    def close(self):
+        rc = GK_OK
+        if self._h:
+            rc = lib().gk_serve_close(self._h)
+            self._h = c_void_p()
+            self._ctx.close()
+        return rc
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def pack_graphs(graphs):
+    """Concatenate host graphs into one CSR with global ids (the layout of
+    gk_packed_csr); returns (Graph, v_off, e_off)."""
+    v_off = np.zeros(len(graphs) + 1, np.int64)
+    e_off = np.zeros(len(graphs) + 1, np.int64)
+    for i, g in enumerate(graphs):
+        v_off[i + 1] = v_off[i] + g.n
+        e_off[i + 1] = e_off[i] + g.nnz
+    xs = [g.xadj[:-1].astype(np.int64) + e_off[i] for i, g in enumerate(graphs)]
+    xadj = np.concatenate(xs + [np.array([e_off[-1]])]) if graphs else np.zeros(1, np.int64)
+    adj = np.concatenate([g.adjncy.astype(np.int64) + v_off[i] for i, g in enumerate(graphs)])
+    w = np.concatenate([g.adjwgt for g in graphs])
+    vw = np.concatenate([g.vwgt for g in graphs])
+    return Graph(xadj, adj, w, vw), v_off.astype(np.int32), e_off.astype(np.int32)
 
 
 def build(force=True, jobs=8):

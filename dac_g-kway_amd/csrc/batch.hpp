@@ -1,0 +1,75 @@
+// Launch wrappers of the batched-form kernels (batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace gkd {
+
+// One job's source arrays for a pack.  Vertex-side arrays are indexed by the
+// job-local vertex id, entry-side arrays by the job-local entry index; xbase /
+// abase are subtracted from xadj / adjncy values (0 for job-owned levels, the
+// job's offsets when the source is a slice of a caller's packed CSR).
+struct PackSrc {
+  const int32_t* xadj;
+  const int32_t* adj;
+  const int32_t* w;
+  const int32_t* vw;
+  const int32_t* cmap;   // refinement packs: fine -> coarse (minus cbase = local ids), else null
+  const int32_t* cpart;  // refinement packs: coarse partition
+  const int32_t* cbnd;   // refinement packs: coarse boundary flags
+  int32_t xbase;
+  int32_t abase;
+  int32_t cbase;
+};
+
+// One job's destination arrays for an unpack (null members are skipped).
+struct UnpackDst {
+  int32_t* xadj;
+  int32_t* adj;
+  int32_t* w;
+  int32_t* vw;
+  int32_t* cmap;
+  int32_t* part;
+  int32_t* bnd;
+};
+
+// Pack nj jobs: vertices at v_off, entries at e_off (nj+1 prefix arrays);
+// writes xadj[ntot+1], adj/w[etot], vw/vjob[ntot]; with cmap != null also the
+// packed fine->coarse map (+c_off) and, with cpart != null, the packed coarse
+// partition/boundary arrays (ctot entries).
+int launch_pack(int32_t nj, const PackSrc* src, const int32_t* v_off, const int32_t* e_off, const int32_t* c_off,
+                int32_t ntot, int32_t etot, int32_t ctot, int32_t* xadj, int32_t* adj, int32_t* w, int32_t* vw,
+                int32_t* vjob, int32_t* cmap, int32_t* cpart, int32_t* cbnd, hipStream_t st);
+// out[j] = total vertex weight of job j (v_off on the device, nj+1).
+int launch_job_sum(int32_t nj, const int32_t* v_off, const int32_t* vw, long long* out, hipStream_t st);
+// c_off[j] = first coarse id of job j (c_off[nj] = nc) from the clustering.
+int launch_job_offsets(int32_t nj, const int32_t* v_off, const int32_t* order, const int32_t* cmap, int32_t nc,
+                       int32_t* c_off, hipStream_t st);
+// ce_off[j] = cxadj[c_off[j]].
+int launch_job_eoffsets(int32_t nj, const int32_t* c_off, const int32_t* cxadj, int32_t* ce_off, hipStream_t st);
+int launch_unpack_coarse(int32_t nj, const UnpackDst* dst, const int32_t* v_off, const int32_t* c_off,
+                         const int32_t* ce_off, int32_t n, int32_t nc, int32_t ce, const int32_t* cmap,
+                         const int32_t* cxadj, const int32_t* cadj, const int32_t* cw, const int32_t* cvw,
+                         hipStream_t st);
+int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, int32_t n, const int32_t* part,
+                       const int32_t* bnd, hipStream_t st);
+int launch_select_b(const DGraph& g, const int32_t* vjob, const int32_t* bnd, const int32_t* gain,
+                    const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
+                    int32_t* list, int32_t* ctl, int32_t* reset, hipStream_t st);
+int launch_keys_b(int32_t count, const int32_t* list, const int32_t* gain, const int32_t* vjob, int32_t gmax,
+                  int gbits, int idbits, uint64_t* keys, hipStream_t st);
+int launch_segments(int32_t nj, int32_t count, const uint64_t* keys, int shift, int32_t* seg_beg, int32_t* seg_end,
+                    hipStream_t st);
+int launch_balance_b(int32_t nj, int32_t kmax, const uint64_t* keys, const int32_t* seg_beg, const int32_t* seg_end,
+                     int32_t M, int idbits, const int32_t* kk, const long long* cap, const int32_t* pw_off,
+                     const int32_t* part, const int32_t* dest, const int32_t* gain, const int32_t* vw, int32_t* pw,
+                     int32_t* pos, long long* cut, int32_t* rounds, long long* moved, hipStream_t st);
+int launch_apply_b(const DGraph& g, int32_t count, const uint64_t* keys, int idbits, int shift,
+                   const int32_t* seg_beg, const int32_t* pos, const int32_t* dest, int32_t* part, int32_t* upd,
+                   int32_t* wl, int32_t* wlc, hipStream_t st);
+int launch_cut_b(const DGraph& g, const int32_t* vjob, const int32_t* part, const int32_t* pw_off,
+                 const int32_t* done, unsigned long long* cut, unsigned long long* pw64, hipStream_t st);
+
+}  // namespace gkd

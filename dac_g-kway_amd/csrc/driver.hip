@@ -24,229 +24,14 @@
 #include "host_graph.hpp"
 #include "kernels.hpp"
 #include "prim.hpp"
+#include "runtime.hpp"
 
 
 using gkd::DGraph;
 
-namespace {
+using namespace gkrt;
 
-constexpr int32_t kMaxK = 8192;
-constexpr int32_t kMaxJumpRounds = 40;
-
-// ---------------------------------------------------------------- pool
-// Caching device allocator: blocks return to a size-keyed free list and are
-// reused by later requests of up to twice smaller size, so steady-state
-// partitioning calls perform no hipMalloc.
-class Pool {
- public:
-  ~Pool() { release_all(); }
-  int get(size_t bytes, void** out) {
-    bytes = (bytes + 255) & ~size_t(255);
-    if (bytes == 0) bytes = 256;
-    auto it = free_.lower_bound(bytes);
-    if (it != free_.end() && it->first <= 2 * bytes) {
-      *out = it->second;
-      free_.erase(it);
-      return GK_OK;
-    }
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      trim();
-      if (hipMalloc(&p, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        gkh::set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
-        return GK_ERR_NOMEM;
-      }
-    }
-    size_[p] = bytes;
-    *out = p;
-    return GK_OK;
-  }
-  void put(void* p) {
-    if (!p) return;
-    auto it = size_.find(p);
-    if (it != size_.end()) free_.emplace(it->second, p);
-  }
-  void trim() {
-    for (auto& kv : free_) {
-      (void)hipFree(kv.second);
-      size_.erase(kv.second);
-    }
-    free_.clear();
-  }
-  void release_all() {
-    for (auto& kv : size_) (void)hipFree(kv.first);
-    size_.clear();
-    free_.clear();
-  }
-
- private:
-  std::multimap<size_t, void*> free_;
-  std::unordered_map<void*, size_t> size_;
-};
-
-template <class T>
-struct Buf {
-  Pool* pool = nullptr;
-  T* p = nullptr;
-  Buf() = default;
-  Buf(const Buf&) = delete;
-  Buf& operator=(const Buf&) = delete;
-  Buf(Buf&& o) noexcept : pool(o.pool), p(o.p) { o.p = nullptr; }
-  Buf& operator=(Buf&& o) noexcept {
-    if (this != &o) { release(); pool = o.pool; p = o.p; o.p = nullptr; }
-    return *this;
-  }
-  ~Buf() { release(); }
-  int alloc(Pool& pl, int64_t n) {
-    release();
-    pool = &pl;
-    void* q = nullptr;
-    int rc = pl.get(sizeof(T) * static_cast<size_t>(n > 0 ? n : 1), &q);
-    if (rc) return rc;
-    p = static_cast<T*>(q);
-    return GK_OK;
-  }
-  void release() {
-    if (pool && p) pool->put(p);
-    p = nullptr;
-  }
-};
-
-int bits_for(int64_t x) {  // bits needed to hold values 0..x
-  int b = 0;
-  while (b < 63 && (int64_t(1) << b) <= x) ++b;
-  return b < 1 ? 1 : b;
-}
-
-uint64_t fnv_host(const std::vector<int32_t>& a, uint64_t h = 0xcbf29ce484222325ull) {
-  for (int32_t x : a) { h ^= uint32_t(x); h *= 0x100000001b3ull; }
-  return h;
-}
-
-}  // namespace
-
-// Per-stage device timing: each timed call is bracketed by two HIP events
-// recorded on the work stream; elapsed times are summed at the end of a run.
-class StageClock {
- public:
-  ~StageClock() {
-    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
-  }
-  void start(bool on, hipStream_t st) {
-    on_ = on;
-    st_ = st;
-    for (auto& v : recs_) v.clear();
-    used_ = 0;
-  }
-  void begin(int id) {
-    if (!on_) return;
-    cur_[id] = take();
-    (void)hipEventRecord(cur_[id], st_);
-  }
-  void end(int id) {
-    if (!on_) return;
-    hipEvent_t e = take();
-    (void)hipEventRecord(e, st_);
-    recs_[id].emplace_back(cur_[id], e);
-  }
-  int collect(gk_stats& S) {
-    if (!on_) return GK_OK;
-    GK_HIP(hipStreamSynchronize(st_));
-    for (int id = 0; id < GK_ST_COUNT; ++id) {
-      double ms = 0;
-      for (auto& pr : recs_[id]) {
-        float t = 0;
-        GK_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
-        ms += t;
-      }
-      S.stage_ms[id] = ms;
-    }
-    return GK_OK;
-  }
-
- private:
-  hipEvent_t take() {
-    if (used_ == pool_.size()) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return nullptr;
-      pool_.push_back(e);
-    }
-    return pool_[used_++];
-  }
-  bool on_ = false;
-  hipStream_t st_ = nullptr;
-  std::vector<hipEvent_t> pool_;
-  size_t used_ = 0;
-  hipEvent_t cur_[GK_ST_COUNT] = {};
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> recs_[GK_ST_COUNT];
-};
-
-struct gk_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Pool pool;
-  char* hpin = nullptr;  // pinned staging for small read-backs
-  StageClock clock;
-  gk_stats* cur = nullptr;  // stats of the running partition (stage counters)
-};
-
-namespace {
-
-// Stage bracket: times `expr` when stage timing is on, counts the call and
-// adds the stage's algorithmic bytes.
-#define GK_STAGE(ctx, id, bytes, expr)                \
-  do {                                                \
-    (ctx)->clock.begin(id);                           \
-    int rc_s_ = (expr);                               \
-    (ctx)->clock.end(id);                             \
-    if (rc_s_) return rc_s_;                          \
-    count_stage(ctx, id, (int64_t)(bytes));           \
-  } while (0)
-
-inline void count_stage(gk_ctx* ctx, int id, int64_t bytes) {
-  if (!ctx->cur) return;
-  ctx->cur->stage_launches[id] += 1;
-  ctx->cur->stage_bytes[id] += bytes;
-}
-
-// Copy small device values to host variables through pinned staging, then
-// synchronise the stream once.
-struct Fetch {
-  void* host;
-  const void* dev;
-  size_t bytes;
-};
-int fetch(gk_ctx* ctx, hipStream_t st, std::initializer_list<Fetch> items) {
-  size_t off = 0;
-  for (const Fetch& f : items) {
-    GK_HIP(hipMemcpyAsync(ctx->hpin + off, f.dev, f.bytes, hipMemcpyDeviceToHost, st));
-    off += (f.bytes + 7) & ~size_t(7);
-  }
-  GK_HIP(hipStreamSynchronize(st));
-  off = 0;
-  for (const Fetch& f : items) {
-    std::memcpy(f.host, ctx->hpin + off, f.bytes);
-    off += (f.bytes + 7) & ~size_t(7);
-  }
-  return GK_OK;
-}
-
-template <class T>
-int to_host(hipStream_t st, std::vector<T>& out, const T* dev, int64_t n) {
-  out.resize(static_cast<size_t>(n));
-  if (n <= 0) return GK_OK;
-  GK_HIP(hipMemcpyAsync(out.data(), dev, sizeof(T) * n, hipMemcpyDeviceToHost, st));
-  GK_HIP(hipStreamSynchronize(st));
-  return GK_OK;
-}
-
-struct Level {
-  DGraph g;
-  Buf<int32_t> xadj, adj, w, vw;  // owned storage (coarse levels, or level-0 fills)
-  Buf<int32_t> cmap;              // fine -> coarse ids of the next level
-};
+namespace gkrt {
 
 // ---------------------------------------------------------------- stages
 // Grouping: pointer jumping to the root pair, then the component maximum.
@@ -341,7 +126,7 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
   GK_TRY(lists.alloc(P, int64_t(4) * nc));
   GK_TRY(lcnt.alloc(P, 4));
   GK_TRY(scratch.alloc(P, gkd::scan_scratch_ints(int64_t(nc) + 1) + 64));
-  GK_STAGE(ctx, GK_ST_CONTRACT, 12LL * g.n + 28LL * nc,
+  GK_STAGE(ctx, GK_ST_CONTRACT_AUX, 12LL * g.n + 28LL * nc,
            gkd::launch_contract_bounds(g, nc, cptr, order, ub.p, toff.p, lists.p, lcnt.p, scratch.p, st));
   int32_t hc[4];
   GK_TRY(fetch(ctx, st, {{hc, lcnt.p, sizeof hc}}));
@@ -354,6 +139,7 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
     GK_TRY(gkd::launch_contract_rows(g, lanes[cls], lists.p + int64_t(cls) * nc, hc[cls], cptr, order,
                                      cmap, toff.p, tadj.p, tw.p, len.p, st));
   ctx->clock.end(GK_ST_CONTRACT);
+  count_stage(ctx, GK_ST_CONTRACT, 0);  // bytes added once the coarse entry count is known
   if (hc[3] > 0) {
     const int32_t nbig = hc[3];
     const int32_t* rows = lists.p + int64_t(3) * nc;
@@ -385,14 +171,15 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
   }
   GK_TRY(cnnzd.alloc(P, 1));
   GK_TRY(c.xadj.alloc(P, int64_t(nc) + 1));
-  GK_STAGE(ctx, GK_ST_CONTRACT, 12LL * nc, gkd::launch_len_scan(nc, len.p, c.xadj.p, cnnzd.p, scratch.p, st));
+  GK_STAGE(ctx, GK_ST_CONTRACT_AUX, 12LL * nc, gkd::launch_len_scan(nc, len.p, c.xadj.p, cnnzd.p, scratch.p, st));
   int32_t cnnz = 0;
   GK_TRY(fetch(ctx, st, {{&cnnz, cnnzd.p, sizeof cnnz}}));
   GK_TRY(c.adj.alloc(P, cnnz));
   GK_TRY(c.w.alloc(P, cnnz));
-  // Row bytes (member rows read once, merged rows written twice, read once)
-  // are counted with the copy, when the coarse entry count is known.
-  GK_STAGE(ctx, GK_ST_CONTRACT, 12LL * nc + 16LL * cnnz + 12LL * g.n + 20LL * nc + 12LL * g.nnz + 8LL * cnnz,
+  // Row merge: member rows and the cmap gathers read once, merged rows
+  // written once (known now that cnnz is).
+  if (ctx->cur) ctx->cur->stage_bytes[GK_ST_CONTRACT] += 12LL * g.n + 20LL * nc + 12LL * g.nnz + 8LL * cnnz;
+  GK_STAGE(ctx, GK_ST_CONTRACT_AUX, 12LL * nc + 16LL * cnnz,
            gkd::launch_contract_copy(nc, toff.p, len.p, c.xadj.p, tadj.p, tw.p, c.adj.p, c.w.p, st));
   c.g.n = nc;
   c.g.nnz = cnnz;
@@ -408,6 +195,10 @@ int gain_lanes(const DGraph& g) {
   while (s < 64 && s < 2.0 * avg) s <<= 1;
   return s;
 }
+
+}  // namespace gkrt
+
+namespace {
 
 // ---------------------------------------------------------------- V-cycle
 int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params& prm,
@@ -583,6 +374,8 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       GK_STAGE(ctx, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
                                    work.p, g.n, st));
+      GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
+               gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
       // A round updates at most B movers and their neighbours.
       const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
       const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
@@ -626,6 +419,8 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
         GK_STAGE(ctx, GK_ST_GAIN, 0,
                  gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
                                      work.p, int64_t(B) * (1 + 2 * avg_deg), st));
+        GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
+                 gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
         pending = true;
       }
       long long hcut = 0;
@@ -948,6 +743,7 @@ int gk_k_gain(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, con
   gkd::fill<int32_t>(cnt.p, 0, 2, st);
   GK_TRY(gkd::launch_compact(g.n, flags.p, wl.p, cnt.p, st));
   GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, cnt.p, heavy.p, part, flags.p, bnd, gain, dest, nullptr, g.n, st));
+  GK_TRY(gkd::launch_gain_heavy(g, k, heavy.p, cnt.p, part, flags.p, bnd, gain, dest, nullptr, st));
   GK_HIP(hipStreamSynchronize(st));
   return GK_OK;
 }

@@ -8,6 +8,7 @@
 
 #include "kernels.hpp"
 #include "prim.hpp"
+#include "balance.hpp"
 #include "../../include/gkway.h"
 
 namespace gkd {
@@ -243,6 +244,13 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
       break;
   }
 #undef GK_GAIN_CASE
+  return hip_status();
+}
+
+int launch_gain_heavy(const DGraph& g, int32_t k, const int32_t* heavy, const int32_t* wlc, const int32_t* part,
+                      int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest, unsigned long long* work,
+                      hipStream_t st) {
+  if (g.n <= 0) return 0;
   k_gain_heavy<<<256, 64, sizeof(int32_t) * k, st>>>(heavy, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain,
                                                      dest, work);
   return hip_status();
@@ -316,16 +324,8 @@ int launch_select(const DGraph& g, const int32_t* bnd, const int32_t* gain, cons
   return hip_status();
 }
 
-// One workgroup: order the (at most 1024) first moves, then find the largest
-// balanced prefix (find_mv_wgt_sequence + scan + find_balance_sequence,
-// gkway/uncoarsen.hpp:239-288, 37-80).  Instead of the reference's dense
-// k x B delta matrix, each wave owns a set of parts and runs a wave64 prefix
-// scan of that part's weight delta along the move sequence; positions where
-// the part would exceed the cap are OR-ed into a bitmap, and the answer is the
-// highest clear bit.  Then the prefix is committed: pw advanced, cut reduced.
-constexpr int kBalThreads = 1024;
-constexpr int kBalMax = 1024;
-
+// One workgroup: order the (at most 1024) first moves, then the largest
+// balanced prefix (balance.hpp); ctl[2] receives its last index.
 __global__ void __launch_bounds__(kBalThreads) k_balance(int32_t k, const uint64_t* __restrict__ keys,
                                                          int32_t count, int32_t M, int presorted,
                                                          int idbits, const int32_t* __restrict__ part,
@@ -336,92 +336,10 @@ __global__ void __launch_bounds__(kBalThreads) k_balance(int32_t k, const uint64
                                                          int32_t* __restrict__ moves,
                                                          int32_t* __restrict__ ctl,
                                                          long long* __restrict__ cut) {
-  __shared__ uint64_t sk[kBalMax];
-  __shared__ int32_t sv[kBalMax], ssrc[kBalMax], sdst[kBalMax], sw[kBalMax], sg[kBalMax];
-  __shared__ unsigned long long bad[kBalMax / 64];
-  __shared__ int32_t spos;
-  __shared__ long long sgain[kBalThreads / 64];
+  __shared__ BalanceLds L;
   extern __shared__ int32_t dpw[];  // k entries
-  const int t = threadIdx.x;
-  const int B = count < M ? count : M;
-  sk[t] = (t < (presorted ? B : count)) ? keys[t] : ~0ull;
-  if (t < kBalMax / 64) bad[t] = 0;
-  for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
-  __syncthreads();
-  if (!presorted) {
-    // Bitonic sort of the first P = next_pow2(count) keys (unused slots hold
-    // the maximum key); P is uniform across the block.
-    int P = 2;
-    while (P < count) P <<= 1;
-    for (int k2 = 2; k2 <= P; k2 <<= 1) {
-      for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
-        const int ix = t ^ j2;
-        if (t < P && ix > t) {
-          const uint64_t a = sk[t], b = sk[ix];
-          const bool up = (t & k2) == 0;
-          if (up ? a > b : a < b) { sk[t] = b; sk[ix] = a; }
-        }
-        __syncthreads();
-      }
-    }
-  }
   const uint64_t idmask = (idbits >= 64) ? ~0ull : ((1ull << idbits) - 1);
-  if (t < B) {
-    const int32_t v = int32_t(sk[t] & idmask);
-    sv[t] = v;
-    ssrc[t] = part[v];
-    sdst[t] = dest[v];
-    sw[t] = vw[v];
-    sg[t] = gain[v];
-  }
-  __syncthreads();
-  const int lane = lane_id();
-  const int wave = t >> 6;
-  const int nchunk = (B + 63) >> 6;
-  for (int32_t q = wave; q < k; q += kBalThreads / 64) {
-    const long long pq = pw[q];
-    long long carry = 0;
-    for (int c = 0; c < nchunk; ++c) {
-      const int j = c * 64 + lane;
-      int x = 0;
-      if (j < B) {
-        if (sdst[j] == q) x += sw[j];
-        if (ssrc[j] == q) x -= sw[j];
-      }
-      const long long incl = carry + wave_incl_scan(x);
-      const bool b = j < B && incl > 0 && pq + incl > cap;
-      const uint64_t m = ballot(b);
-      if (m != 0 && lane == 0) atomicOr(&bad[c], (unsigned long long)m);
-      carry = __shfl(incl, 63, 64);
-    }
-  }
-  __syncthreads();
-  if (t == 0) {
-    int pos = -1;
-    for (int i = B - 1; i >= 0; --i) {
-      if (((bad[i >> 6] >> (i & 63)) & 1ull) == 0) { pos = i; break; }
-    }
-    spos = pos;
-  }
-  __syncthreads();
-  const int pos = spos;
-  long long gsum = 0;
-  if (t <= pos) {
-    atomicAdd(&dpw[ssrc[t]], -sw[t]);
-    atomicAdd(&dpw[sdst[t]], sw[t]);
-    moves[t] = sv[t];
-    gsum = sg[t];
-  }
-  gsum = wave_sum64(gsum);
-  if (lane == 0) sgain[wave] = gsum;
-  __syncthreads();
-  for (int32_t q = t; q < k; q += kBalThreads) pw[q] += dpw[q];
-  if (t == 0) {
-    long long s = 0;
-    for (int w2 = 0; w2 < kBalThreads / 64; ++w2) s += sgain[w2];
-    *cut -= s;
-    ctl[2] = pos;
-  }
+  balance_block(L, dpw, k, keys, count, M, presorted, idmask, part, dest, gain, vw, pw, cap, moves, &ctl[2], cut);
 }
 
 int launch_balance(const DGraph& g, int32_t k, const uint64_t* keys, int32_t count, int32_t M,

@@ -93,7 +93,7 @@ typedef struct gk_params {
 #define GK_ST_GROUP 1      /* pointer jumping + component labels           */
 #define GK_ST_SORT 2       /* radix sort by (label, stamp, id)             */
 #define GK_ST_CLUSTER 3    /* chunking, coarse ids, coarse vertex weights  */
-#define GK_ST_CONTRACT 4   /* row bounds, row merge, copy                  */
+#define GK_ST_CONTRACT 4   /* per-row sort + merge (rows <= 64 entries)    */
 #define GK_ST_CONTRACT_BIG 5 /* gather + sort + merge of rows > 64 entries */
 #define GK_ST_PROJECT 6
 #define GK_ST_GAIN 7
@@ -102,6 +102,9 @@ typedef struct gk_params {
 #define GK_ST_BALANCE 10
 #define GK_ST_APPLY 11
 #define GK_ST_CUT 12       /* final cut and part weights                   */
+#define GK_ST_GAIN_HEAVY 13 /* gain of rows longer than the lane count     */
+#define GK_ST_CONTRACT_AUX 14 /* row bounds, offset scans, final copy      */
+#define GK_ST_PACK 15      /* batched form: copies of levels into a pack   */
 #define GK_ST_COUNT 16
 
 /* Fill *p with the defaults: ubfactor 30, max_group 6, max_moves 1024,
@@ -162,6 +165,82 @@ int gk_partition_device(gk_ctx* ctx, const gk_csr* dev_graph, const gk_params* p
  * "<OUT_FILE>.out" (path taken as given; the C++ wrapper prefixes the cwd). */
 int gk_partition_file(gk_ctx* ctx, const char* graph_file, const char* out_file,
                       int32_t k, gk_stats* stats);
+
+/* ---- batched form ------------------------------------------------------------
+ * Many graphs partitioned together: every stage of every level is one launch
+ * over all graphs in flight (graphs packed into one CSR with global ids), and
+ * the host initial partitions of some graphs overlap the device work of the
+ * others.  Each graph's result is bit-identical to gk_partition on that graph
+ * alone (no counterpart in the reference, which partitions one graph per
+ * process run).
+ *
+ * gk_packed_csr: graphs concatenated; graph g owns vertices [v_off[g],
+ * v_off[g+1]) and entries [e_off[g], e_off[g+1]); xadj is global and adjncy
+ * holds global vertex ids inside the graph's own range.  Arrays are device
+ * pointers except v_off / e_off (host).                                       */
+typedef struct gk_packed_csr {
+  int32_t ngraphs;
+  int32_t n;
+  int32_t nnz;
+  const int32_t* xadj;     /* device, n + 1                                   */
+  const int32_t* adjncy;   /* device, nnz                                     */
+  const int32_t* adjwgt;   /* device, nnz, or NULL (unit)                     */
+  const int32_t* vwgt;     /* device, n, or NULL (unit)                       */
+  const int32_t* v_off;    /* host, ngraphs + 1                               */
+  const int32_t* e_off;    /* host, ngraphs + 1                               */
+} gk_packed_csr;
+
+/* params[g] and stats[g] per graph (stats may be NULL); dev_part_out[n]
+ * receives each graph's partition (part ids local to the graph). */
+int gk_partition_packed(gk_ctx* ctx, const gk_packed_csr* pack, const gk_params* params,
+                        int32_t* dev_part_out, gk_stats* stats, void* stream);
+/* Host graphs in, host partitions out (packed and copied inside). */
+int gk_partition_batch(gk_ctx* ctx, int32_t count, const gk_csr* graphs, const gk_params* params,
+                       int32_t* const* parts_out, gk_stats* stats);
+
+/* ---- serving form --------------------------------------------------------------
+ * A request loop on its own host thread with continuous batching: submitted
+ * graphs wait in FIFO order until their device footprint (CSR bytes times the
+ * estimated level count, plus per-vertex scratch) fits the HBM budget, are
+ * then admitted and join the batch engine at the next step boundary, and
+ * advance level-synchronously with every other request in flight.  Each
+ * result is bit-identical to gk_partition on that graph alone.  While a server
+ * is open it owns the context: make no other call on that gk_ctx.  Every
+ * request of one server must use max_group and max_moves equal to the first
+ * request's.  No counterpart in the reference.                                */
+typedef struct gk_server gk_server;
+
+typedef struct gk_serve_config {
+  int64_t hbm_budget;      /* device bytes admitted requests may hold
+                              (0 = half of the free HBM at open)             */
+  int32_t max_inflight;    /* requests stepped together (0 = no limit)        */
+  int32_t workers;         /* host threads for initial partitions (0 = auto)  */
+} gk_serve_config;
+
+typedef struct gk_serve_stats {
+  int64_t submitted, completed, failed, queued, inflight;
+  int64_t edges_done;      /* undirected edges of completed requests         */
+  int64_t steps;           /* engine steps run                               */
+  int64_t footprint_now;   /* admitted footprint (estimate), bytes           */
+  int64_t footprint_peak;
+  int64_t device_peak;     /* peak device bytes held by the context's pool   */
+  double p50_ms, p99_ms, max_ms;  /* latency = completion - submission      */
+  double edges_per_s;      /* edges_done / (last completion - first submit)  */
+} gk_serve_stats;
+
+int gk_serve_open(gk_ctx* ctx, const gk_serve_config* cfg, gk_server** out);
+/* Copies the host graph; *request receives the request id. */
+int gk_serve_submit(gk_server* s, const gk_csr* graph, const gk_params* params, int64_t* request);
+/* 1 = finished (part_out[n] and *stats written, the request is released),
+ * 0 = still queued or running, < 0 = the request failed (also released). */
+int gk_serve_poll(gk_server* s, int64_t request, int32_t* part_out, gk_stats* stats);
+/* Blocks until the request finishes; GK_OK or its error. */
+int gk_serve_wait(gk_server* s, int64_t request, int32_t* part_out, gk_stats* stats);
+int gk_serve_query(gk_server* s, gk_serve_stats* out);
+/* Footprint estimate used for admission (bytes) of a graph at parameters p. */
+int64_t gk_serve_footprint(int32_t n, int32_t nnz, const gk_params* p);
+/* Finishes every queued and running request, then stops the loop. */
+int gk_serve_close(gk_server* s);
 
 /* ---- host I/O --------------------------------------------------------------
  * gk_graph_read allocates the arrays with malloc; free with gk_graph_free.    */

@@ -54,6 +54,50 @@ def test_cap_and_floor_match_oracle(gk, oracle):
     assert L.gk_coarsen_floor(1 << 20, 64, 0, 0) == (1 << 20) // 120
 
 
+def test_serve_footprint(gk):
+    L = gk.lib()
+    p = gk.params(8)
+    assert L.gk_serve_footprint(0, 0, ctypes.byref(p)) == 0
+    # At or below the coarsening floor there is one level: CSR + scratch.
+    n, nnz = 200, 1000
+    assert L.gk_coarsen_floor(n, 8, 0, 0) >= n
+    csr = 4 * (n + 1) + 8 * nnz + 4 * n
+    assert L.gk_serve_footprint(n, nnz, ctypes.byref(p)) == csr + 96 * n + 4096
+    # Above it, the CSR term is multiplied by the estimated level count
+    # (levels shrink at least 3x down to the floor).
+    n, nnz = 1 << 20, 6 << 20
+    fl = L.gk_coarsen_floor(n, 8, 0, 0)
+    lv, m = 1, n
+    while m > fl:
+        m //= 3
+        lv += 1
+    csr = 4 * (n + 1) + 8 * nnz + 4 * n
+    assert L.gk_serve_footprint(n, nnz, ctypes.byref(p)) == csr * lv + 96 * n + 4096
+    assert L.gk_serve_footprint(n, nnz + 10, ctypes.byref(p)) > L.gk_serve_footprint(n, nnz, ctypes.byref(p))
+
+
+def test_pack_graphs_layout(gk):
+    gs = [gk.Graph([0, 1, 2], [1, 0]), gk.Graph([0], []), gk.Graph([0, 2, 3, 4], [1, 2, 0, 0], [5, 6, 5, 6], [1, 2, 3])]
+    pg, v_off, e_off = gk.pack_graphs(gs)
+    assert list(v_off) == [0, 2, 2, 5] and list(e_off) == [0, 2, 2, 6]
+    assert list(pg.xadj) == [0, 1, 2, 4, 5, 6]
+    assert list(pg.adjncy) == [1, 0, 3, 4, 2, 2]
+    assert list(pg.adjwgt) == [1, 1, 5, 6, 5, 6] and list(pg.vwgt) == [1, 1, 1, 2, 3]
+    for i, g in enumerate(gs):
+        xs = pg.xadj[v_off[i]:v_off[i + 1] + 1] - e_off[i]
+        assert np.array_equal(xs, g.xadj)
+        assert np.array_equal(pg.adjncy[e_off[i]:e_off[i + 1]] - v_off[i], g.adjncy)
+
+
+def test_batch_and_serve_argument_checks(gk):
+    L = gk.lib()
+    assert L.gk_partition_batch(None, 1, None, None, None, None) == -1
+    assert L.gk_partition_packed(None, None, None, None, None, None) == -1
+    h = ctypes.c_void_p()
+    assert L.gk_serve_open(None, None, ctypes.byref(h)) == -1
+    assert L.gk_serve_poll(None, 0, None, None) == -1
+
+
 def test_no_device_is_a_status_not_a_crash(gk):
     L = gk.lib()
     h = ctypes.c_void_p()
