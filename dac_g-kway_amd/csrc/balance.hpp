@@ -24,41 +24,39 @@ struct BalanceLds {
   long long sgain[kBalThreads / 64];
 };
 
-// keys: the move keys (vertex id in the low bits, idmask); when !presorted
-// the first `count` (<= kBalMax) keys are sorted here, else the first
-// min(count, M) are taken in order.  dpw: k ints of LDS.  Writes the chosen
-// prefix end to *pos_out (-1 only for an empty sequence), the moved vertices
-// to moves[0..pos] when moves != null, advances pw and subtracts the prefix's
-// gains from *cut.
-__device__ inline void balance_block(BalanceLds& L, int32_t* dpw, int32_t k, const uint64_t* __restrict__ keys,
-                                     int32_t count, int32_t M, int presorted, uint64_t idmask,
+// Ascending bitonic sort of L.sk[0, P), P = next_pow2(count) (<= kBalMax);
+// slots in [count, P) must hold the maximum key.  All threads of the block
+// call it.
+__device__ inline void lds_bitonic(BalanceLds& L, int32_t count) {
+  const int t = threadIdx.x;
+  int P = 2;
+  while (P < count) P <<= 1;
+  for (int k2 = 2; k2 <= P; k2 <<= 1) {
+    for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
+      const int ix = t ^ j2;
+      if (t < P && ix > t) {
+        const uint64_t a = L.sk[t], b = L.sk[ix];
+        const bool up = (t & k2) == 0;
+        if (up ? a > b : a < b) { L.sk[t] = b; L.sk[ix] = a; }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Largest balanced prefix of the B sorted moves in L.sk (vertex id in the
+// bits of idmask).  dpw: k ints of LDS.  Fills L.sv/ssrc/sdst/sw/sg, writes
+// the prefix end to *pos_out (-1 only for B == 0) and, when moves != null,
+// the moved vertices to moves[0..pos]; advances pw and subtracts the
+// prefix's gains from *cut.  Returns the prefix end to every thread.
+__device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int32_t B, uint64_t idmask,
                                      const int32_t* __restrict__ part, const int32_t* __restrict__ dest,
                                      const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
                                      int32_t* __restrict__ pw, long long cap, int32_t* __restrict__ moves,
                                      int32_t* __restrict__ pos_out, long long* __restrict__ cut) {
   const int t = threadIdx.x;
-  const int B = count < M ? count : M;
-  L.sk[t] = (t < (presorted ? B : count)) ? keys[t] : ~0ull;
   if (t < kBalMax / 64) L.bad[t] = 0;
   for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
-  __syncthreads();
-  if (!presorted) {
-    // Bitonic sort of the first P = next_pow2(count) keys (unused slots hold
-    // the maximum key); P is uniform across the block.
-    int P = 2;
-    while (P < count) P <<= 1;
-    for (int k2 = 2; k2 <= P; k2 <<= 1) {
-      for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
-        const int ix = t ^ j2;
-        if (t < P && ix > t) {
-          const uint64_t a = L.sk[t], b = L.sk[ix];
-          const bool up = (t & k2) == 0;
-          if (up ? a > b : a < b) { L.sk[t] = b; L.sk[ix] = a; }
-        }
-        __syncthreads();
-      }
-    }
-  }
   if (t < B) {
     const int32_t v = int32_t(L.sk[t] & idmask);
     L.sv[t] = v;
@@ -113,8 +111,26 @@ __device__ inline void balance_block(BalanceLds& L, int32_t* dpw, int32_t k, con
     long long s = 0;
     for (int w2 = 0; w2 < kBalThreads / 64; ++w2) s += L.sgain[w2];
     *cut -= s;
-    *pos_out = pos;
+    if (pos_out) *pos_out = pos;
   }
+  return pos;
+}
+
+// keys: the move keys (vertex id in the low bits, idmask); when !presorted
+// the first `count` (<= kBalMax) keys are sorted here, else the first
+// min(count, M) are taken in order.  Then balance_prefix.
+__device__ inline void balance_block(BalanceLds& L, int32_t* dpw, int32_t k, const uint64_t* __restrict__ keys,
+                                     int32_t count, int32_t M, int presorted, uint64_t idmask,
+                                     const int32_t* __restrict__ part, const int32_t* __restrict__ dest,
+                                     const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
+                                     int32_t* __restrict__ pw, long long cap, int32_t* __restrict__ moves,
+                                     int32_t* __restrict__ pos_out, long long* __restrict__ cut) {
+  const int t = threadIdx.x;
+  const int B = count < M ? count : M;
+  L.sk[t] = (t < (presorted ? B : count)) ? keys[t] : ~0ull;
+  __syncthreads();
+  if (!presorted) lds_bitonic(L, count);
+  balance_prefix(L, dpw, k, B, idmask, part, dest, gain, vw, pw, cap, moves, pos_out, cut);
 }
 
 }  // namespace gkd

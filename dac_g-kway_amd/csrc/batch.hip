@@ -2,14 +2,14 @@
 // global vertex ids, processed by one launch per stage.  Jobs never share an
 // edge, so the coarsening kernels of coarsen.hip run on the pack unchanged;
 // this file adds the per-job glue:
-//   pack / unpack    descriptor-driven copies between per-job storage (local
-//                    ids) and the pack (global ids), one launch each;
+//   pack / unpack    descriptor-driven copies from per-job slices (local ids)
+//                    into a pack (global ids), and of partitions back out;
 //   job offsets      first coarse id / first coarse entry of every job;
-//   select / keys    independent-set selection with per-job caps and part
-//                    weights, then keys ordered by (job, gain desc, id asc);
-//   segments         per-job ranges of the sorted keys;
-//   balance          one workgroup per job (balance.hpp);
-//   apply, cut       flat over the pack with per-job offsets.
+//   rounds           independent-set selection with per-job caps and part
+//                    weights as a bitmask, then one workgroup per job for the
+//                    ordered move list, the best moves by (gain desc, id asc),
+//                    the balanced prefix (balance.hpp) and the apply;
+//   cut              final cut and part weights with per-job offsets.
 // Every per-job result equals the single-graph path's: ids keep their order
 // inside a job and every order-sensitive step is segmented by job.
 #include <algorithm>
@@ -138,61 +138,6 @@ int launch_job_eoffsets(int32_t nj, const int32_t* c_off, const int32_t* cxadj, 
 }
 
 // ---------------------------------------------------------------- unpack
-// Coarse level of the pack -> per-job storage with local ids; jobs whose
-// destination is null (level discarded) are skipped.
-__global__ void __launch_bounds__(256) k_unpack_cvert(int32_t nj, const UnpackDst* __restrict__ dst,
-                                                      const int32_t* __restrict__ c_off,
-                                                      const int32_t* __restrict__ ce_off, int32_t nc,
-                                                      const int32_t* __restrict__ cxadj,
-                                                      const int32_t* __restrict__ cvw) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= nc) return;
-  const int32_t j = job_of(c_off, nj, i);
-  const UnpackDst d = dst[j];
-  if (!d.xadj) return;
-  const int32_t li = int32_t(i - c_off[j]);
-  d.xadj[li] = cxadj[i] - ce_off[j];
-  d.vw[li] = cvw[i];
-  if (i + 1 == c_off[j + 1]) d.xadj[li + 1] = cxadj[i + 1] - ce_off[j];
-}
-
-__global__ void __launch_bounds__(256) k_unpack_cedge(int32_t nj, const UnpackDst* __restrict__ dst,
-                                                      const int32_t* __restrict__ c_off,
-                                                      const int32_t* __restrict__ ce_off, int32_t ce,
-                                                      const int32_t* __restrict__ cadj,
-                                                      const int32_t* __restrict__ cw) {
-  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (e >= ce) return;
-  const int32_t j = job_of(ce_off, nj, e);
-  const UnpackDst d = dst[j];
-  if (!d.adj) return;
-  const int32_t le = int32_t(e - ce_off[j]);
-  d.adj[le] = cadj[e] - c_off[j];
-  d.w[le] = cw[e];
-}
-
-__global__ void __launch_bounds__(256) k_unpack_cmap(int32_t nj, const UnpackDst* __restrict__ dst,
-                                                     const int32_t* __restrict__ v_off,
-                                                     const int32_t* __restrict__ c_off, int32_t n,
-                                                     const int32_t* __restrict__ cmap) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int32_t j = job_of(v_off, nj, i);
-  const UnpackDst d = dst[j];
-  if (!d.cmap) return;
-  d.cmap[i - v_off[j]] = cmap[i] - c_off[j];
-}
-
-int launch_unpack_coarse(int32_t nj, const UnpackDst* dst, const int32_t* v_off, const int32_t* c_off,
-                         const int32_t* ce_off, int32_t n, int32_t nc, int32_t ce, const int32_t* cmap,
-                         const int32_t* cxadj, const int32_t* cadj, const int32_t* cw, const int32_t* cvw,
-                         hipStream_t st) {
-  if (nc > 0) k_unpack_cvert<<<grid_for(nc, 256), 256, 0, st>>>(nj, dst, c_off, ce_off, nc, cxadj, cvw);
-  if (ce > 0) k_unpack_cedge<<<grid_for(ce, 256), 256, 0, st>>>(nj, dst, c_off, ce_off, ce, cadj, cw);
-  if (n > 0) k_unpack_cmap<<<grid_for(n, 256), 256, 0, st>>>(nj, dst, v_off, c_off, n, cmap);
-  return hip_status();
-}
-
 // Refined partition of the pack -> per-job arrays.
 __global__ void __launch_bounds__(256) k_unpack_part(int32_t nj, const UnpackDst* __restrict__ dst,
                                                      const int32_t* __restrict__ v_off, int32_t n,
@@ -213,29 +158,34 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
   return hip_status();
 }
 
-// ---------------------------------------------------------------- selection
-// create_independent_move_buffer (gkway/uncoarsen.hpp:111-173) on the pack:
-// the cap and the part weights are those of the vertex's job.  Appends the
-// selected vertex ids; ctl[0] = count, ctl[1] = max gain.
-__global__ void __launch_bounds__(256) k_select_b(int32_t n, const int32_t* __restrict__ xadj,
-                                                  const int32_t* __restrict__ adj,
-                                                  const int32_t* __restrict__ vw,
-                                                  const int32_t* __restrict__ vjob,
-                                                  const int32_t* __restrict__ bnd,
-                                                  const int32_t* __restrict__ gain,
-                                                  const int32_t* __restrict__ dest,
-                                                  const int32_t* __restrict__ pw,
-                                                  const int32_t* __restrict__ pw_off,
-                                                  const long long* __restrict__ cap,
-                                                  int32_t* __restrict__ list, int32_t* __restrict__ ctl,
-                                                  int32_t* __restrict__ reset) {
+// ---------------------------------------------------------------- rounds
+// Selection of a refinement round over the pack as a bitmask (one word per
+// wave of 64 consecutive vertices): create_independent_move_buffer
+// (gkway/uncoarsen.hpp:111-173) with the cap and part weights of the vertex's
+// job.  Thread 0
+// also clears the worklist counters and the round's move counter.
+__global__ void __launch_bounds__(256) k_select_bits(int32_t n, const int32_t* __restrict__ xadj,
+                                                     const int32_t* __restrict__ adj,
+                                                     const int32_t* __restrict__ vw,
+                                                     const int32_t* __restrict__ vjob,
+                                                     const int32_t* __restrict__ bnd,
+                                                     const int32_t* __restrict__ gain,
+                                                     const int32_t* __restrict__ dest,
+                                                     const int32_t* __restrict__ pw,
+                                                     const int32_t* __restrict__ pw_off,
+                                                     const long long* __restrict__ cap,
+                                                     uint64_t* __restrict__ bits, int32_t* __restrict__ reset,
+                                                     int32_t* __restrict__ active) {
   const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (reset && v == 0) { reset[0] = 0; reset[1] = 0; }
+  if (v == 0) {
+    reset[0] = 0;
+    reset[1] = 0;
+    active[0] = 0;
+  }
   bool sel = false;
-  int32_t g = 0;
-  if (v < n) {
-    g = gain[v];
-    if (bnd[v] && g > 0) {
+  if (v < n && bnd[v]) {
+    const int32_t g = gain[v];
+    if (g > 0) {
       const int32_t j = vjob[v];
       const int32_t* jpw = pw + pw_off[j];
       const long long jc = cap[j];
@@ -249,173 +199,259 @@ __global__ void __launch_bounds__(256) k_select_b(int32_t n, const int32_t* __re
     }
   }
   const uint64_t m = ballot(sel);
-  if (m == 0) return;
-  const int leader = __ffsll(static_cast<long long>(m)) - 1;
-  const int gmax = wave_max(sel ? g : 0);
-  int base = 0;
-  if (lane_id() == leader) {
-    base = atomicAdd(&ctl[0], popc64(m));
-    atomicMax(&ctl[1], gmax);
-  }
-  base = __shfl(base, leader, 64);
-  if (sel) list[base + popc64(m & lanemask_lt())] = int32_t(v);
+  if (lane_id() == 0 && (v >> 6) <= (int64_t(n) - 1) / 64) bits[v >> 6] = m;
 }
 
-int launch_select_b(const DGraph& g, const int32_t* vjob, const int32_t* bnd, const int32_t* gain,
-                    const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                    int32_t* list, int32_t* ctl, int32_t* reset, hipStream_t st) {
-  fill<int32_t>(ctl, 0, 2, st);
+int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* bnd, const int32_t* gain,
+                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
+                       uint64_t* bits, int32_t* reset, int32_t* active, hipStream_t st) {
   if (g.n <= 0) return 0;
-  k_select_b<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, vjob, bnd, gain, dest, pw, pw_off, cap,
-                                                 list, ctl, reset);
+  k_select_bits<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, vjob, bnd, gain, dest, pw, pw_off, cap,
+                                                    bits, reset, active);
   return hip_status();
 }
 
-// key = job << (gbits + idbits) | (gmax - gain) << idbits | v: ascending order
-// is (job, gain desc, id asc) -- the mvRequest order (gkway/move_request.hpp:
-// 36-41) inside each job.
-__global__ void __launch_bounds__(256) k_keys_b(int32_t count, const int32_t* __restrict__ list,
-                                                const int32_t* __restrict__ gain,
-                                                const int32_t* __restrict__ vjob, int32_t gmax, int gbits,
-                                                int idbits, uint64_t* __restrict__ keys) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= count) return;
-  const int32_t v = list[i];
-  keys[i] = (uint64_t(uint32_t(vjob[v])) << (gbits + idbits)) | (uint64_t(uint32_t(gmax - gain[v])) << idbits) |
-            uint64_t(uint32_t(v));
-}
-
-int launch_keys_b(int32_t count, const int32_t* list, const int32_t* gain, const int32_t* vjob, int32_t gmax,
-                  int gbits, int idbits, uint64_t* keys, hipStream_t st) {
-  if (count <= 0) return 0;
-  k_keys_b<<<grid_for(count, 256), 256, 0, st>>>(count, list, gain, vjob, gmax, gbits, idbits, keys);
-  return hip_status();
-}
-
-__global__ void __launch_bounds__(256) k_segments(int32_t count, const uint64_t* __restrict__ keys, int shift,
-                                                  int32_t* __restrict__ seg_beg, int32_t* __restrict__ seg_end) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= count) return;
-  const uint64_t j = keys[i] >> shift;
-  if (i == 0 || (keys[i - 1] >> shift) != j) seg_beg[j] = int32_t(i);
-  if (i + 1 == count || (keys[i + 1] >> shift) != j) seg_end[j] = int32_t(i + 1);
-}
-
-int launch_segments(int32_t nj, int32_t count, const uint64_t* keys, int shift, int32_t* seg_beg, int32_t* seg_end,
-                    hipStream_t st) {
-  fill<int32_t>(seg_beg, 0, nj, st);
-  fill<int32_t>(seg_end, 0, nj, st);
-  if (count > 0) k_segments<<<grid_for(count, 256), 256, 0, st>>>(count, keys, shift, seg_beg, seg_end);
-  return hip_status();
-}
-
-// One workgroup per job: its first min(count_j, M) moves (presorted).  Per-job
-// round and move counters advance for jobs that had moves.
-__global__ void __launch_bounds__(kBalThreads) k_balance_b(const uint64_t* __restrict__ keys,
-                                                           const int32_t* __restrict__ seg_beg,
-                                                           const int32_t* __restrict__ seg_end, int32_t M,
-                                                           uint64_t idmask, const int32_t* __restrict__ kk,
-                                                           const long long* __restrict__ cap,
-                                                           const int32_t* __restrict__ pw_off,
-                                                           const int32_t* __restrict__ part,
-                                                           const int32_t* __restrict__ dest,
-                                                           const int32_t* __restrict__ gain,
-                                                           const int32_t* __restrict__ vw, int32_t* __restrict__ pw,
-                                                           int32_t* __restrict__ pos, long long* __restrict__ cut,
-                                                           int32_t* __restrict__ rounds, long long* __restrict__ moved) {
-  __shared__ BalanceLds L;
-  extern __shared__ int32_t dpw[];
-  const int32_t j = blockIdx.x;
-  const int32_t b = seg_beg[j], cnt = seg_end[j] - b;
-  if (cnt <= 0) {
-    if (threadIdx.x == 0) pos[j] = -1;
-    return;
-  }
-  balance_block(L, dpw, kk[j], keys + b, cnt, M, 1, idmask, part, dest, gain, vw, pw + pw_off[j], cap[j], nullptr,
-                &pos[j], &cut[j]);
+// Exclusive scan of one int per thread over a kBalThreads block; *tot gets
+// the block total.  ws: >= kBalThreads / 64 + 1 ints of LDS.
+__device__ __forceinline__ int block_excl_scan_big(int x, int* tot, int32_t* ws) {
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int incl = wave_incl_scan(x);
+  if (lane == 63) ws[wave] = incl;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < kBalThreads / 64; ++w) {
+      const int t = ws[w];
+      ws[w] = run;
+      run += t;
+    }
+    ws[kBalThreads / 64] = run;
+  }
+  __syncthreads();
+  const int r = incl - x + ws[wave];
+  *tot = ws[kBalThreads / 64];
+  __syncthreads();
+  return r;
+}
+
+struct RoundLds {
+  BalanceLds b;
+  int32_t hist[256];
+  int32_t ws[kBalThreads / 64 + 1];
+  int32_t misc[4];
+};
+
+// Move key of a selected vertex: ascending order = gain descending, id
+// ascending (the mvRequest order of gkway/move_request.hpp:36-41).
+__device__ __forceinline__ uint64_t move_key(const int32_t* __restrict__ gain, int32_t v) {
+  return (uint64_t(uint32_t(0x7fffffff - gain[v])) << 32) | uint64_t(uint32_t(v));
+}
+
+// One refinement round of one job per workgroup, after k_select_bits:
+//   1. ordered compaction of the job's selected vertices (ascending ids) into
+//      list[v_off[j] ...];
+//   2. the min(count, M) best moves by (gain desc, id asc): all of them when
+//      count <= M, else a radix select on the gain (the smallest ids win
+//      among equal gains, list order being id order);
+//   3. LDS bitonic sort, balanced prefix (balance_prefix);
+//   4. apply of the prefix: part, update flags and the gain worklist
+//      (apply_sequence_move, gkway/uncoarsen.hpp:290-323).
+// A job with nothing selected is left untouched, so extra rounds after the
+// last productive one change nothing.
+__global__ void __launch_bounds__(kBalThreads) k_round_b(
+    const int32_t* __restrict__ v_off, const uint64_t* __restrict__ bits, int32_t M,
+    const int32_t* __restrict__ kk, const long long* __restrict__ cap, const int32_t* __restrict__ pw_off,
+    const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj, int32_t* __restrict__ part,
+    const int32_t* __restrict__ dest, const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
+    int32_t* __restrict__ pw, long long* __restrict__ cut, int32_t* __restrict__ rounds,
+    long long* __restrict__ moved, int32_t* __restrict__ list, int32_t* __restrict__ upd, int32_t* __restrict__ wl,
+    int32_t* __restrict__ wlc, int32_t* __restrict__ active) {
+  __shared__ RoundLds R;
+  extern __shared__ int32_t dpw[];
+  const int t = threadIdx.x;
+  const int32_t j = blockIdx.x;
+  const int32_t vb = v_off[j], ve = v_off[j + 1];
+  int32_t* seg = list + vb;
+  // 1. ordered compaction
+  int running = 0;
+  const int64_t w0 = int64_t(vb) >> 6, w1 = (int64_t(ve) + 63) >> 6;
+  for (int64_t base = w0; base < w1; base += kBalThreads) {
+    const int64_t w = base + t;
+    uint64_t m = 0;
+    if (w < w1) {
+      m = bits[w];
+      const int64_t lo = w << 6;
+      if (lo < vb) m &= ~0ull << (vb - lo);
+      if (lo + 64 > ve) m &= (1ull << (ve - lo)) - 1;
+    }
+    int tot;
+    int off = running + block_excl_scan_big(popc64(m), &tot, R.ws);
+    while (m) {
+      const int b = __ffsll(static_cast<long long>(m)) - 1;
+      seg[off++] = int32_t((w << 6) + b);
+      m &= m - 1;
+    }
+    running += tot;
+  }
+  const int32_t count = running;
+  if (count == 0) return;
+  __syncthreads();
+  if (t == 0) atomicAdd(active, count);
+  const int32_t B = count < M ? count : M;
+  // 2. the B best keys into L.sk[0, B)
+  if (count <= M) {
+    R.b.sk[t] = t < count ? move_key(gain, seg[t]) : ~0ull;
+  } else {
+    uint32_t prefix = 0, pmask = 0;
+    int32_t need = B;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+      if (t < 256) R.hist[t] = 0;
+      __syncthreads();
+      for (int32_t i = t; i < count; i += kBalThreads) {
+        const uint32_t h = uint32_t(0x7fffffff - gain[seg[i]]);
+        if ((h & pmask) == prefix) atomicAdd(&R.hist[(h >> shift) & 255], 1);
+      }
+      __syncthreads();
+      if (t == 0) {
+        int32_t cum = 0, d = 0;
+        for (; d < 255; ++d) {
+          if (cum + R.hist[d] >= need) break;
+          cum += R.hist[d];
+        }
+        R.misc[0] = d;
+        R.misc[1] = cum;
+      }
+      __syncthreads();
+      need -= R.misc[1];
+      prefix |= uint32_t(R.misc[0]) << shift;
+      pmask |= 255u << shift;
+      __syncthreads();
+    }
+    // prefix = gain key of the B-th best move; every key below it is taken,
+    // and the first `need` (smallest ids) equal to it.
+    if (t == 0) R.misc[2] = 0;
+    R.b.sk[t] = ~0ull;
+    __syncthreads();
+    int32_t eqrun = 0;
+    for (int32_t base = 0; base < count; base += kBalThreads) {
+      const int32_t i = base + t;
+      bool less = false, eq = false;
+      int32_t v = 0;
+      if (i < count) {
+        v = seg[i];
+        const uint32_t h = uint32_t(0x7fffffff - gain[v]);
+        less = h < prefix;
+        eq = h == prefix;
+      }
+      if (less) R.b.sk[atomicAdd(&R.misc[2], 1)] = move_key(gain, v);
+      int tot;
+      const int r = eqrun + block_excl_scan_big(eq ? 1 : 0, &tot, R.ws);
+      if (eq && r < need) R.b.sk[(B - need) + r] = move_key(gain, v);
+      eqrun += tot;
+    }
+  }
+  __syncthreads();
+  // 3. sort and balanced prefix
+  lds_bitonic(R.b, B);
+  const int pos = balance_prefix(R.b, dpw, kk[j], B, 0xffffffffull, part, dest, gain, vw, pw + pw_off[j], cap[j],
+                                 nullptr, nullptr, &cut[j]);
+  // 4. apply moves 0..pos (balance_prefix read part[] before its last barrier)
+  const bool mv = t <= pos;
+  const int32_t v = mv ? R.b.sv[t] : 0;
+  int32_t e = 0, es = 0, deg = 0;
+  if (mv) {
+    part[v] = dest[v];
+    es = xadj[v];
+    deg = xadj[v + 1] - es;
+  }
+  bool act = mv;
+  e = -1;  // -1: the mover itself, then its neighbours
+  while (ballot(act) != 0) {
+    int32_t u = 0;
+    bool app = false;
+    if (act) {
+      u = e < 0 ? v : adj[es + e];
+      app = atomicExch(&upd[u], 1) == 0;
+    }
+    const uint64_t am = ballot(app);
+    if (am) {
+      const int leader = __ffsll(static_cast<long long>(am)) - 1;
+      int wbase = 0;
+      if (lane_id() == leader) wbase = atomicAdd(&wlc[0], popc64(am));
+      wbase = __shfl(wbase, leader, 64);
+      if (app) wl[wbase + popc64(am & lanemask_lt())] = u;
+    }
+    if (act) {
+      ++e;
+      act = e < deg;
+    }
+  }
+  if (t == 0) {
     rounds[j] += 1;
-    moved[j] += pos[j] + 1;
+    moved[j] += pos + 1;
   }
 }
 
-int launch_balance_b(int32_t nj, int32_t kmax, const uint64_t* keys, const int32_t* seg_beg, const int32_t* seg_end,
-                     int32_t M, int idbits, const int32_t* kk, const long long* cap, const int32_t* pw_off,
-                     const int32_t* part, const int32_t* dest, const int32_t* gain, const int32_t* vw, int32_t* pw,
-                     int32_t* pos, long long* cut, int32_t* rounds, long long* moved, hipStream_t st) {
+int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, const uint64_t* bits, int32_t M,
+                   const int32_t* kk, const long long* cap, const int32_t* pw_off, const DGraph& g, int32_t* part,
+                   const int32_t* dest, const int32_t* gain, int32_t* pw, long long* cut, int32_t* rounds,
+                   long long* moved, int32_t* list, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* active,
+                   hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
-  const uint64_t idmask = (1ull << idbits) - 1;
-  k_balance_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(keys, seg_beg, seg_end, M, idmask, kk, cap, pw_off,
-                                                               part, dest, gain, vw, pw, pos, cut, rounds, moved);
+  k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, bits, M, kk, cap, pw_off, g.xadj, g.adj, part,
+                                                             dest, gain, g.vw, pw, cut, rounds, moved, list, upd, wl,
+                                                             wlc, active);
   return hip_status();
 }
 
-// apply_sequence_move (gkway/uncoarsen.hpp:290-323) over the sorted keys:
-// a key at rank r of its job's segment is applied when r <= pos[job].
-__global__ void __launch_bounds__(256) k_apply_b(int32_t count, const uint64_t* __restrict__ keys,
-                                                 uint64_t idmask, int shift,
-                                                 const int32_t* __restrict__ seg_beg,
-                                                 const int32_t* __restrict__ pos,
-                                                 const int32_t* __restrict__ xadj,
-                                                 const int32_t* __restrict__ adj,
-                                                 const int32_t* __restrict__ dest, int32_t* __restrict__ part,
-                                                 int32_t* __restrict__ upd, int32_t* __restrict__ wl,
-                                                 int32_t* __restrict__ wlc) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= count) return;
-  const uint64_t key = keys[i];
-  const int32_t j = int32_t(key >> shift);
-  if (i - seg_beg[j] > pos[j]) return;
-  const int32_t v = int32_t(key & idmask);
-  part[v] = dest[v];
-  if (atomicExch(&upd[v], 1) == 0) wl[atomicAdd(&wlc[0], 1)] = v;
-  for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) {
-    const int32_t u = adj[e];
-    if (atomicExch(&upd[u], 1) == 0) wl[atomicAdd(&wlc[0], 1)] = u;
+// Final cut and part weights (calculate_cutsize + calculate_partition_wgt,
+// gkway/uncoarsen.hpp:398-424, gkway/cuda_check.hpp:61-73) per chunk of one
+// job's vertices: the block sums its vertex weights per part in an LDS
+// histogram and its cut edges in registers, then adds them to the job's
+// totals with one atomic per touched part and one for the cut.
+__global__ void __launch_bounds__(256) k_cut_chunks(const int32_t* __restrict__ cj, const int32_t* __restrict__ cs,
+                                                    const int32_t* __restrict__ ce, const int32_t* __restrict__ kk,
+                                                    const int32_t* __restrict__ xadj,
+                                                    const int32_t* __restrict__ adj, const int32_t* __restrict__ w,
+                                                    const int32_t* __restrict__ vw,
+                                                    const int32_t* __restrict__ part,
+                                                    const int32_t* __restrict__ pw_off,
+                                                    unsigned long long* __restrict__ cut,
+                                                    unsigned long long* __restrict__ pw64) {
+  extern __shared__ unsigned int hist[];
+  __shared__ long long wsum[4];
+  const int32_t c = blockIdx.x;
+  const int32_t j = cj[c], k = kk[j];
+  for (int32_t q = threadIdx.x; q < k; q += 256) hist[q] = 0;
+  __syncthreads();
+  long long cc = 0;
+  for (int32_t v = cs[c] + threadIdx.x; v < ce[c]; v += 256) {
+    const int32_t p = part[v];
+    atomicAdd(&hist[p], (unsigned int)vw[v]);
+    for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) {
+      const int32_t u = adj[e];
+      if (v < u && part[u] != p) cc += w[e];
+    }
   }
-}
-
-int launch_apply_b(const DGraph& g, int32_t count, const uint64_t* keys, int idbits, int shift,
-                   const int32_t* seg_beg, const int32_t* pos, const int32_t* dest, int32_t* part, int32_t* upd,
-                   int32_t* wl, int32_t* wlc, hipStream_t st) {
-  if (count <= 0) return 0;
-  k_apply_b<<<grid_for(count, 256), 256, 0, st>>>(count, keys, (1ull << idbits) - 1, shift, seg_beg, pos, g.xadj, g.adj,
-                                                  dest, part, upd, wl, wlc);
-  return hip_status();
-}
-
-// Final cut and part weights per job (calculate_cutsize +
-// calculate_partition_wgt), 8 lanes per vertex, only for jobs with done[j].
-__global__ void __launch_bounds__(256) k_cut_b(int32_t n, const int32_t* __restrict__ xadj,
-                                               const int32_t* __restrict__ adj, const int32_t* __restrict__ w,
-                                               const int32_t* __restrict__ vw, const int32_t* __restrict__ vjob,
-                                               const int32_t* __restrict__ part,
-                                               const int32_t* __restrict__ pw_off,
-                                               const int32_t* __restrict__ done,
-                                               unsigned long long* __restrict__ cut,
-                                               unsigned long long* __restrict__ pw64) {
-  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  const int64_t v = gt / 8;
-  const int sub = static_cast<int>(gt % 8);
-  if (v >= n) return;
-  const int32_t j = vjob[v];
-  if (!done[j]) return;
-  const int32_t p = part[v];
-  long long c = 0;
-  for (int32_t e = xadj[v] + sub; e < xadj[v + 1]; e += 8) {
-    const int32_t u = adj[e];
-    if (v < u && part[u] != p) c += w[e];
+  cc = wave_sum64(cc);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = cc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long long s = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (s) atomicAdd(&cut[j], (unsigned long long)s);
   }
-  if (c) atomicAdd(&cut[j], (unsigned long long)c);
-  if (sub == 0) atomicAdd(&pw64[pw_off[j] + p], (unsigned long long)vw[v]);
+  for (int32_t q = threadIdx.x; q < k; q += 256)
+    if (hist[q]) atomicAdd(&pw64[pw_off[j] + q], (unsigned long long)hist[q]);
 }
 
-int launch_cut_b(const DGraph& g, const int32_t* vjob, const int32_t* part, const int32_t* pw_off,
-                 const int32_t* done, unsigned long long* cut, unsigned long long* pw64, hipStream_t st) {
-  if (g.n <= 0) return 0;
-  k_cut_b<<<grid_for(int64_t(g.n) * 8, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, g.vw, vjob, part, pw_off, done,
-                                                           cut, pw64);
+int launch_cut_chunks(int32_t nchunks, const int32_t* cj, const int32_t* cs, const int32_t* ce, int32_t kmax,
+                      const int32_t* kk, const DGraph& g, const int32_t* part, const int32_t* pw_off,
+                      unsigned long long* cut, unsigned long long* pw64, hipStream_t st) {
+  if (nchunks <= 0) return 0;
+  k_cut_chunks<<<nchunks, 256, sizeof(unsigned int) * kmax, st>>>(cj, cs, ce, kk, g.xadj, g.adj, g.w, g.vw, part,
+                                                                  pw_off, cut, pw64);
   return hip_status();
 }
 

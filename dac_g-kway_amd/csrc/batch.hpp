@@ -26,11 +26,6 @@ struct PackSrc {
 
 // One job's destination arrays for an unpack (null members are skipped).
 struct UnpackDst {
-  int32_t* xadj;
-  int32_t* adj;
-  int32_t* w;
-  int32_t* vw;
-  int32_t* cmap;
   int32_t* part;
   int32_t* bnd;
 };
@@ -49,27 +44,25 @@ int launch_job_offsets(int32_t nj, const int32_t* v_off, const int32_t* order, c
                        int32_t* c_off, hipStream_t st);
 // ce_off[j] = cxadj[c_off[j]].
 int launch_job_eoffsets(int32_t nj, const int32_t* c_off, const int32_t* cxadj, int32_t* ce_off, hipStream_t st);
-int launch_unpack_coarse(int32_t nj, const UnpackDst* dst, const int32_t* v_off, const int32_t* c_off,
-                         const int32_t* ce_off, int32_t n, int32_t nc, int32_t ce, const int32_t* cmap,
-                         const int32_t* cxadj, const int32_t* cadj, const int32_t* cw, const int32_t* cvw,
-                         hipStream_t st);
 int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, int32_t n, const int32_t* part,
                        const int32_t* bnd, hipStream_t st);
-int launch_select_b(const DGraph& g, const int32_t* vjob, const int32_t* bnd, const int32_t* gain,
-                    const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                    int32_t* list, int32_t* ctl, int32_t* reset, hipStream_t st);
-int launch_keys_b(int32_t count, const int32_t* list, const int32_t* gain, const int32_t* vjob, int32_t gmax,
-                  int gbits, int idbits, uint64_t* keys, hipStream_t st);
-int launch_segments(int32_t nj, int32_t count, const uint64_t* keys, int shift, int32_t* seg_beg, int32_t* seg_end,
-                    hipStream_t st);
-int launch_balance_b(int32_t nj, int32_t kmax, const uint64_t* keys, const int32_t* seg_beg, const int32_t* seg_end,
-                     int32_t M, int idbits, const int32_t* kk, const long long* cap, const int32_t* pw_off,
-                     const int32_t* part, const int32_t* dest, const int32_t* gain, const int32_t* vw, int32_t* pw,
-                     int32_t* pos, long long* cut, int32_t* rounds, long long* moved, hipStream_t st);
-int launch_apply_b(const DGraph& g, int32_t count, const uint64_t* keys, int idbits, int shift,
-                   const int32_t* seg_beg, const int32_t* pos, const int32_t* dest, int32_t* part, int32_t* upd,
-                   int32_t* wl, int32_t* wlc, hipStream_t st);
-int launch_cut_b(const DGraph& g, const int32_t* vjob, const int32_t* part, const int32_t* pw_off,
-                 const int32_t* done, unsigned long long* cut, unsigned long long* pw64, hipStream_t st);
+// Refinement round of the batched form without host round trips: selection as
+// a bitmask, then one workgroup per job (ordered compaction, best min(count, M)
+// moves, LDS sort, balanced prefix, apply).  *active receives the number of
+// selected moves of the round (0 once every job has converged).
+int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* bnd, const int32_t* gain,
+                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
+                       uint64_t* bits, int32_t* reset, int32_t* active, hipStream_t st);
+int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, const uint64_t* bits, int32_t M,
+                   const int32_t* kk, const long long* cap, const int32_t* pw_off, const DGraph& g, int32_t* part,
+                   const int32_t* dest, const int32_t* gain, int32_t* pw, long long* cut, int32_t* rounds,
+                   long long* moved, int32_t* list, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* active,
+                   hipStream_t st);
+// Final cut and part weights of the jobs covered by the chunk table: chunk c
+// is vertices [cs[c], ce[c]) of job cj[c] (pack ids); cut[job] and
+// pw64[pw_off[job] + q] accumulate (zeroed by the caller).
+int launch_cut_chunks(int32_t nchunks, const int32_t* cj, const int32_t* cs, const int32_t* ce, int32_t kmax,
+                      const int32_t* kk, const DGraph& g, const int32_t* part, const int32_t* pw_off,
+                      unsigned long long* cut, unsigned long long* pw64, hipStream_t st);
 
 }  // namespace gkd

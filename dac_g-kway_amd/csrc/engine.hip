@@ -535,10 +535,9 @@ int Engine::refine_step(bool* worked) {
   const Store& F = *pk;
   const DGraph g = F.graph();
   const int32_t N = g.n;
-  Buf<int32_t> dpw_off, dkk, dpw, gain, dest, upd, wl, heavy, list, hcnt, ctl, seg_beg, seg_end, pos, drounds,
-      sscr;
+  Buf<int32_t> dpw_off, dkk, dpw, gain, dest, upd, wl, heavy, list, hcnt, active, drounds;
   Buf<long long> dcap, dcut, dmoved;
-  Buf<uint64_t> keys, keys2;
+  Buf<uint64_t> bits;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
   GK_TRY(upload(dpw_off, pw_off.data(), pw_off.size()));
@@ -554,14 +553,9 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(wl.alloc(P_, N));
   GK_TRY(heavy.alloc(P_, N));
   GK_TRY(list.alloc(P_, N));
-  GK_TRY(keys.alloc(P_, N));
-  GK_TRY(keys2.alloc(P_, N));
-  GK_TRY(sscr.alloc(P_, gkd::sort_scratch_ints(N)));
+  GK_TRY(bits.alloc(P_, (int64_t(N) + 63) / 64 + 1));
   GK_TRY(hcnt.alloc(P_, 2));
-  GK_TRY(ctl.alloc(P_, 2));
-  GK_TRY(seg_beg.alloc(P_, J));
-  GK_TRY(seg_end.alloc(P_, J));
-  GK_TRY(pos.alloc(P_, J));
+  GK_TRY(active.alloc(P_, 1));
   GK_TRY(drounds.alloc(P_, J));
   GK_TRY(dmoved.alloc(P_, J));
   gkd::fill<int32_t>(drounds.p, 0, J, st_);
@@ -577,48 +571,30 @@ int Engine::refine_step(bool* worked) {
   GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
            gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr, st_));
   const int64_t avg_deg = N > 0 ? (int64_t(g.nnz) + N - 1) / N : 0;
-  const int idbits = bits_for(N > 0 ? N - 1 : 0);
-  const int jbits = bits_for(J > 0 ? J - 1 : 0);
+  const int64_t wl_items = std::min<int64_t>(N, int64_t(J) * M * (1 + 2 * avg_deg));
+  // Rounds run back to back on the device; the host reads the last round's
+  // move count every kRoundsPerCheck rounds.  A round in which no job selects
+  // a move changes nothing, so rounds past convergence are harmless.
+  constexpr int kRoundsPerCheck = 4;
   while (true) {
-    GK_STAGE(ctx_, GK_ST_SELECT, 12LL * N,
-             gkd::launch_select_b(g, rs.vjob.p, bnd->p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, list.p, ctl.p,
-                                  hcnt.p, st_));
-    int32_t hc[2];
-    GK_TRY(fetch(ctx_, st_, {{hc, ctl.p, sizeof hc}}));
-    const int32_t count = hc[0];
-    if (count == 0) break;
-    const int gbits = bits_for(hc[1]);
-    if (jbits + gbits + idbits > 64) {
-      gkh::set_error("move keys need more than 64 bits (graphs x gain x vertices)");
-      return GK_ERR_LIMIT;
+    for (int r = 0; r < kRoundsPerCheck; ++r) {
+      GK_STAGE(ctx_, GK_ST_SELECT, 12LL * N + N / 8,
+               gkd::launch_select_bits(g, rs.vjob.p, bnd->p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, bits.p, hcnt.p,
+                                       active.p, st_));
+      GK_STAGE(ctx_, GK_ST_BALANCE, N / 8 + 8LL * J,
+               gkd::launch_round_b(J, kmax, F.dv_off.p, bits.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p,
+                                   gain.p, dpw.p, dcut.p, drounds.p, dmoved.p, list.p, upd.p, wl.p, hcnt.p, active.p,
+                                   st_));
+      GK_STAGE(ctx_, GK_ST_GAIN, 0,
+               gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
+                                   nullptr, wl_items, st_));
+      GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
+               gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
+                                      st_));
     }
-    const int shift = gbits + idbits;
-    bool alt = false;
-    const int passes = (jbits + shift + 7) / 8;
-    GK_STAGE(ctx_, GK_ST_MSORT, 16LL * count + 16LL * count * passes,
-             (gkd::launch_keys_b(count, list.p, gain.p, rs.vjob.p, hc[1], gbits, idbits, keys.p, st_) ||
-                      gkd::radix_sort_pairs<uint64_t>(keys.p, nullptr, keys2.p, nullptr, count, 0, jbits + shift,
-                                                      sscr.p, st_, &alt)
-                  ? GK_ERR_HIP
-                  : GK_OK));
-    const uint64_t* sk = alt ? keys2.p : keys.p;
-    const int64_t B = std::min<int64_t>(count, int64_t(J) * M);
-    GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * count + 24LL * B,
-             gkd::launch_segments(J, count, sk, shift, seg_beg.p, seg_end.p, st_) ||
-                     gkd::launch_balance_b(J, kmax, sk, seg_beg.p, seg_end.p, M, idbits, dkk.p, dcap.p, dpw_off.p,
-                                           part->p, dest.p, gain.p, g.vw, dpw.p, pos.p, dcut.p, drounds.p, dmoved.p,
-                                           st_)
-                 ? GK_ERR_HIP
-                 : GK_OK);
-    GK_STAGE(ctx_, GK_ST_APPLY, (24LL + 8LL * avg_deg) * B,
-             gkd::launch_apply_b(g, count, sk, idbits, shift, seg_beg.p, pos.p, dest.p, part->p, upd.p, wl.p,
-                                 hcnt.p, st_));
-    GK_STAGE(ctx_, GK_ST_GAIN, 0,
-             gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                                 nullptr, B * (1 + 2 * avg_deg), st_));
-    GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
-             gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
-                                    st_));
+    int32_t live = 0;
+    GK_TRY(fetch(ctx_, st_, {{&live, active.p, sizeof live}}));
+    if (live == 0) break;
   }
   std::vector<long long> hcut(J), hmoved(J);
   std::vector<int32_t> hrounds(J);
@@ -651,18 +627,36 @@ int Engine::refine_step(bool* worked) {
     }
   }
   if (any_done) {
+    // Chunks of at most kCutChunk vertices of each finished job.
+    constexpr int32_t kCutChunk = 16384;
+    std::vector<int32_t> cj, cs, ce;
+    for (int32_t i = 0; i < J; ++i) {
+      if (!done[i]) continue;
+      for (int32_t a = F.v_off[i]; a < F.v_off[i + 1]; a += kCutChunk) {
+        cj.push_back(i);
+        cs.push_back(a);
+        ce.push_back(std::min(F.v_off[i + 1], a + kCutChunk));
+      }
+    }
     Buf<UnpackDst> ddst;
-    Buf<int32_t> ddone;
+    Buf<int32_t> dcj, dcs, dce;
     Buf<unsigned long long> c64, pw64;
     GK_TRY(upload(ddst, dst.data(), dst.size()));
-    GK_TRY(upload(ddone, done.data(), done.size()));
+    GK_TRY(upload(dcj, cj.data(), cj.size()));
+    GK_TRY(upload(dcs, cs.data(), cs.size()));
+    GK_TRY(upload(dce, ce.data(), ce.size()));
     GK_TRY(c64.alloc(P_, J));
     GK_TRY(pw64.alloc(P_, pw_off[J]));
     gkd::fill<unsigned long long>(c64.p, 0ull, J, st_);
     gkd::fill<unsigned long long>(pw64.p, 0ull, pw_off[J], st_);
     GK_TRY(gkd::launch_unpack_part(J, ddst.p, F.dv_off.p, N, part->p, nullptr, st_));
-    GK_STAGE(ctx_, GK_ST_CUT, 16LL * N + 12LL * g.nnz,
-             gkd::launch_cut_b(g, rs.vjob.p, part->p, dpw_off.p, ddone.p, c64.p, pw64.p, st_));
+    int64_t cut_bytes = 0;
+    for (size_t c = 0; c < cj.size(); ++c)
+      cut_bytes += 16LL * (ce[c] - cs[c]) + 12LL * (F.e_off[cj[c] + 1] - F.e_off[cj[c]]) * (ce[c] - cs[c]) /
+                                               std::max<int64_t>(1, F.v_off[cj[c] + 1] - F.v_off[cj[c]]);
+    GK_STAGE(ctx_, GK_ST_CUT, cut_bytes,
+             gkd::launch_cut_chunks(static_cast<int32_t>(cj.size()), dcj.p, dcs.p, dce.p, kmax, dkk.p, g, part->p,
+                                    dpw_off.p, c64.p, pw64.p, st_));
     std::vector<unsigned long long> hc64(J), hpw64(pw_off[J]);
     GK_TRY(download({{hc64.data(), c64.p, sizeof(unsigned long long) * J},
                      {hpw64.data(), pw64.p, sizeof(unsigned long long) * hpw64.size()}}));

@@ -16,9 +16,13 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
-#include <future>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -420,11 +424,91 @@ G induced(const G& g, const std::vector<uint8_t>& side, uint8_t s, std::vector<i
   return h;
 }
 
+// ------------------------------------------------------------------ fork-join
+// Process-wide worker threads for the independent pieces of the recursive
+// bisection (the trial bisections of a node and the two halves below it).
+// A thread waiting for its own tasks runs queued tasks meanwhile, so nested
+// fork-join never blocks a worker.  Results never depend on which thread runs
+// a task.
+class ForkJoin {
+ public:
+  struct Task {
+    std::function<void()> fn;
+    std::atomic<bool> done{false};
+  };
+  static ForkJoin& get() {
+    static ForkJoin pool;
+    return pool;
+  }
+  int workers() const { return static_cast<int>(th_.size()); }
+  std::shared_ptr<Task> spawn(std::function<void()> fn) {
+    auto t = std::make_shared<Task>();
+    t->fn = std::move(fn);
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      q_.push_back(t);
+    }
+    cv_.notify_one();
+    return t;
+  }
+  void wait(const std::shared_ptr<Task>& t) {
+    while (!t->done.load(std::memory_order_acquire)) {
+      std::shared_ptr<Task> o = pop();
+      if (o) run(*o);
+      else std::this_thread::yield();
+    }
+  }
+
+ private:
+  ForkJoin() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int n = static_cast<int>(hw ? std::min(hw, 16u) : 1u) - 1;
+    for (int i = 0; i < n; ++i)
+      th_.emplace_back([this] {
+        while (true) {
+          std::shared_ptr<Task> t;
+          {
+            std::unique_lock<std::mutex> lk(m_);
+            cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+            if (stop_ && q_.empty()) return;
+            t = q_.front();
+            q_.pop_front();
+          }
+          run(*t);
+        }
+      });
+  }
+  ~ForkJoin() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  std::shared_ptr<Task> pop() {
+    std::lock_guard<std::mutex> lk(m_);
+    if (q_.empty()) return nullptr;
+    auto t = q_.back();  // newest first: the waiter's own, most likely
+    q_.pop_back();
+    return t;
+  }
+  static void run(Task& t) {
+    t.fn();
+    t.done.store(true, std::memory_order_release);
+  }
+  std::vector<std::thread> th_;
+  std::deque<std::shared_ptr<Task>> q_;
+  std::mutex m_;
+  std::condition_variable cv_;
+  bool stop_ = false;
+};
+
 struct RBCtx {
   double tol;
   int ncuts0, ncuts1, ncuts2;
   uint64_t seed;
-  std::atomic<int> threads_left;
+  bool parallel;  // use the fork-join pool
 };
 
 void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* out,
@@ -457,18 +541,13 @@ void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* ou
     key_ol[c] = std::max<int64_t>(0, pw0 - m0) + std::max<int64_t>(0, (g.tw - pw0) - m1);
     key_cut[c] = cut2(g, trial[c]);
   };
-  {
-    std::vector<std::future<void>> futs;
-    int c = 1;
-    for (; c < ncuts && g.n > 2000; ++c) {
-      int left = ctx.threads_left.load();
-      while (left > 0 && !ctx.threads_left.compare_exchange_weak(left, left - 1)) {}
-      if (left <= 0) break;
-      futs.push_back(std::async(std::launch::async, [&, c] { run_trial(c); ctx.threads_left.fetch_add(1); }));
-    }
+  if (ctx.parallel && g.n > 2000) {
+    std::vector<std::shared_ptr<ForkJoin::Task>> ts;
+    for (int c = 1; c < ncuts; ++c) ts.push_back(ForkJoin::get().spawn([&, c] { run_trial(c); }));
     run_trial(0);
-    for (; c < ncuts; ++c) run_trial(c);
-    for (auto& f : futs) f.get();
+    for (auto& t : ts) ForkJoin::get().wait(t);
+  } else {
+    for (int c = 0; c < ncuts; ++c) run_trial(c);
   }
   int bc = 0;
   for (int c = 1; c < ncuts; ++c)
@@ -480,19 +559,11 @@ void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* ou
   std::vector<int32_t> ids0(mb0.size()), ids1(mb1.size());
   for (size_t i = 0; i < mb0.size(); ++i) ids0[i] = ids[mb0[i]];
   for (size_t i = 0; i < mb1.size(); ++i) ids1[i] = ids[mb1[i]];
-  // Run the two halves concurrently while threads remain; the result does
-  // not depend on which branch runs where.
-  bool spawn = false;
-  if (g.n > 2000) {
-    int left = ctx.threads_left.load();
-    while (left > 0 && !ctx.threads_left.compare_exchange_weak(left, left - 1)) {}
-    spawn = left > 0;
-  }
-  if (spawn) {
-    auto fut = std::async(std::launch::async, [&] { rb(g0, k0, part0, depth + 1, ctx, out, ids0); });
+  // The two halves are independent: one may run on another thread.
+  if (ctx.parallel && g.n > 2000) {
+    auto t = ForkJoin::get().spawn([&] { rb(g0, k0, part0, depth + 1, ctx, out, ids0); });
     rb(g1, k1, part0 + k0, depth + 1, ctx, out, ids1);
-    fut.get();
-    ctx.threads_left.fetch_add(1);
+    ForkJoin::get().wait(t);
   } else {
     rb(g0, k0, part0, depth + 1, ctx, out, ids0);
     rb(g1, k1, part0 + k0, depth + 1, ctx, out, ids1);
@@ -582,9 +653,9 @@ extern "C" int gk_initpart(const gk_csr* hg, int32_t k, int64_t cap, int32_t see
   ctx.ncuts1 = GKIP_NCUTS1;
   ctx.ncuts2 = GKIP_NCUTS2;
   ctx.seed = static_cast<uint64_t>(seed) + 1;
-  unsigned hw = std::thread::hardware_concurrency();
-  int nt = nthreads > 0 ? nthreads : static_cast<int>(hw ? std::min(hw, 16u) : 1u);
-  ctx.threads_left.store(std::max(0, nt - 1));
+  // nthreads == 1: everything on the calling thread; otherwise the
+  // process-wide pool (up to 16 threads) helps.
+  ctx.parallel = nthreads != 1 && ForkJoin::get().workers() > 0;
   std::vector<int32_t> ids(g.n);
   std::iota(ids.begin(), ids.end(), 0);
   rb(g, k, 0, 0, ctx, part, ids);
