@@ -200,6 +200,8 @@ def run_batch(args, gk, torch):
                          f"by one by the single-threaded oracle ({tc:.1f} s)"}
         log(f"oracle: {ng} graphs in {tc:.1f}s, bit_exact={exact}")
 
+    if args.calibrate_copy:
+        calibrate_copy()
     value = m_tot / (total / args.steps)
     ok = deterministic and exact and within_cap
     out = {
@@ -235,19 +237,152 @@ def run_batch(args, gk, torch):
     ctx.close()
 
 
+def calibrate_copy():
+    """Elementwise kernel of known traffic: reads 1 GiB and writes 1 GiB per
+    dispatch; FETCH_SIZE / WRITE_SIZE of it give the counters' scale."""
+    import torch as _t
+    a = _t.ones(1 << 28, dtype=_t.int32, device="cuda")
+    b = _t.empty_like(a)
+    _t.cuda.synchronize()
+    for _ in range(3):
+        _t.mul(a, 3, out=b)
+    _t.cuda.synchronize()
+    log("calibration: 3 x elementwise multiply, 1 GiB in + 1 GiB out each")
+
+
+def serve_pool(gk, count, seed, lo=1000, hi=2_000_000):
+    """Distinct request graphs: vertex counts log-uniform in [lo, hi]; 40%
+    Delaunay meshes, 40% layered DAGs (edge weights 1..100, vertex weights
+    1..4), 20% road-like (average degree 2.1)."""
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+    kinds = rng.permutation(np.array(["delaunay"] * (count * 2 // 5) + ["dag"] * (count * 2 // 5) +
+                                     ["road"] * (count - 2 * (count * 2 // 5))))
+    out = []
+    for i, (n, kind) in enumerate(zip(ns, kinds)):
+        s = seed * 1_000_003 + i
+        if kind == "delaunay":
+            out.append(gk.delaunay(int(n), s))
+        elif kind == "dag":
+            out.append(gk.dag(int(n), int(min(200, max(2, round(float(np.sqrt(n)))))), s))
+        else:
+            out.append(gk.road(int(n), s))
+    return out
+
+
+def run_serve(args, gk, torch):
+    """Serving form: open-loop Poisson arrivals into gk_serve_*; each request
+    is a graph drawn from a pool of distinct graphs with its own k."""
+    import threading
+    t = time.time()
+    pool = serve_pool(gk, args.pool, args.seed)
+    log(f"generated a pool of {len(pool)} graphs (n {min(g.n for g in pool)}..{max(g.n for g in pool)}) "
+        f"in {time.time() - t:.1f}s")
+    rng = np.random.default_rng(args.seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, int(args.rate * args.duration * 1.2) + 16))
+    arrivals = arrivals[arrivals < args.duration]
+    picks = rng.integers(0, len(pool), len(arrivals))
+    ks = rng.choice(np.array([8, 16, 64, 256]), len(arrivals))
+    srv = gk.Server(0, hbm_budget=0)
+    ids = [None] * len(arrivals)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, at in enumerate(arrivals):
+            d = t0 + at - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids[i] = srv.submit(pool[picks[i]], int(ks[i]))
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    results = {}
+    check_every = max(1, len(arrivals) // args.check)
+    nxt = 0
+    while nxt < len(arrivals):
+        if ids[nxt] is None:
+            time.sleep(0.001)
+            continue
+        part, st = srv.wait(ids[nxt])
+        if nxt % check_every == 0:
+            results[nxt] = part
+        nxt += 1
+        if nxt % 500 == 0:
+            log(f"{nxt} of {len(arrivals)} requests done at {time.perf_counter() - t0:.1f}s")
+    th.join()
+    wall = time.perf_counter() - t0
+    s = srv.stats()
+    srv.close()
+    m_done = sum(pool[p].m for p in picks)
+    # Alone vs in the mix: the sampled requests, each partitioned alone.
+    exact = True
+    with gk.Context(0) as ctx:
+        for i, part in results.items():
+            p1, _ = ctx.partition(pool[picks[i]], int(ks[i]))
+            exact &= bool(np.array_equal(p1, part))
+    log(f"serve: {len(arrivals)} requests in {wall:.1f}s; p50 {s.p50_ms:.1f} ms p99 {s.p99_ms:.1f} ms; "
+        f"{s.edges_per_s / 1e6:.1f} M edges/s; {len(results)} checked alone-vs-mix: {exact}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle
+        tc, mc, ng = 0.0, 0, 0
+        for i in sorted(results):
+            g = pool[picks[i]]
+            a = time.perf_counter()
+            _oracle.partition(g, int(ks[i]), initpart_threads=1)
+            tc += time.perf_counter() - a
+            mc += g.m
+            ng += 1
+            if tc > args.cpu_budget:
+                break
+        cpu = {"value": round(mc / tc, 1), "unit": "edges/s", "cores": 1, "kind": "port",
+               "sample": f"{ng} of the checked requests partitioned one by one by the single-threaded oracle "
+                         f"({tc:.1f} s)"}
+    ok = exact and s.failed == 0 and s.completed == len(arrivals)
+    out = {
+        "metric": "sustained partitioned edges/s at bit-exact cut, eps=3% (serving form, continuous batching)",
+        "value": round(s.edges_per_s, 1) if ok else 0.0,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "requests": len(arrivals),
+        "p50_ms": round(s.p50_ms, 3),
+        "p99_ms": round(s.p99_ms, 3),
+        "max_ms": round(s.max_ms, 3),
+        "offered_edges_per_s": round(m_done / args.duration, 1),
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": f"synthetic: a pool of {len(pool)} distinct seeded graphs, requests draw from it",
+        "config": {"workload": f"Poisson arrivals at {args.rate}/s for {args.duration}s; n log-uniform in [1e3, 2e6]; "
+                               f"40% Delaunay, 40% DAG (w 1..100), 20% road (deg 2.1); k in {{8,16,64,256}}",
+                   "eps": 0.03, "window": "host CSR submitted -> host partition returned"},
+        "engine_steps": int(s.steps),
+        "footprint_peak_bytes": int(s.footprint_peak),
+        "device_peak_bytes": int(s.device_peak),
+        "bit_exact_alone_vs_mix": exact,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", default="plain", choices=["plain", "batch"])
+    ap.add_argument("--form", default="plain", choices=["plain", "batch", "serve"])
     ap.add_argument("--workload", default="delaunay")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--graphs", type=int, default=4096, help="batch form: graphs in the pack")
     ap.add_argument("--plain-sample", type=int, default=64, help="batch form: graphs timed on the plain path")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="batch form: seconds of oracle work")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="batch/serve forms: seconds of oracle work")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve form: Poisson arrival rate (requests/s)")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve form: arrival window (s)")
+    ap.add_argument("--pool", type=int, default=48, help="serve form: distinct graphs requests draw from")
+    ap.add_argument("--check", type=int, default=24, help="serve form: requests re-run alone for bit-exactness")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline + bit-exact check)")
     ap.add_argument("--no-stage-time", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -266,6 +401,11 @@ def main():
         if args.seed is None:
             args.seed = 4
         run_batch(args, gk, torch)
+        return
+    if args.form == "serve":
+        if args.seed is None:
+            args.seed = 5
+        run_serve(args, gk, torch)
         return
     if args.seed is None:
         args.seed = 1
@@ -345,16 +485,7 @@ def main():
         log(f"oracle: {tcpu:.1f}s cut={ost.cut} bit_exact={bit_exact}")
 
     if args.calibrate_copy:
-        # Elementwise kernel of known traffic: reads 1 GiB, writes 1 GiB per
-        # dispatch; FETCH_SIZE / WRITE_SIZE of it give the counter scale.
-        import torch as _t
-        a = _t.ones(1 << 28, dtype=_t.int32, device="cuda")
-        b = _t.empty_like(a)
-        _t.cuda.synchronize()
-        for _ in range(3):
-            _t.mul(a, 3, out=b)
-        _t.cuda.synchronize()
-        log("calibration: 3 x elementwise multiply, 1 GiB in + 1 GiB out each")
+        calibrate_copy()
 
     ok = deterministic and (bit_exact is not False) and st.max_part_wgt <= st.cap
     out = {

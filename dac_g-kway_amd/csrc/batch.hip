@@ -265,8 +265,8 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj, int32_t* __restrict__ part,
     const int32_t* __restrict__ dest, const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
     int32_t* __restrict__ pw, long long* __restrict__ cut, int32_t* __restrict__ rounds,
-    long long* __restrict__ moved, int32_t* __restrict__ list, int32_t* __restrict__ upd, int32_t* __restrict__ wl,
-    int32_t* __restrict__ wlc, int32_t* __restrict__ active) {
+    long long* __restrict__ moved, int32_t* __restrict__ list, uint32_t* __restrict__ hkey, int32_t* __restrict__ upd,
+    int32_t* __restrict__ wl, int32_t* __restrict__ wlc, int32_t* __restrict__ active) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
@@ -303,13 +303,17 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   if (count <= M) {
     R.b.sk[t] = t < count ? move_key(gain, seg[t]) : ~0ull;
   } else {
+    // Gain keys gathered once into the job's slice of hkey, then streamed.
+    uint32_t* hk = hkey + vb;
+    for (int32_t i = t; i < count; i += kBalThreads) hk[i] = uint32_t(0x7fffffff - gain[seg[i]]);
+    __syncthreads();
     uint32_t prefix = 0, pmask = 0;
     int32_t need = B;
     for (int shift = 24; shift >= 0; shift -= 8) {
       if (t < 256) R.hist[t] = 0;
       __syncthreads();
       for (int32_t i = t; i < count; i += kBalThreads) {
-        const uint32_t h = uint32_t(0x7fffffff - gain[seg[i]]);
+        const uint32_t h = hk[i];
         if ((h & pmask) == prefix) atomicAdd(&R.hist[(h >> shift) & 255], 1);
       }
       __syncthreads();
@@ -338,16 +342,17 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
       const int32_t i = base + t;
       bool less = false, eq = false;
       int32_t v = 0;
+      uint32_t h = 0;
       if (i < count) {
         v = seg[i];
-        const uint32_t h = uint32_t(0x7fffffff - gain[v]);
+        h = hk[i];
         less = h < prefix;
         eq = h == prefix;
       }
-      if (less) R.b.sk[atomicAdd(&R.misc[2], 1)] = move_key(gain, v);
+      if (less) R.b.sk[atomicAdd(&R.misc[2], 1)] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
       int tot;
       const int r = eqrun + block_excl_scan_big(eq ? 1 : 0, &tot, R.ws);
-      if (eq && r < need) R.b.sk[(B - need) + r] = move_key(gain, v);
+      if (eq && r < need) R.b.sk[(B - need) + r] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
       eqrun += tot;
     }
   }
@@ -396,13 +401,13 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, const uint64_t* bits, int32_t M,
                    const int32_t* kk, const long long* cap, const int32_t* pw_off, const DGraph& g, int32_t* part,
                    const int32_t* dest, const int32_t* gain, int32_t* pw, long long* cut, int32_t* rounds,
-                   long long* moved, int32_t* list, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* active,
-                   hipStream_t st) {
+                   long long* moved, int32_t* list, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc,
+                   int32_t* active, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
   k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, bits, M, kk, cap, pw_off, g.xadj, g.adj, part,
-                                                             dest, gain, g.vw, pw, cut, rounds, moved, list, upd, wl,
-                                                             wlc, active);
+                                                             dest, gain, g.vw, pw, cut, rounds, moved, list, hkey, upd,
+                                                             wl, wlc, active);
   return hip_status();
 }
 

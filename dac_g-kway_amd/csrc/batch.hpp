@@ -56,8 +56,8 @@ int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* bnd,
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, const uint64_t* bits, int32_t M,
                    const int32_t* kk, const long long* cap, const int32_t* pw_off, const DGraph& g, int32_t* part,
                    const int32_t* dest, const int32_t* gain, int32_t* pw, long long* cut, int32_t* rounds,
-                   long long* moved, int32_t* list, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* active,
-                   hipStream_t st);
+                   long long* moved, int32_t* list, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc,
+                   int32_t* active, hipStream_t st);
 // Final cut and part weights of the jobs covered by the chunk table: chunk c
 // is vertices [cs[c], ce[c]) of job cj[c] (pack ids); cut[job] and
 // pw64[pw_off[job] + q] accumulate (zeroed by the caller).

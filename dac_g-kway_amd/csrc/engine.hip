@@ -538,6 +538,7 @@ int Engine::refine_step(bool* worked) {
   Buf<int32_t> dpw_off, dkk, dpw, gain, dest, upd, wl, heavy, list, hcnt, active, drounds;
   Buf<long long> dcap, dcut, dmoved;
   Buf<uint64_t> bits;
+  Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
   GK_TRY(upload(dpw_off, pw_off.data(), pw_off.size()));
@@ -554,6 +555,7 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(heavy.alloc(P_, N));
   GK_TRY(list.alloc(P_, N));
   GK_TRY(bits.alloc(P_, (int64_t(N) + 63) / 64 + 1));
+  GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
   GK_TRY(drounds.alloc(P_, J));
@@ -583,8 +585,8 @@ int Engine::refine_step(bool* worked) {
                                        active.p, st_));
       GK_STAGE(ctx_, GK_ST_BALANCE, N / 8 + 8LL * J,
                gkd::launch_round_b(J, kmax, F.dv_off.p, bits.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p,
-                                   gain.p, dpw.p, dcut.p, drounds.p, dmoved.p, list.p, upd.p, wl.p, hcnt.p, active.p,
-                                   st_));
+                                   gain.p, dpw.p, dcut.p, drounds.p, dmoved.p, list.p, hkey.p, upd.p, wl.p, hcnt.p,
+                                   active.p, st_));
       GK_STAGE(ctx_, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
                                    nullptr, wl_items, st_));

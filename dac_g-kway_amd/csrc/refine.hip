@@ -364,13 +364,36 @@ __global__ void __launch_bounds__(256) k_apply(int32_t M, const int32_t* __restr
                                                int32_t* __restrict__ part, int32_t* __restrict__ upd,
                                                int32_t* __restrict__ wl, int32_t* __restrict__ wlc) {
   const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= M || j > ctl[2]) return;
-  const int32_t v = moves[j];
-  part[v] = dest[v];
-  if (atomicExch(&upd[v], 1) == 0) wl[atomicAdd(&wlc[0], 1)] = v;
-  for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) {
-    const int32_t u = adj[e];
-    if (atomicExch(&upd[u], 1) == 0) wl[atomicAdd(&wlc[0], 1)] = u;
+  const bool mv = j < M && j <= ctl[2];
+  int32_t v = 0, es = 0, deg = 0;
+  if (mv) {
+    v = moves[j];
+    part[v] = dest[v];
+    es = xadj[v];
+    deg = xadj[v + 1] - es;
+  }
+  // The mover (e = -1), then its neighbours; appends are aggregated per wave.
+  int32_t e = -1;
+  bool act = mv;
+  while (ballot(act) != 0) {
+    int32_t u = 0;
+    bool app = false;
+    if (act) {
+      u = e < 0 ? v : adj[es + e];
+      app = atomicExch(&upd[u], 1) == 0;
+    }
+    const uint64_t am = ballot(app);
+    if (am) {
+      const int leader = __ffsll(static_cast<long long>(am)) - 1;
+      int base = 0;
+      if (lane_id() == leader) base = atomicAdd(&wlc[0], popc64(am));
+      base = __shfl(base, leader, 64);
+      if (app) wl[base + popc64(am & lanemask_lt())] = u;
+    }
+    if (act) {
+      ++e;
+      act = e < deg;
+    }
   }
 }
 
