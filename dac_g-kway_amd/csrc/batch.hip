@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(256) k_pack_vert(int32_t nj, const PackSrc* __
   xadj[i] = e_off[j] + (s.xadj[li] - s.xbase);
   vw[i] = s.vw[li];
   if (vjob) vjob[i] = j;
-  if (cmap) cmap[i] = c_off[j] + (s.cmap[li] - s.cbase);
+  // No source map: the coarse side has the fine level's ids (identity).
+  if (cmap) cmap[i] = c_off[j] + (s.cmap ? s.cmap[li] - s.cbase : li);
 }
 
 __global__ void __launch_bounds__(256) k_pack_edge(int32_t nj, const PackSrc* __restrict__ src,
@@ -68,24 +69,80 @@ __global__ void __launch_bounds__(256) k_pack_edge(int32_t nj, const PackSrc* __
 // the next coarser level, one range per job).
 __global__ void __launch_bounds__(256) k_pack_coarse(int32_t nj, const PackSrc* __restrict__ src,
                                                      const int32_t* __restrict__ c_off, int32_t ctot,
-                                                     int32_t* __restrict__ cpart, int32_t* __restrict__ cbnd) {
+                                                     int32_t* __restrict__ cpart, int32_t* __restrict__ cbnd,
+                                                     int32_t* __restrict__ cgain, int32_t* __restrict__ cdest) {
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= ctot) return;
   const int32_t j = job_of(c_off, nj, i);
   const int32_t li = int32_t(i - c_off[j]);
-  cpart[i] = src[j].cpart[li];
-  cbnd[i] = src[j].cbnd[li];
+  const PackSrc& s = src[j];
+  cpart[i] = s.cpart[li];
+  cbnd[i] = s.cbnd[li];
+  if (cgain) {
+    cgain[i] = s.cgain ? s.cgain[li] : 0;
+    cdest[i] = s.cgain ? s.cdest[li] : -1;
+  }
 }
 
 int launch_pack(int32_t nj, const PackSrc* src, const int32_t* v_off, const int32_t* e_off, const int32_t* c_off,
                 int32_t ntot, int32_t etot, int32_t ctot, int32_t* xadj, int32_t* adj, int32_t* w, int32_t* vw,
-                int32_t* vjob, int32_t* cmap, int32_t* cpart, int32_t* cbnd, hipStream_t st) {
+                int32_t* vjob, int32_t* cmap, int32_t* cpart, int32_t* cbnd, int32_t* cgain, int32_t* cdest,
+                hipStream_t st) {
   if (ntot > 0)
     k_pack_vert<<<grid_for(ntot, 256), 256, 0, st>>>(nj, src, v_off, e_off, c_off, ntot, xadj, vw, vjob, cmap);
   fill<int32_t>(xadj + ntot, etot, 1, st);
   if (etot > 0) k_pack_edge<<<grid_for(etot, 256), 256, 0, st>>>(nj, src, v_off, e_off, etot, adj, w);
   if (cpart && ctot > 0)
-    k_pack_coarse<<<grid_for(ctot, 256), 256, 0, st>>>(nj, src, c_off, ctot, cpart, cbnd);
+    k_pack_coarse<<<grid_for(ctot, 256), 256, 0, st>>>(nj, src, c_off, ctot, cpart, cbnd, cgain, cdest);
+  return hip_status();
+}
+
+// expand_partition (gkway/uncoarsen.hpp:325-356) on a refinement pack, with
+// resumed levels taking their carried state.
+__global__ void __launch_bounds__(256) k_project_b(int32_t n, const int32_t* __restrict__ cmap,
+                                                   const int32_t* __restrict__ cpart,
+                                                   const int32_t* __restrict__ cbnd,
+                                                   const int32_t* __restrict__ cgain,
+                                                   const int32_t* __restrict__ cdest, int32_t* __restrict__ part,
+                                                   int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
+                                                   int32_t* __restrict__ dest, int32_t* __restrict__ upd) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  const int32_t c = cmap[v];
+  const int32_t p = cpart[c];
+  const int32_t b = cbnd[c] ? 1 : 0;
+  const int32_t d = cdest[c];
+  part[v] = p;
+  bnd[v] = b;
+  if (d >= 0) {
+    gain[v] = cgain[c];
+    dest[v] = d;
+    upd[v] = 0;
+  } else {
+    gain[v] = 0;
+    dest[v] = p;
+    upd[v] = b;
+  }
+}
+
+int launch_project_b(int32_t n, const int32_t* cmap, const int32_t* cpart, const int32_t* cbnd, const int32_t* cgain,
+                     const int32_t* cdest, int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* upd,
+                     hipStream_t st) {
+  if (n > 0) k_project_b<<<grid_for(n, 256), 256, 0, st>>>(n, cmap, cpart, cbnd, cgain, cdest, part, bnd, gain, dest, upd);
+  return hip_status();
+}
+
+// Candidate bitmask from boundary flags and gains, one wave per word.
+__global__ void __launch_bounds__(256) k_cand_init(int32_t n, const int32_t* __restrict__ bnd,
+                                                   const int32_t* __restrict__ gain, uint64_t* __restrict__ cand) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const bool c = v < n && bnd[v] && gain[v] > 0;
+  const uint64_t m = ballot(c);
+  if (lane_id() == 0 && (v >> 6) <= (int64_t(n) - 1) / 64) cand[v >> 6] = m;
+}
+
+int launch_cand_init(int32_t n, const int32_t* bnd, const int32_t* gain, uint64_t* cand, hipStream_t st) {
+  if (n > 0) k_cand_init<<<grid_for(n, 256), 256, 0, st>>>(n, bnd, gain, cand);
   return hip_status();
 }
 
@@ -159,55 +216,63 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 }
 
 // ---------------------------------------------------------------- rounds
-// Selection of a refinement round over the pack as a bitmask (one word per
-// wave of 64 consecutive vertices): create_independent_move_buffer
+// Selection of a refinement round: create_independent_move_buffer
 // (gkway/uncoarsen.hpp:111-173) with the cap and part weights of the vertex's
-// job.  Thread 0
-// also clears the worklist counters and the round's move counter.
-__global__ void __launch_bounds__(256) k_select_bits(int32_t n, const int32_t* __restrict__ xadj,
+// job, over the candidate bitmask kept by the gain kernels (boundary vertices
+// with positive gain), one thread per 64-vertex word.  Writes the selection
+// as a bitmask of the same layout and marks the jobs with a selection in
+// jflag.  Thread 0 also clears the worklist counters and the move counter.
+__global__ void __launch_bounds__(256) k_select_bits(int64_t nwords, const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ vw,
                                                      const int32_t* __restrict__ vjob,
-                                                     const int32_t* __restrict__ bnd,
                                                      const int32_t* __restrict__ gain,
                                                      const int32_t* __restrict__ dest,
                                                      const int32_t* __restrict__ pw,
                                                      const int32_t* __restrict__ pw_off,
                                                      const long long* __restrict__ cap,
+                                                     const uint64_t* __restrict__ cand,
                                                      uint64_t* __restrict__ bits, int32_t* __restrict__ reset,
-                                                     int32_t* __restrict__ active) {
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (v == 0) {
+                                                     int32_t* __restrict__ active, int32_t* __restrict__ jflag) {
+  const int64_t w = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (w == 0) {
     reset[0] = 0;
     reset[1] = 0;
     active[0] = 0;
   }
-  bool sel = false;
-  if (v < n && bnd[v]) {
-    const int32_t g = gain[v];
-    if (g > 0) {
-      const int32_t j = vjob[v];
-      const int32_t* jpw = pw + pw_off[j];
-      const long long jc = cap[j];
-      if ((long long)jpw[dest[v]] + vw[v] <= jc) {
-        sel = true;
-        for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) {
-          const int32_t u = adj[e];
-          if (u < v && gain[u] > 0 && (long long)jpw[dest[u]] + vw[u] <= jc) { sel = false; break; }
-        }
-      }
+  if (w >= nwords) return;
+  uint64_t c = cand[w], out = 0;
+  int32_t lastj = -1;
+  while (c) {
+    const int b = __ffsll(static_cast<long long>(c)) - 1;
+    c &= c - 1;
+    const int32_t v = int32_t((w << 6) + b);
+    const int32_t j = vjob[v];
+    const int32_t* jpw = pw + pw_off[j];
+    const long long jc = cap[j];
+    if ((long long)jpw[dest[v]] + vw[v] > jc) continue;
+    bool sel = true;
+    for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) {
+      const int32_t u = adj[e];
+      if (u < v && gain[u] > 0 && (long long)jpw[dest[u]] + vw[u] <= jc) { sel = false; break; }
+    }
+    if (!sel) continue;
+    out |= 1ull << b;
+    if (j != lastj) {
+      atomicOr(&jflag[j], 1);
+      lastj = j;
     }
   }
-  const uint64_t m = ballot(sel);
-  if (lane_id() == 0 && (v >> 6) <= (int64_t(n) - 1) / 64) bits[v >> 6] = m;
+  bits[w] = out;
 }
 
-int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* bnd, const int32_t* gain,
-                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                       uint64_t* bits, int32_t* reset, int32_t* active, hipStream_t st) {
+int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
+                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const uint64_t* cand,
+                       uint64_t* bits, int32_t* reset, int32_t* active, int32_t* jflag, hipStream_t st) {
   if (g.n <= 0) return 0;
-  k_select_bits<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, vjob, bnd, gain, dest, pw, pw_off, cap,
-                                                    bits, reset, active);
+  const int64_t nwords = (int64_t(g.n) + 63) / 64;
+  k_select_bits<<<grid_for(nwords, 256), 256, 0, st>>>(nwords, g.xadj, g.adj, g.vw, vjob, gain, dest, pw, pw_off, cap,
+                                                       cand, bits, reset, active, jflag);
   return hip_status();
 }
 
@@ -266,35 +331,61 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     const int32_t* __restrict__ dest, const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
     int32_t* __restrict__ pw, long long* __restrict__ cut, int32_t* __restrict__ rounds,
     long long* __restrict__ moved, int32_t* __restrict__ list, uint32_t* __restrict__ hkey, int32_t* __restrict__ upd,
-    int32_t* __restrict__ wl, int32_t* __restrict__ wlc, int32_t* __restrict__ active) {
+    int32_t* __restrict__ wl, int32_t* __restrict__ wlc, int32_t* __restrict__ active,
+    int32_t* __restrict__ lastcnt, int32_t* __restrict__ jflag) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
   const int32_t j = blockIdx.x;
   const int32_t vb = v_off[j], ve = v_off[j + 1];
   int32_t* seg = list + vb;
-  // 1. ordered compaction
+  // A job with nothing selected this round (flag from k_select_bits) is done
+  // at once; the flag is cleared for the next round.
+  if (t == 0) {
+    R.misc[3] = jflag[j];
+    jflag[j] = 0;
+  }
+  __syncthreads();
+  if (R.misc[3] == 0) {
+    if (t == 0) lastcnt[j] = 0;
+    return;
+  }
+  // 1. ordered compaction, kWordsPerThread consecutive words per thread
+  constexpr int kWordsPerThread = 8;
   int running = 0;
   const int64_t w0 = int64_t(vb) >> 6, w1 = (int64_t(ve) + 63) >> 6;
-  for (int64_t base = w0; base < w1; base += kBalThreads) {
-    const int64_t w = base + t;
-    uint64_t m = 0;
-    if (w < w1) {
-      m = bits[w];
-      const int64_t lo = w << 6;
-      if (lo < vb) m &= ~0ull << (vb - lo);
-      if (lo + 64 > ve) m &= (1ull << (ve - lo)) - 1;
+  for (int64_t base = w0; base < w1; base += int64_t(kBalThreads) * kWordsPerThread) {
+    const int64_t wt = base + int64_t(t) * kWordsPerThread;
+    uint64_t m[kWordsPerThread];
+    int c = 0;
+#pragma unroll
+    for (int q = 0; q < kWordsPerThread; ++q) {
+      const int64_t w = wt + q;
+      uint64_t x = 0;
+      if (w < w1) {
+        x = bits[w];
+        const int64_t lo = w << 6;
+        if (lo < vb) x &= ~0ull << (vb - lo);
+        if (lo + 64 > ve) x &= (1ull << (ve - lo)) - 1;
+      }
+      m[q] = x;
+      c += popc64(x);
     }
     int tot;
-    int off = running + block_excl_scan_big(popc64(m), &tot, R.ws);
-    while (m) {
-      const int b = __ffsll(static_cast<long long>(m)) - 1;
-      seg[off++] = int32_t((w << 6) + b);
-      m &= m - 1;
+    int off = running + block_excl_scan_big(c, &tot, R.ws);
+#pragma unroll
+    for (int q = 0; q < kWordsPerThread; ++q) {
+      uint64_t x = m[q];
+      while (x) {
+        const int b = __ffsll(static_cast<long long>(x)) - 1;
+        seg[off++] = int32_t(((wt + q) << 6) + b);
+        x &= x - 1;
+      }
     }
     running += tot;
   }
   const int32_t count = running;
+  if (t == 0) lastcnt[j] = count;
   if (count == 0) return;
   __syncthreads();
   if (t == 0) atomicAdd(active, count);
@@ -350,10 +441,12 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
         eq = h == prefix;
       }
       if (less) R.b.sk[atomicAdd(&R.misc[2], 1)] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
-      int tot;
-      const int r = eqrun + block_excl_scan_big(eq ? 1 : 0, &tot, R.ws);
-      if (eq && r < need) R.b.sk[(B - need) + r] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
-      eqrun += tot;
+      if (eqrun < need) {  // block-uniform: the equal keys still wanted
+        int tot;
+        const int r = eqrun + block_excl_scan_big(eq ? 1 : 0, &tot, R.ws);
+        if (eq && r < need) R.b.sk[(B - need) + r] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
+        eqrun += tot;
+      }
     }
   }
   __syncthreads();
@@ -402,12 +495,38 @@ int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, const uint64_
                    const int32_t* kk, const long long* cap, const int32_t* pw_off, const DGraph& g, int32_t* part,
                    const int32_t* dest, const int32_t* gain, int32_t* pw, long long* cut, int32_t* rounds,
                    long long* moved, int32_t* list, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc,
-                   int32_t* active, hipStream_t st) {
+                   int32_t* active, int32_t* lastcnt, int32_t* jflag, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
   k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, bits, M, kk, cap, pw_off, g.xadj, g.adj, part,
                                                              dest, gain, g.vw, pw, cut, rounds, moved, list, hkey, upd,
-                                                             wl, wlc, active);
+                                                             wl, wlc, active, lastcnt, jflag);
+  return hip_status();
+}
+
+// Largest row length of a graph (atomicMax per wave into *out, zeroed here).
+// Grid-stride; one atomic per block.
+__global__ void __launch_bounds__(256) k_max_deg(int32_t n, const int32_t* __restrict__ xadj, int32_t* __restrict__ out) {
+  __shared__ int wm[4];
+  int d = 0;
+  for (int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x; v < n; v += int64_t(gridDim.x) * 256) {
+    const int x = xadj[v + 1] - xadj[v];
+    d = x > d ? x : d;
+  }
+  d = wave_max(d);
+  if (lane_id() == 0) wm[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int m = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+    if (m > 0) atomicMax(out, m);
+  }
+}
+
+int launch_max_deg(const DGraph& g, int32_t* out, hipStream_t st) {
+  fill<int32_t>(out, 0, 1, st);
+  unsigned grid = grid_for(g.n, 256);
+  if (grid > 1024) grid = 1024;
+  if (g.n > 0) k_max_deg<<<grid, 256, 0, st>>>(g.n, g.xadj, out);
   return hip_status();
 }
 

@@ -16,9 +16,11 @@ struct PackSrc {
   const int32_t* adj;
   const int32_t* w;
   const int32_t* vw;
-  const int32_t* cmap;   // refinement packs: fine -> coarse (minus cbase = local ids), else null
+  const int32_t* cmap;   // refinement packs: fine -> coarse (minus cbase = local ids); null = identity
   const int32_t* cpart;  // refinement packs: coarse partition
   const int32_t* cbnd;   // refinement packs: coarse boundary flags
+  const int32_t* cgain;  // resumed levels: gain and destination carried over
+  const int32_t* cdest;  //   (null for a level entered by projection)
   int32_t xbase;
   int32_t abase;
   int32_t cbase;
@@ -34,9 +36,20 @@ struct UnpackDst {
 // writes xadj[ntot+1], adj/w[etot], vw/vjob[ntot]; with cmap != null also the
 // packed fine->coarse map (+c_off) and, with cpart != null, the packed coarse
 // partition/boundary arrays (ctot entries).
+// With cgain != null also cgain/cdest: the carried values of resumed jobs, and
+// dest = -1 for the others.
 int launch_pack(int32_t nj, const PackSrc* src, const int32_t* v_off, const int32_t* e_off, const int32_t* c_off,
                 int32_t ntot, int32_t etot, int32_t ctot, int32_t* xadj, int32_t* adj, int32_t* w, int32_t* vw,
-                int32_t* vjob, int32_t* cmap, int32_t* cpart, int32_t* cbnd, hipStream_t st);
+                int32_t* vjob, int32_t* cmap, int32_t* cpart, int32_t* cbnd, int32_t* cgain, int32_t* cdest,
+                hipStream_t st);
+// Projection of a refinement pack: expand_partition for levels entered from
+// the coarser level (gain 0, dest = part, update = boundary); resumed levels
+// (cdest >= 0) take their carried gain and dest and need no update.
+int launch_project_b(int32_t n, const int32_t* cmap, const int32_t* cpart, const int32_t* cbnd, const int32_t* cgain,
+                     const int32_t* cdest, int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* upd,
+                     hipStream_t st);
+// cand bit v = (bnd[v] && gain[v] > 0), all (n + 63) / 64 words written.
+int launch_cand_init(int32_t n, const int32_t* bnd, const int32_t* gain, uint64_t* cand, hipStream_t st);
 // out[j] = total vertex weight of job j (v_off on the device, nj+1).
 int launch_job_sum(int32_t nj, const int32_t* v_off, const int32_t* vw, long long* out, hipStream_t st);
 // c_off[j] = first coarse id of job j (c_off[nj] = nc) from the clustering.
@@ -46,18 +59,23 @@ int launch_job_offsets(int32_t nj, const int32_t* v_off, const int32_t* order, c
 int launch_job_eoffsets(int32_t nj, const int32_t* c_off, const int32_t* cxadj, int32_t* ce_off, hipStream_t st);
 int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, int32_t n, const int32_t* part,
                        const int32_t* bnd, hipStream_t st);
-// Refinement round of the batched form without host round trips: selection as
-// a bitmask, then one workgroup per job (ordered compaction, best min(count, M)
+// Refinement round of the batched form without host round trips: selection
+// over the candidate bitmask (kept by the gain kernels) into a bitmask, then
+// one workgroup per job (ordered compaction, best min(count, M)
 // moves, LDS sort, balanced prefix, apply).  *active receives the number of
-// selected moves of the round (0 once every job has converged).
-int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* bnd, const int32_t* gain,
-                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                       uint64_t* bits, int32_t* reset, int32_t* active, hipStream_t st);
+// selected moves of the round (0 once every job has converged) and lastcnt[j]
+// job j's count of the round.  jflag[j] (zero before the first round) marks
+// the jobs with a selection between the two launches.
+int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
+                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const uint64_t* cand,
+                       uint64_t* bits, int32_t* reset, int32_t* active, int32_t* jflag, hipStream_t st);
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, const uint64_t* bits, int32_t M,
                    const int32_t* kk, const long long* cap, const int32_t* pw_off, const DGraph& g, int32_t* part,
                    const int32_t* dest, const int32_t* gain, int32_t* pw, long long* cut, int32_t* rounds,
                    long long* moved, int32_t* list, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc,
-                   int32_t* active, hipStream_t st);
+                   int32_t* active, int32_t* lastcnt, int32_t* jflag, hipStream_t st);
+// *out = largest row length of g.
+int launch_max_deg(const DGraph& g, int32_t* out, hipStream_t st);
 // Final cut and part weights of the jobs covered by the chunk table: chunk c
 // is vertices [cs[c], ce[c]) of job cj[c] (pack ids); cut[job] and
 // pw64[pw_off[job] + q] accumulate (zeroed by the caller).

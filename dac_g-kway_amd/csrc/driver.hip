@@ -373,9 +373,10 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       GK_STAGE(ctx, GK_ST_PROJECT, 4LL * g.n, gkd::launch_compact(g.n, upd.p, wl.p, hcnt.p, st));
       GK_STAGE(ctx, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
-                                   work.p, g.n, st));
+                                   work.p, g.n, nullptr, st));
       GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
-               gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
+               gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, nullptr,
+                                       st));
       // A round updates at most B movers and their neighbours.
       const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
       const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
@@ -418,9 +419,10 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
                  gkd::launch_apply(g, M, moves.p, ctl.p, dest.p, cur_part, upd.p, wl.p, hcnt.p, st));
         GK_STAGE(ctx, GK_ST_GAIN, 0,
                  gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
-                                     work.p, int64_t(B) * (1 + 2 * avg_deg), st));
+                                     work.p, int64_t(B) * (1 + 2 * avg_deg), nullptr, st));
         GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
-                 gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, st));
+                 gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, nullptr,
+                                       st));
         pending = true;
       }
       long long hcut = 0;
@@ -742,8 +744,9 @@ int gk_k_gain(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, con
   if (g.n) GK_HIP(hipMemcpyAsync(flags.p, upd, sizeof(int32_t) * g.n, hipMemcpyDeviceToDevice, st));
   gkd::fill<int32_t>(cnt.p, 0, 2, st);
   GK_TRY(gkd::launch_compact(g.n, flags.p, wl.p, cnt.p, st));
-  GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, cnt.p, heavy.p, part, flags.p, bnd, gain, dest, nullptr, g.n, st));
-  GK_TRY(gkd::launch_gain_heavy(g, k, heavy.p, cnt.p, part, flags.p, bnd, gain, dest, nullptr, st));
+  GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, cnt.p, heavy.p, part, flags.p, bnd, gain, dest, nullptr, g.n, nullptr,
+                              st));
+  GK_TRY(gkd::launch_gain_heavy(g, k, heavy.p, cnt.p, part, flags.p, bnd, gain, dest, nullptr, nullptr, st));
   GK_HIP(hipStreamSynchronize(st));
   return GK_OK;
 }

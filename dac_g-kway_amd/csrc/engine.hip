@@ -17,6 +17,8 @@
 // order, the store itself is the next pack (no copy).
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <functional>
@@ -115,7 +117,17 @@ Engine::Engine(gk_ctx* ctx, hipStream_t st, int workers)
     : ctx_(ctx), st_(st), P_(ctx->pool), t0_(std::chrono::steady_clock::now()), up_(ctx->up), dl_(ctx->dl),
       pool_(workers) {}
 
-Engine::~Engine() = default;
+Engine::~Engine() {
+  // GK_ENGINE_PROFILE=1: one line per engine with the host wall time of its
+  // step phases (diagnostics only).
+  const char* e = std::getenv("GK_ENGINE_PROFILE");
+  if (e && e[0] == '1' && steps_ > 0)
+    std::fprintf(stderr,
+                 "[gk engine] steps %lld rounds %lld | coarsen %.1f ms, init launch %.1f ms, init collect %.1f ms, "
+                 "refine %.1f ms (rounds %.1f ms), idle wait %.1f ms, total %.1f ms\n",
+                 (long long)steps_, (long long)rounds_, prof_ms_[0], prof_ms_[1], prof_ms_[2], prof_ms_[3],
+                 prof_ms_[5], prof_ms_[4], now_ms());
+}
 
 int Engine::sync() {
   GK_HIP(hipStreamSynchronize(st_));
@@ -208,12 +220,15 @@ int Engine::pack_copy(const std::vector<PackSrc>& src, const std::vector<int32_t
     GK_TRY(rs->cmap.alloc(P_, S->n));
     GK_TRY(rs->cpart.alloc(P_, ctot));
     GK_TRY(rs->cbnd.alloc(P_, ctot));
+    GK_TRY(rs->cgain.alloc(P_, ctot));
+    GK_TRY(rs->cdest.alloc(P_, ctot));
     rs->c_off = c_off;
   }
-  GK_STAGE(ctx_, GK_ST_PACK, 20LL * S->n + 16LL * S->nnz + (rs ? 16LL * S->n + 16LL * ctot : 0),
+  GK_STAGE(ctx_, GK_ST_PACK, 20LL * S->n + 16LL * S->nnz + (rs ? 16LL * S->n + 24LL * ctot : 0),
            gkd::launch_pack(J, dsrc.p, S->dv_off.p, de_off.p, dc_off.p, S->n, S->nnz, ctot, S->bx.p, S->ba.p,
                             S->bw.p, S->bv.p, rs ? rs->vjob.p : nullptr, rs ? rs->cmap.p : nullptr,
-                            rs ? rs->cpart.p : nullptr, rs ? rs->cbnd.p : nullptr, st_));
+                            rs ? rs->cpart.p : nullptr, rs ? rs->cbnd.p : nullptr, rs ? rs->cgain.p : nullptr,
+                            rs ? rs->cdest.p : nullptr, st_));
   S->xadj = S->bx.p;
   S->adj = S->ba.p;
   S->w = S->bw.p;
@@ -520,7 +535,17 @@ int Engine::refine_step(bool* worked) {
     src[i].cbnd = j.cbnd->p + j.coff;
     nv[i] = f.n;
     ne[i] = f.nnz;
-    ncv[i] = j.L[j.cur + 1].n;
+    if (j.resume) {
+      // The level continues from its own state (identity projection): the
+      // partition, boundary flags, gains and destinations the uninterrupted
+      // rounds would have.
+      src[i].cmap = nullptr;
+      src[i].cgain = j.cgain->p + j.coff;
+      src[i].cdest = j.cdest->p + j.coff;
+      ncv[i] = f.n;
+    } else {
+      ncv[i] = j.L[j.cur + 1].n;
+    }
     kk[i] = j.prm.k;
     kmax = std::max(kmax, j.prm.k);
     pw_off[i + 1] = pw_off[i] + j.prm.k;
@@ -535,12 +560,16 @@ int Engine::refine_step(bool* worked) {
   const Store& F = *pk;
   const DGraph g = F.graph();
   const int32_t N = g.n;
-  Buf<int32_t> dpw_off, dkk, dpw, gain, dest, upd, wl, heavy, list, hcnt, active, drounds;
+  Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, jflag, dmax;
   Buf<long long> dcap, dcut, dmoved;
-  Buf<uint64_t> bits;
+  Buf<uint64_t> bits, cand;
   Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
+  auto gainp = std::make_shared<Buf<int32_t>>();
+  auto destp = std::make_shared<Buf<int32_t>>();
+  Buf<int32_t>& gain = *gainp;
+  Buf<int32_t>& dest = *destp;
   GK_TRY(upload(dpw_off, pw_off.data(), pw_off.size()));
   GK_TRY(upload(dkk, kk.data(), kk.size()));
   GK_TRY(upload(dpw, hpw.data(), hpw.size()));
@@ -555,67 +584,103 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(heavy.alloc(P_, N));
   GK_TRY(list.alloc(P_, N));
   GK_TRY(bits.alloc(P_, (int64_t(N) + 63) / 64 + 1));
+  GK_TRY(cand.alloc(P_, (int64_t(N) + 63) / 64 + 1));
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
   GK_TRY(drounds.alloc(P_, J));
+  GK_TRY(lastcnt.alloc(P_, J));
+  GK_TRY(jflag.alloc(P_, J));
+  GK_TRY(dmax.alloc(P_, 1));
+  gkd::fill<int32_t>(jflag.p, 0, J, st_);
   GK_TRY(dmoved.alloc(P_, J));
   gkd::fill<int32_t>(drounds.p, 0, J, st_);
   gkd::fill<long long>(dmoved.p, 0, J, st_);
-  GK_STAGE(ctx_, GK_ST_PROJECT, 32LL * N,
-           gkd::launch_project(N, rs.cmap.p, rs.cpart.p, rs.cbnd.p, part->p, bnd->p, gain.p, dest.p, upd.p, st_));
+  GK_STAGE(ctx_, GK_ST_PROJECT, 44LL * N,
+           gkd::launch_project_b(N, rs.cmap.p, rs.cpart.p, rs.cbnd.p, rs.cgain.p, rs.cdest.p, part->p, bnd->p,
+                                 gain.p, dest.p, upd.p, st_));
+  GK_STAGE(ctx_, GK_ST_PROJECT, 8LL * N + N / 8, gkd::launch_cand_init(N, bnd->p, gain.p, cand.p, st_));
   const int lanes = gain_lanes(g);
+  // Rows longer than the gain pass's lane count go to the heavy-row kernel;
+  // when there are none in the pack, that launch is skipped.
+  int32_t maxdeg = 0;
+  GK_TRY(gkd::launch_max_deg(g, dmax.p, st_));
+  GK_TRY(fetch(ctx_, st_, {{&maxdeg, dmax.p, sizeof maxdeg}}));
+  const bool heavy_rows = maxdeg > lanes;
   gkd::fill<int32_t>(hcnt.p, 0, 2, st_);
   GK_STAGE(ctx_, GK_ST_PROJECT, 4LL * N, gkd::launch_compact(N, upd.p, wl.p, hcnt.p, st_));
   GK_STAGE(ctx_, GK_ST_GAIN, 0,
            gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                               nullptr, N, st_));
-  GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
-           gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr, st_));
+                               nullptr, N, cand.p, st_));
+  if (heavy_rows)
+    GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
+             gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
+                                    cand.p, st_));
   const int64_t avg_deg = N > 0 ? (int64_t(g.nnz) + N - 1) / N : 0;
   const int64_t wl_items = std::min<int64_t>(N, int64_t(J) * M * (1 + 2 * avg_deg));
   // Rounds run back to back on the device; the host reads the last round's
   // move count every kRoundsPerCheck rounds.  A round in which no job selects
-  // a move changes nothing, so rounds past convergence are harmless.
+  // a move changes nothing, so rounds past convergence are harmless.  After
+  // kMaxRounds rounds the step ends; jobs still moving resume their level in
+  // the next step, so one slow job does not hold back the others.
   constexpr int kRoundsPerCheck = 4;
-  while (true) {
+  constexpr int kMaxRounds = 16;
+  const double tr0 = now_ms();
+  for (int done_rounds = 0; done_rounds < kMaxRounds; done_rounds += kRoundsPerCheck) {
+    rounds_ += kRoundsPerCheck;
     for (int r = 0; r < kRoundsPerCheck; ++r) {
-      GK_STAGE(ctx_, GK_ST_SELECT, 12LL * N + N / 8,
-               gkd::launch_select_bits(g, rs.vjob.p, bnd->p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, bits.p, hcnt.p,
-                                       active.p, st_));
+      GK_STAGE(ctx_, GK_ST_SELECT, N / 4,
+               gkd::launch_select_bits(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, cand.p, bits.p,
+                                       hcnt.p, active.p, jflag.p, st_));
       GK_STAGE(ctx_, GK_ST_BALANCE, N / 8 + 8LL * J,
                gkd::launch_round_b(J, kmax, F.dv_off.p, bits.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p,
                                    gain.p, dpw.p, dcut.p, drounds.p, dmoved.p, list.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                   active.p, st_));
+                                   active.p, lastcnt.p, jflag.p, st_));
       GK_STAGE(ctx_, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                                   nullptr, wl_items, st_));
-      GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
-               gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
-                                      st_));
+                                   nullptr, wl_items, cand.p, st_));
+      if (heavy_rows)
+        GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
+                 gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
+                                        cand.p, st_));
     }
     int32_t live = 0;
     GK_TRY(fetch(ctx_, st_, {{&live, active.p, sizeof live}}));
     if (live == 0) break;
   }
+  prof_ms_[5] += now_ms() - tr0;
   std::vector<long long> hcut(J), hmoved(J);
-  std::vector<int32_t> hrounds(J);
+  std::vector<int32_t> hrounds(J), hlast(J);
   GK_TRY(download({{hcut.data(), dcut.p, sizeof(long long) * J},
                    {hpw.data(), dpw.p, sizeof(int32_t) * hpw.size()},
                    {hrounds.data(), drounds.p, sizeof(int32_t) * J},
-                   {hmoved.data(), dmoved.p, sizeof(long long) * J}}));
+                   {hmoved.data(), dmoved.p, sizeof(long long) * J},
+                   {hlast.data(), lastcnt.p, sizeof(int32_t) * J}}));
   std::vector<UnpackDst> dst(J);
   std::vector<int32_t> done(J, 0);
   bool any_done = false;
   for (int32_t i = 0; i < J; ++i) {
     Job& j = *ids[i];
     gk_level_stats& ls = j.S.levels[j.cur];
-    ls.rounds = hrounds[i];
-    ls.moves = hmoved[i];
+    ls.rounds += hrounds[i];
+    ls.moves += hmoved[i];
     ls.cut_after = hcut[i];
     j.cut = hcut[i];
     std::copy(hpw.begin() + pw_off[i], hpw.begin() + pw_off[i + 1], j.hpw.begin());
     std::memset(&dst[i], 0, sizeof dst[i]);
+    if (hlast[i] > 0) {
+      // Still moving after this step's rounds: the level resumes next step.
+      j.resume = true;
+      j.cpart = part;
+      j.cbnd = bnd;
+      j.cgain = gainp;
+      j.cdest = destp;
+      j.coff = F.v_off[i];
+      continue;
+    }
+    j.resume = false;
+    j.cgain.reset();
+    j.cdest.reset();
     if (j.cur == 0) {
       dst[i].part = j.out;
       done[i] = 1;
@@ -630,7 +695,7 @@ int Engine::refine_step(bool* worked) {
   }
   if (any_done) {
     // Chunks of at most kCutChunk vertices of each finished job.
-    constexpr int32_t kCutChunk = 16384;
+    constexpr int32_t kCutChunk = 2048;
     std::vector<int32_t> cj, cs, ce;
     for (int32_t i = 0; i < J; ++i) {
       if (!done[i]) continue;
@@ -684,11 +749,23 @@ int Engine::on_done(Job& j) {
 
 int Engine::step(bool* worked) {
   *worked = false;
+  double t = now_ms();
+  auto lap = [&](int i) {
+    const double u = now_ms();
+    prof_ms_[i] += u - t;
+    t = u;
+  };
   GK_TRY(coarsen_step(worked));
+  lap(0);
   GK_TRY(init_step(worked));
+  lap(1);
   GK_TRY(collect_inits(false, worked));
+  lap(2);
   GK_TRY(refine_step(worked));
+  lap(3);
   if (!*worked) GK_TRY(collect_inits(true, worked));
+  lap(4);
+  ++steps_;
   return GK_OK;
 }
 

@@ -81,8 +81,10 @@ struct Job {
   int32_t nl = 0;    // levels kept (set when coarsening ends)
   int phase = COARSEN;
   int32_t cur = -1;  // level refined by the next refinement step
-  IBufP cpart, cbnd; // partition / boundary of level cur + 1 at offset coff
-  int32_t coff = 0;
+  IBufP cpart, cbnd; // partition / boundary of level cur + 1 at offset coff, or,
+  int32_t coff = 0;  // with resume, of level cur itself (a level not converged yet)
+  bool resume = false;
+  IBufP cgain, cdest;  // with resume: gains and destinations of level cur
   std::vector<int32_t> hpw;  // part weights
   long long cut = 0;         // tracked cut
   // initial partition (host worker)
@@ -108,7 +110,7 @@ struct Job {
 // Refinement pack extras: vertex -> job slot, packed projection map and the
 // packed coarse partition / boundary.
 struct RefineSide {
-  Buf<int32_t> vjob, cmap, cpart, cbnd;
+  Buf<int32_t> vjob, cmap, cpart, cbnd, cgain, cdest;
   std::vector<int32_t> c_off;
 };
 
@@ -163,6 +165,8 @@ class Engine {
   Pinned& up_;  // the context's staging arenas
   Pinned& dl_;
   size_t ndone_ = 0;
+  int64_t steps_ = 0, rounds_ = 0;
+  double prof_ms_[6] = {0, 0, 0, 0, 0, 0};  // host wall time per step phase
   std::vector<std::unique_ptr<Job>> jobs_;
   WorkerPool pool_;  // declared last: joined before the jobs it works on go away
 };

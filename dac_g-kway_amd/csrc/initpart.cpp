@@ -318,8 +318,11 @@ void grow(const G& g, int32_t seedv, int64_t target0, std::vector<uint8_t>& side
   Heap q;
   q.reset(n);
   int64_t w0 = 0;
-  std::vector<uint8_t> touched(n, 0);
   int32_t next_unreached = 0;
+  // Total incident edge weight of every vertex.
+  std::vector<int64_t> inc(n, 0);
+  for (int32_t u = 0; u < n; ++u)
+    for (int32_t k = g.xadj[u]; k < g.xadj[u + 1]; ++k) inc[u] += g.w[k];
   auto add = [&](int32_t v) {
     side[v] = 0;
     w0 += g.vw[v];
@@ -328,9 +331,7 @@ void grow(const G& g, int32_t seedv, int64_t target0, std::vector<uint8_t>& side
       if (side[u] == 0) continue;
       conn0[u] += g.w[j];
       // gain of moving u to side 0: 2*conn0 - total incident weight
-      int64_t tot = 0;
-      for (int32_t k = g.xadj[u]; k < g.xadj[u + 1]; ++k) tot += g.w[k];
-      int64_t gain = 2 * conn0[u] - tot;
+      int64_t gain = 2 * conn0[u] - inc[u];
       if (q.contains(u)) q.update(u, gain); else q.push(u, gain);
     }
   };
@@ -434,7 +435,7 @@ class ForkJoin {
  public:
   struct Task {
     std::function<void()> fn;
-    std::atomic<bool> done{false};
+    bool done = false;  // guarded by the pool mutex
   };
   static ForkJoin& get() {
     static ForkJoin pool;
@@ -448,14 +449,21 @@ class ForkJoin {
       std::lock_guard<std::mutex> lk(m_);
       q_.push_back(t);
     }
-    cv_.notify_one();
+    cv_.notify_all();
     return t;
   }
+  // Runs queued tasks until t is done; sleeps when there is nothing to run.
   void wait(const std::shared_ptr<Task>& t) {
-    while (!t->done.load(std::memory_order_acquire)) {
-      std::shared_ptr<Task> o = pop();
-      if (o) run(*o);
-      else std::this_thread::yield();
+    while (true) {
+      std::shared_ptr<Task> o;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return t->done || !q_.empty(); });
+        if (t->done) return;
+        o = q_.back();  // newest first: the waiter's own, most likely
+        q_.pop_back();
+      }
+      run(*o);
     }
   }
 
@@ -486,16 +494,13 @@ class ForkJoin {
     cv_.notify_all();
     for (auto& t : th_) t.join();
   }
-  std::shared_ptr<Task> pop() {
-    std::lock_guard<std::mutex> lk(m_);
-    if (q_.empty()) return nullptr;
-    auto t = q_.back();  // newest first: the waiter's own, most likely
-    q_.pop_back();
-    return t;
-  }
-  static void run(Task& t) {
+  void run(Task& t) {
     t.fn();
-    t.done.store(true, std::memory_order_release);
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      t.done = true;
+    }
+    cv_.notify_all();
   }
   std::vector<std::thread> th_;
   std::deque<std::shared_ptr<Task>> q_;

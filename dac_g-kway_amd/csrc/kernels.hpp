@@ -81,12 +81,13 @@ int launch_compact(int32_t n, const int32_t* flags, int32_t* list, int32_t* cnt,
 // recomputed, [1] their entries.
 int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
                    const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
-                   unsigned long long* work, int64_t max_items, hipStream_t st);
+                   unsigned long long* work, int64_t max_items, uint64_t* cand, hipStream_t st);
 // Rows collected in heavy[0..wlc[1]) by launch_gain_wl: one wave each, LDS
-// histogram of the k part connectivities.
+// histogram of the k part connectivities.  Both gain kernels keep the optional
+// bitmask cand (bit v = boundary vertex with positive gain) in step.
 int launch_gain_heavy(const DGraph& g, int32_t k, const int32_t* heavy, const int32_t* wlc, const int32_t* part,
                       int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest, unsigned long long* work,
-                      hipStream_t st);
+                      uint64_t* cand, hipStream_t st);
 // Selection: appends keys ((~gain & 0x7fffffff) << idbits | v) of the
 // selected vertices; ctl[0] = count, ctl[1] = max gain.  clear_upd zeroes
 // upd[] (the reference's behaviour); reset (may be null) gets reset[0] =

@@ -87,6 +87,14 @@ int launch_compact(int32_t n, const int32_t* flags, int32_t* list, int32_t* cnt,
   return hip_status();
 }
 
+// Candidate bitmask of the batched form: bit v set iff v is a boundary vertex
+// with positive gain (the vertices its selection looks at).
+__device__ __forceinline__ void set_cand(uint64_t* cand, int32_t v, bool on) {
+  const uint64_t b = 1ull << (v & 63);
+  if (on) atomicOr(reinterpret_cast<unsigned long long*>(&cand[v >> 6]), (unsigned long long)b);
+  else atomicAnd(reinterpret_cast<unsigned long long*>(&cand[v >> 6]), (unsigned long long)~b);
+}
+
 // update_vertex_info + calculate_gain (gkway/uncoarsen.hpp:175-237, 82-109)
 // over a worklist of updated vertices (wlc[0] entries), S lanes per vertex,
 // grid-stride.  Each lane holds one (part, weight) entry of the row and sums
@@ -104,7 +112,8 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
                                                  int32_t* __restrict__ upd, int32_t* __restrict__ bnd,
                                                  int32_t* __restrict__ gain, int32_t* __restrict__ dest,
                                                  int32_t* __restrict__ heavy,
-                                                 unsigned long long* __restrict__ work) {
+                                                 unsigned long long* __restrict__ work,
+                                                 uint64_t* __restrict__ cand) {
   const int32_t cnt = wlc[0];
   const int64_t sg0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) / S;
   const int64_t nsg = int64_t(gridDim.x) * (256 / S);
@@ -150,6 +159,7 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
         gain[v] = 0;
         dest[v] = p;
       }
+      if (cand) set_cand(cand, v, anyext && g > 0);
       upd[v] = 0;
       ++done_v;
       done_e += d;
@@ -175,7 +185,8 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
                                                    int32_t* __restrict__ upd,
                                                    int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
                                                    int32_t* __restrict__ dest,
-                                                   unsigned long long* __restrict__ work) {
+                                                   unsigned long long* __restrict__ work,
+                                                   uint64_t* __restrict__ cand) {
   extern __shared__ int32_t hist[];
   const int lane = lane_id();
   const int32_t cnt = wlc[1];
@@ -211,6 +222,7 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
       bnd[v] = any ? 1 : 0;
       if (any && g > 0) { gain[v] = g; dest[v] = bq; }
       else { gain[v] = 0; dest[v] = p; }
+      if (cand) set_cand(cand, v, any && g > 0);
       upd[v] = 0;
       if (work) {
         atomicAdd(&work[0], 1ull);
@@ -223,7 +235,7 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
 
 int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
                    const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
-                   unsigned long long* work, int64_t max_items, hipStream_t st) {
+                   unsigned long long* work, int64_t max_items, uint64_t* cand, hipStream_t st) {
   if (g.n <= 0) return 0;
   // The grid covers at most max_items worklist entries (a bound the caller
   // knows without reading the count); the kernel grid-strides beyond it.
@@ -232,7 +244,7 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
   if (grid > 2048) grid = 2048;
 #define GK_GAIN_CASE(S)                                                                            \
   case S:                                                                                           \
-    k_gain_wl<S><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work); \
+    k_gain_wl<S><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work, cand); \
     break;
   switch (lanes) {
     GK_GAIN_CASE(4)
@@ -240,7 +252,8 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
     GK_GAIN_CASE(16)
     GK_GAIN_CASE(32)
     default:
-      k_gain_wl<64><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work);
+      k_gain_wl<64><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work,
+                                          cand);
       break;
   }
 #undef GK_GAIN_CASE
@@ -249,10 +262,13 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
 
 int launch_gain_heavy(const DGraph& g, int32_t k, const int32_t* heavy, const int32_t* wlc, const int32_t* part,
                       int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest, unsigned long long* work,
-                      hipStream_t st) {
+                      uint64_t* cand, hipStream_t st) {
   if (g.n <= 0) return 0;
-  k_gain_heavy<<<256, 64, sizeof(int32_t) * k, st>>>(heavy, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain,
-                                                     dest, work);
+  // Grid-stride over the heavy list; more waves when k (the LDS histogram)
+  // is small.
+  const unsigned grid = k <= 1024 ? 2048 : 256;
+  k_gain_heavy<<<grid, 64, sizeof(int32_t) * k, st>>>(heavy, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain,
+                                                      dest, work, cand);
   return hip_status();
 }
 
