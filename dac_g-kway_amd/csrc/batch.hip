@@ -219,91 +219,154 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // Selection of a refinement round: create_independent_move_buffer
 // (gkway/uncoarsen.hpp:111-173) with the cap and part weights of the vertex's
 // job, over the candidate bitmask kept by the gain kernels (boundary vertices
-// with positive gain), one thread per 64-vertex word.  Writes the selection
-// as a bitmask of the same layout and marks the jobs with a selection in
-// jflag.  Thread 0 also clears the worklist counters and the move counter.
-__global__ void __launch_bounds__(256) k_select_bits(int64_t nwords, const int32_t* __restrict__ xadj,
+// with positive gain): one wave per 64-vertex word, so vertices outside the
+// mask cost one shared 8-byte load.  Selected vertices are appended, in no
+// particular order, to their job's segment of list (list + v_off[job],
+// jcount[job] entries; jcount zero before the launch).  Thread 0 also clears
+// the worklist counters and the round's move counter, which only the round
+// kernel adds to.  vjob == null: one job.
+// Appends the selected vertices of one wave to their jobs' list segments:
+// the selected lanes form runs of equal job id (ids do not decrease along the
+// wave) and the first lane of each run reserves the run's slots with one
+// atomic.  Every lane of the wave calls this.
+__device__ __forceinline__ void append_selected(bool sel, int32_t j, int32_t v, const int32_t* __restrict__ v_off,
+                                                int32_t* __restrict__ list, int32_t* __restrict__ jcount) {
+  const uint64_t m = ballot(sel);
+  if (m == 0) return;
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  const uint64_t lower = m & lt;
+  const int p = lower ? 63 - __clzll(static_cast<long long>(lower)) : lane;
+  const int32_t jp = __shfl(j, p, 64);
+  const bool leader = sel && (lower == 0 || jp != j);
+  const uint64_t L = ballot(leader);
+  int base = 0;
+  if (leader) {
+    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const uint64_t rest = L & ~le;
+    const int q = rest ? __ffsll(static_cast<long long>(rest)) - 1 : 64;
+    const uint64_t upto = q == 64 ? ~0ull : ((1ull << q) - 1);
+    base = atomicAdd(&jcount[j], popc64(m & upto & ~((1ull << lane) - 1)));
+  }
+  const uint64_t lmask = L & (lt | (1ull << lane));
+  const int my = lmask ? 63 - __clzll(static_cast<long long>(lmask)) : lane;
+  base = __shfl(base, my, 64);
+  if (sel) {
+    const int off = popc64(m & lt) - popc64(m & ((1ull << my) - 1));
+    list[v_off[j] + base + off] = v;
+  }
+}
+
+// Rows longer than this are checked by k_select_long, one wave per vertex.
+constexpr int32_t kSelectShortRow = 32;
+
+__global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ vw,
                                                      const int32_t* __restrict__ vjob,
+                                                     const int32_t* __restrict__ v_off,
                                                      const int32_t* __restrict__ gain,
                                                      const int32_t* __restrict__ dest,
                                                      const int32_t* __restrict__ pw,
                                                      const int32_t* __restrict__ pw_off,
                                                      const long long* __restrict__ cap,
                                                      const uint64_t* __restrict__ cand,
-                                                     uint64_t* __restrict__ bits, int32_t* __restrict__ reset,
-                                                     int32_t* __restrict__ active, int32_t* __restrict__ jflag) {
-  const int64_t w = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (w == 0) {
+                                                     int32_t* __restrict__ list, int32_t* __restrict__ jcount,
+                                                     int32_t* __restrict__ longv, int32_t* __restrict__ longcnt,
+                                                     int32_t* __restrict__ reset, int32_t* __restrict__ active) {
+  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (v == 0) {
     reset[0] = 0;
     reset[1] = 0;
     active[0] = 0;
   }
-  if (w >= nwords) return;
-  uint64_t c = cand[w], out = 0;
-  int32_t lastj = -1;
-  while (c) {
-    const int b = __ffsll(static_cast<long long>(c)) - 1;
-    c &= c - 1;
-    const int32_t v = int32_t((w << 6) + b);
-    const int32_t j = vjob[v];
+  const bool valid = v < n;
+  const uint64_t cw = valid ? cand[v >> 6] : 0;  // one word per wave
+  bool sel = false, lng = false;
+  int32_t j = 0;
+  if ((cw >> (v & 63)) & 1ull) {
+    j = vjob ? vjob[v] : 0;
     const int32_t* jpw = pw + pw_off[j];
     const long long jc = cap[j];
-    if ((long long)jpw[dest[v]] + vw[v] > jc) continue;
-    bool sel = true;
-    for (int32_t e = xadj[v]; e < xadj[v + 1]; ++e) {
-      const int32_t u = adj[e];
-      if (u < v && gain[u] > 0 && (long long)jpw[dest[u]] + vw[u] <= jc) { sel = false; break; }
-    }
-    if (!sel) continue;
-    out |= 1ull << b;
-    if (j != lastj) {
-      atomicOr(&jflag[j], 1);
-      lastj = j;
+    if ((long long)jpw[dest[v]] + vw[v] <= jc) {
+      const int32_t s = xadj[v], e = xadj[v + 1];
+      if (e - s > kSelectShortRow) {
+        lng = true;
+      } else {
+        sel = true;
+        for (int32_t x = s; x < e; ++x) {
+          const int32_t u = adj[x];
+          if (u < v && gain[u] > 0 && (long long)jpw[dest[u]] + vw[u] <= jc) { sel = false; break; }
+        }
+      }
     }
   }
-  bits[w] = out;
+  const uint64_t lm = ballot(lng);
+  if (lm) {
+    const int leader = __ffsll(static_cast<long long>(lm)) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(longcnt, popc64(lm));
+    base = __shfl(base, leader, 64);
+    if (lng) longv[base + popc64(lm & lanemask_lt())] = int32_t(v);
+  }
+  append_selected(sel, j, int32_t(v), v_off, list, jcount);
 }
 
-int launch_select_bits(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
-                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const uint64_t* cand,
-                       uint64_t* bits, int32_t* reset, int32_t* active, int32_t* jflag, hipStream_t st) {
-  if (g.n <= 0) return 0;
-  const int64_t nwords = (int64_t(g.n) + 63) / 64;
-  k_select_bits<<<grid_for(nwords, 256), 256, 0, st>>>(nwords, g.xadj, g.adj, g.vw, vjob, gain, dest, pw, pw_off, cap,
-                                                       cand, bits, reset, active, jflag);
-  return hip_status();
-}
-
-// Exclusive scan of one int per thread over a kBalThreads block; *tot gets
-// the block total.  ws: >= kBalThreads / 64 + 1 ints of LDS.
-__device__ __forceinline__ int block_excl_scan_big(int x, int* tot, int32_t* ws) {
+// Movable vertices with rows longer than kSelectShortRow: one wave checks 64
+// neighbours at a time.
+__global__ void __launch_bounds__(256) k_select_long(const int32_t* __restrict__ longv,
+                                                     const int32_t* __restrict__ longcnt,
+                                                     const int32_t* __restrict__ xadj,
+                                                     const int32_t* __restrict__ adj,
+                                                     const int32_t* __restrict__ vw,
+                                                     const int32_t* __restrict__ vjob,
+                                                     const int32_t* __restrict__ v_off,
+                                                     const int32_t* __restrict__ gain,
+                                                     const int32_t* __restrict__ dest,
+                                                     const int32_t* __restrict__ pw,
+                                                     const int32_t* __restrict__ pw_off,
+                                                     const long long* __restrict__ cap,
+                                                     int32_t* __restrict__ list, int32_t* __restrict__ jcount) {
+  const int32_t cnt = *longcnt;
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
-  const int incl = wave_incl_scan(x);
-  if (lane == 63) ws[wave] = incl;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int w = 0; w < kBalThreads / 64; ++w) {
-      const int t = ws[w];
-      ws[w] = run;
-      run += t;
+  const int64_t nw = int64_t(gridDim.x) * 4;
+  for (int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); i < cnt; i += nw) {
+    const int32_t v = longv[i];
+    const int32_t j = vjob ? vjob[v] : 0;
+    const int32_t* jpw = pw + pw_off[j];
+    const long long jc = cap[j];
+    const int32_t s = xadj[v], e = xadj[v + 1];
+    bool blocked = false;
+    for (int32_t x0 = s; x0 < e && !blocked; x0 += 64) {
+      const int32_t x = x0 + lane;
+      bool b = false;
+      if (x < e) {
+        const int32_t u = adj[x];
+        b = u < v && gain[u] > 0 && (long long)jpw[dest[u]] + vw[u] <= jc;
+      }
+      blocked = ballot(b) != 0;
     }
-    ws[kBalThreads / 64] = run;
+    if (!blocked && lane == 0) list[v_off[j] + atomicAdd(&jcount[j], 1)] = v;
   }
-  __syncthreads();
-  const int r = incl - x + ws[wave];
-  *tot = ws[kBalThreads / 64];
-  __syncthreads();
-  return r;
+}
+
+int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* gain,
+                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
+                       const uint64_t* cand, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
+                       int32_t* reset, int32_t* active, hipStream_t st) {
+  if (g.n <= 0) return 0;
+  k_select_list<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, vjob, v_off, gain, dest, pw, pw_off, cap,
+                                                    cand, list, jcount, longv, longcnt, reset, active);
+  unsigned grid = grid_for(g.n, 256);
+  if (grid > 1024) grid = 1024;
+  k_select_long<<<grid, 256, 0, st>>>(longv, longcnt, g.xadj, g.adj, g.vw, vjob, v_off, gain, dest, pw, pw_off, cap,
+                                      list, jcount);
+  return hip_status();
 }
 
 struct RoundLds {
   BalanceLds b;
   int32_t hist[256];
-  int32_t ws[kBalThreads / 64 + 1];
   int32_t misc[4];
 };
 
@@ -313,82 +376,44 @@ __device__ __forceinline__ uint64_t move_key(const int32_t* __restrict__ gain, i
   return (uint64_t(uint32_t(0x7fffffff - gain[v])) << 32) | uint64_t(uint32_t(v));
 }
 
-// One refinement round of one job per workgroup, after k_select_bits:
-//   1. ordered compaction of the job's selected vertices (ascending ids) into
-//      list[v_off[j] ...];
+// One refinement round of one job per workgroup, after k_select_list:
+//   1. the job's selected vertices: list[v_off[j] ...], jcount[j] of them
+//      (jcount is cleared here for the next round);
 //   2. the min(count, M) best moves by (gain desc, id asc): all of them when
-//      count <= M, else a radix select on the gain (the smallest ids win
-//      among equal gains, list order being id order);
+//      count <= M, else an 8-pass radix select on the 64-bit key (gain key,
+//      id) -- keys are distinct, so exactly M are taken;
 //   3. LDS bitonic sort, balanced prefix (balance_prefix);
 //   4. apply of the prefix: part, update flags and the gain worklist
 //      (apply_sequence_move, gkway/uncoarsen.hpp:290-323).
 // A job with nothing selected is left untouched, so extra rounds after the
 // last productive one change nothing.
 __global__ void __launch_bounds__(kBalThreads) k_round_b(
-    const int32_t* __restrict__ v_off, const uint64_t* __restrict__ bits, int32_t M,
-    const int32_t* __restrict__ kk, const long long* __restrict__ cap, const int32_t* __restrict__ pw_off,
-    const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj, int32_t* __restrict__ part,
-    const int32_t* __restrict__ dest, const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
-    int32_t* __restrict__ pw, long long* __restrict__ cut, int32_t* __restrict__ rounds,
-    long long* __restrict__ moved, int32_t* __restrict__ list, uint32_t* __restrict__ hkey, int32_t* __restrict__ upd,
-    int32_t* __restrict__ wl, int32_t* __restrict__ wlc, int32_t* __restrict__ active,
-    int32_t* __restrict__ lastcnt, int32_t* __restrict__ jflag) {
+    const int32_t* __restrict__ v_off, int32_t M, const int32_t* __restrict__ kk, const long long* __restrict__ cap,
+    const int32_t* __restrict__ pw_off, const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj,
+    int32_t* __restrict__ part, const int32_t* __restrict__ dest, const int32_t* __restrict__ gain,
+    const int32_t* __restrict__ vw, int32_t* __restrict__ pw, long long* __restrict__ cut,
+    int32_t* __restrict__ rounds, long long* __restrict__ moved, const int32_t* __restrict__ list,
+    int32_t* __restrict__ jcount, uint32_t* __restrict__ hkey, int32_t* __restrict__ upd, int32_t* __restrict__ wl,
+    int32_t* __restrict__ wlc, int32_t* __restrict__ lastcnt, int32_t* __restrict__ active,
+    int32_t* __restrict__ longcnt) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
   const int32_t j = blockIdx.x;
-  const int32_t vb = v_off[j], ve = v_off[j + 1];
-  int32_t* seg = list + vb;
-  // A job with nothing selected this round (flag from k_select_bits) is done
-  // at once; the flag is cleared for the next round.
   if (t == 0) {
-    R.misc[3] = jflag[j];
-    jflag[j] = 0;
+    R.misc[3] = jcount[j];
+    jcount[j] = 0;
+    if (j == 0) *longcnt = 0;  // the selection kernels of this round are done
   }
   __syncthreads();
-  if (R.misc[3] == 0) {
-    if (t == 0) lastcnt[j] = 0;
-    return;
+  const int32_t count = R.misc[3];
+  if (t == 0) {
+    lastcnt[j] = count;
+    if (count) atomicAdd(active, count);
   }
-  // 1. ordered compaction, kWordsPerThread consecutive words per thread
-  constexpr int kWordsPerThread = 8;
-  int running = 0;
-  const int64_t w0 = int64_t(vb) >> 6, w1 = (int64_t(ve) + 63) >> 6;
-  for (int64_t base = w0; base < w1; base += int64_t(kBalThreads) * kWordsPerThread) {
-    const int64_t wt = base + int64_t(t) * kWordsPerThread;
-    uint64_t m[kWordsPerThread];
-    int c = 0;
-#pragma unroll
-    for (int q = 0; q < kWordsPerThread; ++q) {
-      const int64_t w = wt + q;
-      uint64_t x = 0;
-      if (w < w1) {
-        x = bits[w];
-        const int64_t lo = w << 6;
-        if (lo < vb) x &= ~0ull << (vb - lo);
-        if (lo + 64 > ve) x &= (1ull << (ve - lo)) - 1;
-      }
-      m[q] = x;
-      c += popc64(x);
-    }
-    int tot;
-    int off = running + block_excl_scan_big(c, &tot, R.ws);
-#pragma unroll
-    for (int q = 0; q < kWordsPerThread; ++q) {
-      uint64_t x = m[q];
-      while (x) {
-        const int b = __ffsll(static_cast<long long>(x)) - 1;
-        seg[off++] = int32_t(((wt + q) << 6) + b);
-        x &= x - 1;
-      }
-    }
-    running += tot;
-  }
-  const int32_t count = running;
-  if (t == 0) lastcnt[j] = count;
   if (count == 0) return;
-  __syncthreads();
-  if (t == 0) atomicAdd(active, count);
+  const int32_t vb = v_off[j];
+  const int32_t* seg = list + vb;
   const int32_t B = count < M ? count : M;
   // 2. the B best keys into L.sk[0, B)
   if (count <= M) {
@@ -398,14 +423,14 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     uint32_t* hk = hkey + vb;
     for (int32_t i = t; i < count; i += kBalThreads) hk[i] = uint32_t(0x7fffffff - gain[seg[i]]);
     __syncthreads();
-    uint32_t prefix = 0, pmask = 0;
+    uint64_t prefix = 0, pmask = 0;
     int32_t need = B;
-    for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int shift = 56; shift >= 0; shift -= 8) {
       if (t < 256) R.hist[t] = 0;
       __syncthreads();
       for (int32_t i = t; i < count; i += kBalThreads) {
-        const uint32_t h = hk[i];
-        if ((h & pmask) == prefix) atomicAdd(&R.hist[(h >> shift) & 255], 1);
+        const uint64_t key = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
+        if ((key & pmask) == prefix) atomicAdd(&R.hist[(key >> shift) & 255], 1);
       }
       __syncthreads();
       if (t == 0) {
@@ -419,34 +444,18 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
       }
       __syncthreads();
       need -= R.misc[1];
-      prefix |= uint32_t(R.misc[0]) << shift;
-      pmask |= 255u << shift;
+      prefix |= uint64_t(R.misc[0]) << shift;
+      pmask |= 255ull << shift;
       __syncthreads();
     }
-    // prefix = gain key of the B-th best move; every key below it is taken,
-    // and the first `need` (smallest ids) equal to it.
+    // prefix is now the B-th smallest key; keys are distinct, so the keys
+    // <= prefix are exactly the B best.
     if (t == 0) R.misc[2] = 0;
     R.b.sk[t] = ~0ull;
     __syncthreads();
-    int32_t eqrun = 0;
-    for (int32_t base = 0; base < count; base += kBalThreads) {
-      const int32_t i = base + t;
-      bool less = false, eq = false;
-      int32_t v = 0;
-      uint32_t h = 0;
-      if (i < count) {
-        v = seg[i];
-        h = hk[i];
-        less = h < prefix;
-        eq = h == prefix;
-      }
-      if (less) R.b.sk[atomicAdd(&R.misc[2], 1)] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
-      if (eqrun < need) {  // block-uniform: the equal keys still wanted
-        int tot;
-        const int r = eqrun + block_excl_scan_big(eq ? 1 : 0, &tot, R.ws);
-        if (eq && r < need) R.b.sk[(B - need) + r] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
-        eqrun += tot;
-      }
+    for (int32_t i = t; i < count; i += kBalThreads) {
+      const uint64_t key = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
+      if (key <= prefix) R.b.sk[atomicAdd(&R.misc[2], 1)] = key;
     }
   }
   __syncthreads();
@@ -491,16 +500,16 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   }
 }
 
-int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, const uint64_t* bits, int32_t M,
-                   const int32_t* kk, const long long* cap, const int32_t* pw_off, const DGraph& g, int32_t* part,
-                   const int32_t* dest, const int32_t* gain, int32_t* pw, long long* cut, int32_t* rounds,
-                   long long* moved, int32_t* list, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc,
-                   int32_t* active, int32_t* lastcnt, int32_t* jflag, hipStream_t st) {
+int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, const int32_t* kk, const long long* cap,
+                   const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
+                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const int32_t* list,
+                   int32_t* jcount, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
+                   int32_t* active, int32_t* longcnt, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
-  k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, bits, M, kk, cap, pw_off, g.xadj, g.adj, part,
-                                                             dest, gain, g.vw, pw, cut, rounds, moved, list, hkey, upd,
-                                                             wl, wlc, active, lastcnt, jflag);
+  k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, M, kk, cap, pw_off, g.xadj, g.adj, part, dest,
+                                                             gain, g.vw, pw, cut, rounds, moved, list, jcount, hkey,
+                                                             upd, wl, wlc, lastcnt, active, longcnt);
   return hip_status();
 }
 

@@ -22,6 +22,7 @@
 
 #include "../../include/gkway.h"
 #include "host_graph.hpp"
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "prim.hpp"
 #include "runtime.hpp"
@@ -334,30 +335,52 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   const auto t2 = clk::now();
 
   // ---------------------------------------------------------- uncoarsening
+  // Rounds run back to back on the device (select_list, round_b with one
+  // job, gain passes); the host reads the last round's move count every
+  // kRoundsPerCheck rounds.  A round with nothing selected changes nothing.
   int32_t* cur_part = partA.p;
   int32_t* cur_bnd = bndA.p;
   int32_t* oth_part = partB.p;
   int32_t* oth_bnd = bndB.p;
   if (nl > 1) {
     const int32_t M = prm.max_moves;
-    Buf<int32_t> gain, dest, upd, heavy, hcnt, moves, ctl, sscr;
+    Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, jcount, active, drounds, lastcnt, dvoff, dpw_off, dkk, dmax,
+        longv, longcnt;
+    Buf<long long> dcap, dmoved;
+    Buf<uint64_t> cand;
+    Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
     GK_TRY(work.alloc(P, 8));
     gkd::fill<unsigned long long>(work.p, 0ull, 8, st);
-    Buf<uint64_t> keys, keys2;
     GK_TRY(gain.alloc(P, n0));
     GK_TRY(dest.alloc(P, n0));
     GK_TRY(upd.alloc(P, n0));
     GK_TRY(heavy.alloc(P, n0));
-    GK_TRY(hcnt.alloc(P, 2));  // [0] gain worklist count, [1] heavy-row count
-    Buf<int32_t> wl;
     GK_TRY(wl.alloc(P, n0));
-    GK_TRY(moves.alloc(P, M));
-    GK_TRY(ctl.alloc(P, 4));
-    GK_TRY(keys.alloc(P, n0));
-    GK_TRY(keys2.alloc(P, n0));
-    GK_TRY(sscr.alloc(P, gkd::sort_scratch_ints(n0)));
-    gkd::fill<int32_t>(ctl.p, -1, 4, st);
+    GK_TRY(list.alloc(P, n0));
+    GK_TRY(hkey.alloc(P, n0));
+    GK_TRY(cand.alloc(P, (int64_t(n0) + 63) / 64 + 1));
+    GK_TRY(hcnt.alloc(P, 2));  // [0] gain worklist count, [1] heavy-row count
+    GK_TRY(jcount.alloc(P, 1));
+    GK_TRY(active.alloc(P, 1));
+    GK_TRY(drounds.alloc(P, 1));
+    GK_TRY(lastcnt.alloc(P, 1));
+    GK_TRY(dmoved.alloc(P, 1));
+    GK_TRY(dvoff.alloc(P, 2));
+    GK_TRY(dpw_off.alloc(P, 2));
+    GK_TRY(dkk.alloc(P, 1));
+    GK_TRY(dcap.alloc(P, 1));
+    GK_TRY(dmax.alloc(P, 1));
+    GK_TRY(longv.alloc(P, n0));
+    GK_TRY(longcnt.alloc(P, 1));
+    gkd::fill<int32_t>(jcount.p, 0, 1, st);
+    gkd::fill<int32_t>(longcnt.p, 0, 1, st);
+    gkd::fill<int32_t>(dvoff.p, 0, 1, st);
+    gkd::fill<int32_t>(dpw_off.p, 0, 1, st);
+    gkd::fill<int32_t>(dpw_off.p + 1, k, 1, st);
+    gkd::fill<int32_t>(dkk.p, k, 1, st);
+    gkd::fill<long long>(dcap.p, static_cast<long long>(cap), 1, st);
+    constexpr int kRoundsPerCheck = 4;
     for (int32_t l = nl - 2; l >= 0; --l) {
       Level& f = L[l];
       const DGraph& g = f.g;
@@ -368,67 +391,59 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       std::swap(cur_part, oth_part);
       std::swap(cur_bnd, oth_bnd);
       const int lanes = gain_lanes(g);
+      // Rows longer than the lane count go to the heavy-row kernel; without
+      // any, its launches are skipped.
+      int32_t maxdeg = 0;
+      GK_TRY(gkd::launch_max_deg(g, dmax.p, st));
+      GK_TRY(fetch(ctx, st, {{&maxdeg, dmax.p, sizeof maxdeg}}));
+      const bool heavy_rows = maxdeg > lanes;
+      // After projection every gain is 0: no candidate yet.
+      gkd::fill<uint64_t>(cand.p, 0ull, (int64_t(g.n) + 63) / 64 + 1, st);
+      gkd::fill<int32_t>(dvoff.p + 1, g.n, 1, st);
+      gkd::fill<int32_t>(drounds.p, 0, 1, st);
+      gkd::fill<long long>(dmoved.p, 0, 1, st);
       // The projected boundary becomes the first gain worklist.
       gkd::fill<int32_t>(hcnt.p, 0, 2, st);
       GK_STAGE(ctx, GK_ST_PROJECT, 4LL * g.n, gkd::launch_compact(g.n, upd.p, wl.p, hcnt.p, st));
       GK_STAGE(ctx, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
-                                   work.p, g.n, nullptr, st));
-      GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
-               gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, nullptr,
-                                       st));
-      // A round updates at most B movers and their neighbours.
-      const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
-      const int idbits = bits_for(g.n > 0 ? g.n - 1 : 0);
-      int64_t moved = 0;
-      int32_t rounds = 0;
-      bool pending = false;  // a round whose pos has not been read yet
-      while (true) {
-        GK_STAGE(ctx, GK_ST_SELECT, 8LL * g.n,
-                 gkd::launch_select(g, cur_bnd, gain.p, dest.p, pw.p, cap, idbits, upd.p, keys.p, ctl.p, work.p,
-                                    0, hcnt.p, st));
-        int32_t hc3[3];
-        GK_TRY(fetch(ctx, st, {{hc3, ctl.p, sizeof hc3}}));
-        if (pending) {
-          if (hc3[2] < 0) { gkh::set_error("balanced prefix is empty"); return GK_ERR_INTERNAL; }
-          moved += hc3[2] + 1;
-          pending = false;
-        }
-        const int32_t count = hc3[0];
-        S.stage_bytes[GK_ST_SELECT] += 8LL * count;  // keys written by the selection
-        if (count == 0) break;
-        ++rounds;
-        bool presorted = false;
-        const uint64_t* kk = keys.p;
-        if (count > 1024) {
-          bool alt = false;
-          const int mp = (idbits + bits_for(hc3[1]) + 7) / 8;
-          GK_STAGE(ctx, GK_ST_MSORT, 24LL * count * mp,
-                   gkd::radix_sort_pairs<uint64_t>(keys.p, nullptr, keys2.p, nullptr, count, 0,
-                                                   idbits + bits_for(hc3[1]), sscr.p, st, &alt)
-                       ? GK_ERR_HIP
-                       : GK_OK);
-          kk = alt ? keys2.p : keys.p;
-          presorted = true;
-        }
-        const int32_t B = std::min(count, M);
-        GK_STAGE(ctx, GK_ST_BALANCE, 24LL * B + 12LL * k,
-                 gkd::launch_balance(g, k, kk, count, M, presorted, idbits, cur_part, dest.p, gain.p, pw.p,
-                                     cap, moves.p, ctl.p, dcut.p, st));
-        GK_STAGE(ctx, GK_ST_APPLY, (24LL + 8LL * (g.nnz / std::max(1, g.n))) * B,
-                 gkd::launch_apply(g, M, moves.p, ctl.p, dest.p, cur_part, upd.p, wl.p, hcnt.p, st));
-        GK_STAGE(ctx, GK_ST_GAIN, 0,
-                 gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
-                                     work.p, int64_t(B) * (1 + 2 * avg_deg), nullptr, st));
+                                   work.p, g.n, cand.p, st));
+      if (heavy_rows)
         GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
-                 gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p, nullptr,
-                                       st));
-        pending = true;
+                 gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p,
+                                        cand.p, st));
+      // A round updates at most M movers and their neighbours.
+      const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
+      const int64_t wl_items = std::min<int64_t>(g.n, int64_t(M) * (1 + 2 * avg_deg));
+      while (true) {
+        for (int r = 0; r < kRoundsPerCheck; ++r) {
+          GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
+                   gkd::launch_select_list(g, nullptr, dvoff.p, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, cand.p,
+                                           list.p, jcount.p, longv.p, longcnt.p, hcnt.p, active.p, st));
+          GK_STAGE(ctx, GK_ST_BALANCE, 0,
+                   gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p,
+                                       pw.p, dcut.p, drounds.p, dmoved.p, list.p, jcount.p, hkey.p, upd.p, wl.p,
+                                       hcnt.p, lastcnt.p, active.p, longcnt.p, st));
+          GK_STAGE(ctx, GK_ST_GAIN, 0,
+                   gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p,
+                                       dest.p, work.p, wl_items, cand.p, st));
+          if (heavy_rows)
+            GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
+                     gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p,
+                                            cand.p, st));
+        }
+        int32_t live = 0;
+        GK_TRY(fetch(ctx, st, {{&live, active.p, sizeof live}}));
+        if (live == 0) break;
       }
-      long long hcut = 0;
-      GK_TRY(fetch(ctx, st, {{&hcut, dcut.p, sizeof hcut}}));
-      ls.rounds = rounds;
-      ls.moves = moved;
+      long long hcut = 0, hmoved = 0;
+      int32_t hrounds = 0;
+      GK_TRY(fetch(ctx, st, {{&hcut, dcut.p, sizeof hcut}, {&hmoved, dmoved.p, sizeof hmoved},
+                             {&hrounds, drounds.p, sizeof hrounds}}));
+      // Moves: 24 bytes each in the round kernel and its apply.
+      S.stage_bytes[GK_ST_BALANCE] += 24LL * hmoved + 8LL * hmoved * avg_deg;
+      ls.rounds = hrounds;
+      ls.moves = hmoved;
       ls.cut_after = hcut;
       if (trace) {
         std::vector<int32_t> h;
@@ -436,11 +451,10 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
         ls.h_part = fnv_host(h);
       }
     }
-    // Data-dependent bytes of gain and select from the device work counters.
+    // Data-dependent bytes of the gain passes from the device work counters.
     unsigned long long wk[8];
     GK_TRY(fetch(ctx, st, {{wk, work.p, sizeof wk}}));
     S.stage_bytes[GK_ST_GAIN] += int64_t(28 * wk[0] + 12 * wk[1]);
-    S.stage_bytes[GK_ST_SELECT] += int64_t(12 * wk[2] + 8 * wk[3] + 20 * wk[4]);
   }
   GK_HIP(hipStreamSynchronize(st));
   const auto t3 = clk::now();

@@ -560,9 +560,10 @@ int Engine::refine_step(bool* worked) {
   const Store& F = *pk;
   const DGraph g = F.graph();
   const int32_t N = g.n;
-  Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, jflag, dmax;
+  Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, jcount, dmax, longv,
+      longcnt;
   Buf<long long> dcap, dcut, dmoved;
-  Buf<uint64_t> bits, cand;
+  Buf<uint64_t> cand;
   Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
@@ -583,16 +584,18 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(wl.alloc(P_, N));
   GK_TRY(heavy.alloc(P_, N));
   GK_TRY(list.alloc(P_, N));
-  GK_TRY(bits.alloc(P_, (int64_t(N) + 63) / 64 + 1));
   GK_TRY(cand.alloc(P_, (int64_t(N) + 63) / 64 + 1));
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
   GK_TRY(drounds.alloc(P_, J));
   GK_TRY(lastcnt.alloc(P_, J));
-  GK_TRY(jflag.alloc(P_, J));
+  GK_TRY(jcount.alloc(P_, J));
   GK_TRY(dmax.alloc(P_, 1));
-  gkd::fill<int32_t>(jflag.p, 0, J, st_);
+  GK_TRY(longv.alloc(P_, N));
+  GK_TRY(longcnt.alloc(P_, 1));
+  gkd::fill<int32_t>(jcount.p, 0, J, st_);
+  gkd::fill<int32_t>(longcnt.p, 0, 1, st_);
   GK_TRY(dmoved.alloc(P_, J));
   gkd::fill<int32_t>(drounds.p, 0, J, st_);
   gkd::fill<long long>(dmoved.p, 0, J, st_);
@@ -629,13 +632,13 @@ int Engine::refine_step(bool* worked) {
   for (int done_rounds = 0; done_rounds < kMaxRounds; done_rounds += kRoundsPerCheck) {
     rounds_ += kRoundsPerCheck;
     for (int r = 0; r < kRoundsPerCheck; ++r) {
-      GK_STAGE(ctx_, GK_ST_SELECT, N / 4,
-               gkd::launch_select_bits(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, cand.p, bits.p,
-                                       hcnt.p, active.p, jflag.p, st_));
-      GK_STAGE(ctx_, GK_ST_BALANCE, N / 8 + 8LL * J,
-               gkd::launch_round_b(J, kmax, F.dv_off.p, bits.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p,
-                                   gain.p, dpw.p, dcut.p, drounds.p, dmoved.p, list.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                   active.p, lastcnt.p, jflag.p, st_));
+      GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
+               gkd::launch_select_list(g, rs.vjob.p, F.dv_off.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, cand.p,
+                                       list.p, jcount.p, longv.p, longcnt.p, hcnt.p, active.p, st_));
+      GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
+               gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
+                                   dpw.p, dcut.p, drounds.p, dmoved.p, list.p, jcount.p, hkey.p, upd.p, wl.p, hcnt.p,
+                                   lastcnt.p, active.p, longcnt.p, st_));
       GK_STAGE(ctx_, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
                                    nullptr, wl_items, cand.p, st_));

@@ -45,6 +45,9 @@
 #ifndef GKIP_TRIALS
 #define GKIP_TRIALS 8
 #endif
+#ifndef GKIP_SPAWN_MIN
+#define GKIP_SPAWN_MIN 200  // smallest subgraph whose trials / halves become pool tasks
+#endif
 
 namespace {
 
@@ -546,7 +549,7 @@ void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* ou
     key_ol[c] = std::max<int64_t>(0, pw0 - m0) + std::max<int64_t>(0, (g.tw - pw0) - m1);
     key_cut[c] = cut2(g, trial[c]);
   };
-  if (ctx.parallel && g.n > 2000) {
+  if (ctx.parallel && g.n > GKIP_SPAWN_MIN) {
     std::vector<std::shared_ptr<ForkJoin::Task>> ts;
     for (int c = 1; c < ncuts; ++c) ts.push_back(ForkJoin::get().spawn([&, c] { run_trial(c); }));
     run_trial(0);
@@ -565,7 +568,7 @@ void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* ou
   for (size_t i = 0; i < mb0.size(); ++i) ids0[i] = ids[mb0[i]];
   for (size_t i = 0; i < mb1.size(); ++i) ids1[i] = ids[mb1[i]];
   // The two halves are independent: one may run on another thread.
-  if (ctx.parallel && g.n > 2000) {
+  if (ctx.parallel && g.n > GKIP_SPAWN_MIN) {
     auto t = ForkJoin::get().spawn([&] { rb(g0, k0, part0, depth + 1, ctx, out, ids0); });
     rb(g1, k1, part0 + k0, depth + 1, ctx, out, ids1);
     ForkJoin::get().wait(t);
