@@ -365,10 +365,55 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_of
 }
 
 struct RoundLds {
-  BalanceLds b;
-  int32_t hist[256];
+  BalanceLds b;  // b.sk doubles as the 2048-bin radix histogram
+  int32_t ws[40];
   int32_t misc[4];
 };
+
+// Exclusive scan of one int per thread over a kBalThreads block; *tot gets
+// the block total.  ws: >= kBalThreads / 64 + 1 ints of LDS.
+__device__ __forceinline__ int block_excl_scan_1024(int x, int* tot, int32_t* ws) {
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int incl = wave_incl_scan(x);
+  if (lane == 63) ws[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < kBalThreads / 64; ++w) {
+      const int q = ws[w];
+      ws[w] = run;
+      run += q;
+    }
+    ws[kBalThreads / 64] = run;
+  }
+  __syncthreads();
+  const int r = incl - x + ws[wave];
+  *tot = ws[kBalThreads / 64];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t y = __shfl_xor(x, o, 64);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t y = __shfl_xor(x, o, 64);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+// Bits needed to hold values 0..x (at least 1).
+__device__ __forceinline__ int bits_of(uint32_t x) { return x ? 32 - __clz(x) : 1; }
 
 // Move key of a selected vertex: ascending order = gain descending, id
 // ascending (the mvRequest order of gkway/move_request.hpp:36-41).
@@ -419,44 +464,80 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   if (count <= M) {
     R.b.sk[t] = t < count ? move_key(gain, seg[t]) : ~0ull;
   } else {
-    // Gain keys gathered once into the job's slice of hkey, then streamed.
+    // Exact top-B by radix select on a compressed key: (gain key - its
+    // minimum) above (id - first id of the job); the key is monotone in
+    // (gain desc, id asc) and has only as many bits as the ranges need.
+    // Pass 0 gathers the gain keys into the job's slice of hkey and finds
+    // their range; the passes then use 11-bit digits from the top.
     uint32_t* hk = hkey + vb;
-    for (int32_t i = t; i < count; i += kBalThreads) hk[i] = uint32_t(0x7fffffff - gain[seg[i]]);
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (int32_t i = t; i < count; i += kBalThreads) {
+      const uint32_t h = uint32_t(0x7fffffff - gain[seg[i]]);
+      hk[i] = h;
+      lo = h < lo ? h : lo;
+      hi = h > hi ? h : hi;
+    }
+    lo = wave_min_u32(lo);
+    hi = wave_max_u32(hi);
+    if (lane_id() == 0) {
+      R.ws[t >> 6] = int32_t(lo);
+      R.ws[16 + (t >> 6)] = int32_t(hi);
+    }
     __syncthreads();
+    if (t == 0) {
+      uint32_t a2 = 0xffffffffu, b2 = 0;
+      for (int w = 0; w < kBalThreads / 64; ++w) {
+        a2 = uint32_t(R.ws[w]) < a2 ? uint32_t(R.ws[w]) : a2;
+        b2 = uint32_t(R.ws[16 + w]) > b2 ? uint32_t(R.ws[16 + w]) : b2;
+      }
+      R.ws[32] = int32_t(a2);
+      R.ws[33] = int32_t(b2);
+    }
+    __syncthreads();
+    const uint32_t hmin = uint32_t(R.ws[32]), hrange = uint32_t(R.ws[33]) - hmin;
+    const int ib = bits_of(uint32_t(v_off[j + 1] - vb - 1));
+    const int tb = bits_of(hrange) + ib;
+    auto ckey = [&](int32_t i) -> uint64_t {
+      return (uint64_t(hk[i] - hmin) << ib) | uint64_t(uint32_t(seg[i] - vb));
+    };
+    constexpr int kDigit = 11;
+    int32_t* hist = reinterpret_cast<int32_t*>(R.b.sk);  // 2048 ints, free until the keys are collected
     uint64_t prefix = 0, pmask = 0;
     int32_t need = B;
-    for (int shift = 56; shift >= 0; shift -= 8) {
-      if (t < 256) R.hist[t] = 0;
+    for (int shift = ((tb + kDigit - 1) / kDigit - 1) * kDigit; shift >= 0; shift -= kDigit) {
+      hist[2 * t] = 0;
+      hist[2 * t + 1] = 0;
       __syncthreads();
       for (int32_t i = t; i < count; i += kBalThreads) {
-        const uint64_t key = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
-        if ((key & pmask) == prefix) atomicAdd(&R.hist[(key >> shift) & 255], 1);
+        const uint64_t key = ckey(i);
+        if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 2047], 1);
       }
       __syncthreads();
-      if (t == 0) {
-        int32_t cum = 0, d = 0;
-        for (; d < 255; ++d) {
-          if (cum + R.hist[d] >= need) break;
-          cum += R.hist[d];
-        }
-        R.misc[0] = d;
-        R.misc[1] = cum;
+      // The digit where the running count reaches `need`: a block scan over
+      // the 2048 bins, two per thread.
+      const int32_t h0 = hist[2 * t], h1 = hist[2 * t + 1];
+      int tot;
+      const int32_t ex = block_excl_scan_1024(h0 + h1, &tot, R.ws);
+      if (ex < need && need <= ex + h0) {
+        R.misc[0] = 2 * t;
+        R.misc[1] = ex;
+      } else if (ex + h0 < need && need <= ex + h0 + h1) {
+        R.misc[0] = 2 * t + 1;
+        R.misc[1] = ex + h0;
       }
       __syncthreads();
       need -= R.misc[1];
       prefix |= uint64_t(R.misc[0]) << shift;
-      pmask |= 255ull << shift;
+      pmask |= 2047ull << shift;
       __syncthreads();
     }
-    // prefix is now the B-th smallest key; keys are distinct, so the keys
-    // <= prefix are exactly the B best.
+    // prefix is now the B-th smallest compressed key; keys are distinct, so
+    // the keys <= prefix are exactly the B best.
     if (t == 0) R.misc[2] = 0;
     R.b.sk[t] = ~0ull;
     __syncthreads();
-    for (int32_t i = t; i < count; i += kBalThreads) {
-      const uint64_t key = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
-      if (key <= prefix) R.b.sk[atomicAdd(&R.misc[2], 1)] = key;
-    }
+    for (int32_t i = t; i < count; i += kBalThreads)
+      if (ckey(i) <= prefix) R.b.sk[atomicAdd(&R.misc[2], 1)] = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
   }
   __syncthreads();
   // 3. sort and balanced prefix
