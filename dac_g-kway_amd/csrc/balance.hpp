@@ -2,12 +2,15 @@
 // (find_mv_wgt_sequence + scan + find_balance_sequence, gkway/uncoarsen.hpp:
 // 239-288, 37-80, plus the pw/cut update of apply_sequence_move, :305-316).
 //
-// Instead of the reference's dense k x B matrix of weight deltas, each wave
-// owns a set of parts and runs a wave64 prefix scan of that part's delta
-// along the move sequence; positions where the part's weight would rise above
-// the cap are OR-ed into a bitmap, and the answer is the highest clear bit.
-// Shared by the single-graph kernel (refine.hip) and the batched kernel with
-// one workgroup per graph (batch.hip).
+// Instead of the reference's dense k x B matrix of weight deltas, each move
+// gives two events, (source part, position, -w) and (destination part,
+// position, +w).  The 2B events are sorted by (part, position) in LDS and a
+// segmented prefix sum gives each part's weight change after each of its
+// events.  A part stays above the cap from such an event up to its next
+// event; those position intervals are marked in a difference array, and the
+// answer is the last position no interval covers.  The cost is O(B log B),
+// independent of k.  Shared by the single-graph kernel (refine.hip) and the
+// batched kernel with one workgroup per graph (batch.hip).
 #pragma once
 #include "dev_common.hpp"
 
@@ -17,96 +20,220 @@ constexpr int kBalThreads = 1024;
 constexpr int kBalMax = 1024;
 
 struct BalanceLds {
-  uint64_t sk[kBalMax];
-  int32_t sv[kBalMax], ssrc[kBalMax], sdst[kBalMax], sw[kBalMax], sg[kBalMax];
-  unsigned long long bad[kBalMax / 64];
-  int32_t spos;
+  uint64_t sk[kBalMax];  // move keys; then the 2 * kBalMax event keys (uint32)
+  int32_t sv[kBalMax], sw[kBalMax];
+  int32_t cover[kBalMax + 1];
+  int32_t ws[kBalThreads / 64 + 1];
+  int32_t wmax[kBalThreads / 64];
   long long sgain[kBalThreads / 64];
 };
 
-// Ascending bitonic sort of L.sk[0, P), P = next_pow2(count) (<= kBalMax);
-// slots in [count, P) must hold the maximum key.  All threads of the block
-// call it.
-__device__ inline void lds_bitonic(BalanceLds& L, int32_t count) {
+// Exclusive scan of one int per thread over a kBalThreads block; *tot gets
+// the block total.  ws: >= kBalThreads / 64 + 1 ints of LDS.
+__device__ __forceinline__ int block_excl_scan_1024(int x, int* tot, int32_t* ws) {
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int incl = wave_incl_scan(x);
+  if (lane == 63) ws[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < kBalThreads / 64; ++w) {
+      const int q = ws[w];
+      ws[w] = run;
+      run += q;
+    }
+    ws[kBalThreads / 64] = run;
+  }
+  __syncthreads();
+  const int r = incl - x + ws[wave];
+  *tot = ws[kBalThreads / 64];
+  __syncthreads();
+  return r;
+}
+
+// Ascending bitonic sort of a[0, P) in LDS, P a power of two in
+// [128, 2 * kBalThreads]; all kBalThreads threads call it, after a barrier
+// that follows the writes of a[].  Wave w < P / 128 keeps a[128w, 128w + 128)
+// in registers (lane l: elements 128w + l and 128w + l + 64): strides up to 64
+// run in registers and wave shuffles, larger strides through LDS, so only the
+// stages with stride >= 128 need a barrier.  Ends with a barrier.
+template <typename T>
+__device__ inline void bitonic_lds(T* a, int P) {
   const int t = threadIdx.x;
-  int P = 2;
-  while (P < count) P <<= 1;
-  for (int k2 = 2; k2 <= P; k2 <<= 1) {
-    for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
-      const int ix = t ^ j2;
-      if (t < P && ix > t) {
-        const uint64_t a = L.sk[t], b = L.sk[ix];
-        const bool up = (t & k2) == 0;
-        if (up ? a > b : a < b) { L.sk[t] = b; L.sk[ix] = a; }
+  const int lane = lane_id();
+  const int wave = t >> 6;
+  const bool own = wave < (P >> 7);
+  const int i0 = (wave << 7) + lane;
+  const int i1 = i0 + 64;
+  T x0 = 0, x1 = 0;
+  if (own) {
+    x0 = a[i0];
+    x1 = a[i1];
+  }
+  // Strides j2 = jstart .. 1 of merge size k2 on the wave's registers.
+  auto inwave = [&](int k2, int jstart) {
+    if (!own) return;
+    for (int j2 = jstart; j2 > 0; j2 >>= 1) {
+      if (j2 == 64) {
+        // i0 and i1 = i0 + 64 are partners; k2 >= 128 gives both the same direction
+        const bool up = (i0 & k2) == 0;
+        const T lo = x0 < x1 ? x0 : x1;
+        const T hi = x0 < x1 ? x1 : x0;
+        x0 = up ? lo : hi;
+        x1 = up ? hi : lo;
+      } else {
+        const T y0 = __shfl_xor(x0, j2, 64);
+        const T y1 = __shfl_xor(x1, j2, 64);
+        const bool lower = (lane & j2) == 0;
+        // the lower element of an ascending pair keeps the minimum
+        const bool mn0 = lower == ((i0 & k2) == 0);
+        const bool mn1 = lower == ((i1 & k2) == 0);
+        x0 = mn0 ? (y0 < x0 ? y0 : x0) : (y0 > x0 ? y0 : x0);
+        x1 = mn1 ? (y1 < x1 ? y1 : x1) : (y1 > x1 ? y1 : x1);
+      }
+    }
+  };
+  for (int k2 = 2; k2 <= 128; k2 <<= 1) inwave(k2, k2 >> 1);
+  for (int k2 = 256; k2 <= P; k2 <<= 1) {
+    if (own) {
+      a[i0] = x0;
+      a[i1] = x1;
+    }
+    __syncthreads();
+    for (int j2 = k2 >> 1; j2 >= 128; j2 >>= 1) {
+      for (int p = t; p < (P >> 1); p += kBalThreads) {
+        const int lo = ((p & ~(j2 - 1)) << 1) | (p & (j2 - 1));
+        const int hi = lo + j2;
+        const T u = a[lo], v = a[hi];
+        const bool up = (lo & k2) == 0;
+        if (up ? u > v : u < v) {
+          a[lo] = v;
+          a[hi] = u;
+        }
       }
       __syncthreads();
     }
+    if (own) {
+      x0 = a[i0];
+      x1 = a[i1];
+    }
+    inwave(k2, 64);
   }
+  if (own) {
+    a[i0] = x0;
+    a[i1] = x1;
+  }
+  __syncthreads();
+}
+
+// Ascending sort of L.sk[0, count) (count <= kBalMax); slots [count, kBalMax)
+// must hold the maximum key.  All threads of the block call it, after a
+// barrier that follows the writes of L.sk.
+__device__ inline void lds_bitonic(BalanceLds& L, int32_t count) {
+  int P = 128;
+  while (P < count) P <<= 1;
+  bitonic_lds(L.sk, P);
 }
 
 // Largest balanced prefix of the B sorted moves in L.sk (vertex id in the
-// bits of idmask).  dpw: k ints of LDS.  Fills L.sv/ssrc/sdst/sw/sg, writes
-// the prefix end to *pos_out (-1 only for B == 0) and, when moves != null,
-// the moved vertices to moves[0..pos]; advances pw and subtracts the
-// prefix's gains from *cut.  Returns the prefix end to every thread.
+// bits of idmask): the last position pos such that, after moves 0..pos, no
+// part whose weight grew is above cap.  dpw: k ints of LDS (k parts, k <=
+// 8192).  Fills L.sv, writes pos to *pos_out (-1 when no position
+// qualifies) and, when moves != null, the moved vertices to moves[0..pos];
+// advances pw and subtracts the prefix's gains from *cut.  Returns pos to
+// every thread.
 __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int32_t B, uint64_t idmask,
                                      const int32_t* __restrict__ part, const int32_t* __restrict__ dest,
                                      const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
                                      int32_t* __restrict__ pw, long long cap, int32_t* __restrict__ moves,
                                      int32_t* __restrict__ pos_out, long long* __restrict__ cut) {
+  (void)k;
   const int t = threadIdx.x;
-  if (t < kBalMax / 64) L.bad[t] = 0;
-  for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
+  const int lane = lane_id();
+  const int wave = t >> 6;
+  // 1. event keys (part << 11 | position << 1 | kind), kind 0 = leave, 1 = enter
+  int32_t g = 0;
+  uint32_t e0 = ~0u, e1 = ~0u;
   if (t < B) {
     const int32_t v = int32_t(L.sk[t] & idmask);
     L.sv[t] = v;
-    L.ssrc[t] = part[v];
-    L.sdst[t] = dest[v];
     L.sw[t] = vw[v];
-    L.sg[t] = gain[v];
+    g = gain[v];
+    e0 = (uint32_t(part[v]) << 11) | (uint32_t(t) << 1);
+    e1 = (uint32_t(dest[v]) << 11) | (uint32_t(t) << 1) | 1u;
+  }
+  L.cover[t] = 0;
+  if (t == 0) L.cover[kBalMax] = 0;
+  __syncthreads();
+  uint32_t* ev = reinterpret_cast<uint32_t*>(L.sk);
+  ev[2 * t] = e0;
+  ev[2 * t + 1] = e1;
+  __syncthreads();
+  int P = 128;
+  while (P < 2 * B) P <<= 1;
+  bitonic_lds(ev, P);
+  // 2. thread t holds sorted events 2t and 2t + 1; per-part inclusive sums
+  int32_t qa = 0, qb = 0, ja = 0, jb = 0, xa = 0, xb = 0;
+  if (t < B) {
+    const uint32_t ea = ev[2 * t], eb = ev[2 * t + 1];
+    qa = int32_t(ea >> 11);
+    qb = int32_t(eb >> 11);
+    ja = int32_t((ea >> 1) & 1023u);
+    jb = int32_t((eb >> 1) & 1023u);
+    xa = (ea & 1u) ? L.sw[ja] : -L.sw[ja];
+    xb = (eb & 1u) ? L.sw[jb] : -L.sw[jb];
+  }
+  int tot;
+  const int ex = block_excl_scan_1024(xa + xb, &tot, L.ws);
+  const int pa = ex + xa, pb = pa + xb;
+  if (t < B) {
+    if (t == 0 || int32_t(ev[2 * t - 1] >> 11) != qa) dpw[qa] = ex;  // first event of part qa
+    if (qb != qa) dpw[qb] = pa;
   }
   __syncthreads();
-  const int lane = lane_id();
-  const int wave = t >> 6;
-  const int nchunk = (B + 63) >> 6;
-  for (int32_t q = wave; q < k; q += kBalThreads / 64) {
-    const long long pq = pw[q];
-    long long carry = 0;
-    for (int c = 0; c < nchunk; ++c) {
-      const int j = c * 64 + lane;
-      int x = 0;
-      if (j < B) {
-        if (L.sdst[j] == q) x += L.sw[j];
-        if (L.ssrc[j] == q) x -= L.sw[j];
-      }
-      const long long incl = carry + wave_incl_scan(x);
-      const bool b = j < B && incl > 0 && pq + incl > cap;
-      const uint64_t m = ballot(b);
-      if (m != 0 && lane == 0) atomicOr(&L.bad[c], (unsigned long long)m);
-      carry = __shfl(incl, 63, 64);
+  // 3. part q is above the cap from its event at j up to its next event
+  int32_t sa = 0, sb = 0, na = B, nb = B;
+  if (t < B) {
+    sa = pa - dpw[qa];
+    sb = pb - dpw[qb];
+    if (qb == qa) na = jb;
+    if (t + 1 < B) {
+      const uint32_t ec = ev[2 * t + 2];
+      if (int32_t(ec >> 11) == qb) nb = int32_t((ec >> 1) & 1023u);
+    }
+    if (ja < na && sa > 0 && (long long)pw[qa] + sa > cap) {
+      atomicAdd(&L.cover[ja], 1);
+      atomicAdd(&L.cover[na], -1);
+    }
+    if (jb < nb && sb > 0 && (long long)pw[qb] + sb > cap) {
+      atomicAdd(&L.cover[jb], 1);
+      atomicAdd(&L.cover[nb], -1);
     }
   }
   __syncthreads();
-  if (t == 0) {
-    int pos = -1;
-    for (int i = B - 1; i >= 0; --i) {
-      if (((L.bad[i >> 6] >> (i & 63)) & 1ull) == 0) { pos = i; break; }
-    }
-    L.spos = pos;
-  }
+  // 4. pos = last position no interval covers
+  const int c = t < B ? L.cover[t] : 0;
+  const int cov = block_excl_scan_1024(c, &tot, L.ws) + c;
+  const int cand = wave_max((t < B && cov == 0) ? t : -1);
+  if (lane == 0) L.wmax[wave] = cand;
   __syncthreads();
-  const int pos = L.spos;
+  int pos = -1;
+#pragma unroll
+  for (int w2 = 0; w2 < kBalThreads / 64; ++w2) pos = max(pos, L.wmax[w2]);
+  // 5. part weights after the prefix: the last event of each part at or before pos
+  if (t < B) {
+    if (ja <= pos && na > pos) pw[qa] += sa;
+    if (jb <= pos && nb > pos) pw[qb] += sb;
+  }
   long long gsum = 0;
   if (t <= pos) {
-    atomicAdd(&dpw[L.ssrc[t]], -L.sw[t]);
-    atomicAdd(&dpw[L.sdst[t]], L.sw[t]);
     if (moves) moves[t] = L.sv[t];
-    gsum = L.sg[t];
+    gsum = g;
   }
   gsum = wave_sum64(gsum);
   if (lane == 0) L.sgain[wave] = gsum;
   __syncthreads();
-  for (int32_t q = t; q < k; q += kBalThreads) pw[q] += dpw[q];
   if (t == 0) {
     long long s = 0;
     for (int w2 = 0; w2 < kBalThreads / 64; ++w2) s += L.sgain[w2];

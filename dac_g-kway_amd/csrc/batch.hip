@@ -370,30 +370,6 @@ struct RoundLds {
   int32_t misc[4];
 };
 
-// Exclusive scan of one int per thread over a kBalThreads block; *tot gets
-// the block total.  ws: >= kBalThreads / 64 + 1 ints of LDS.
-__device__ __forceinline__ int block_excl_scan_1024(int x, int* tot, int32_t* ws) {
-  const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
-  const int incl = wave_incl_scan(x);
-  if (lane == 63) ws[wave] = incl;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int w = 0; w < kBalThreads / 64; ++w) {
-      const int q = ws[w];
-      ws[w] = run;
-      run += q;
-    }
-    ws[kBalThreads / 64] = run;
-  }
-  __syncthreads();
-  const int r = incl - x + ws[wave];
-  *tot = ws[kBalThreads / 64];
-  __syncthreads();
-  return r;
-}
-
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
