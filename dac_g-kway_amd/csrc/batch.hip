@@ -219,8 +219,8 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // Selection of a refinement round: create_independent_move_buffer
 // (gkway/uncoarsen.hpp:111-173) with the cap and part weights of the vertex's
 // job, over the candidate bitmask kept by the gain kernels (boundary vertices
-// with positive gain): one wave per 64-vertex word, so vertices outside the
-// mask cost one shared 8-byte load.  Selected vertices are appended, in no
+// with positive gain), so vertices outside the mask cost one bit of a
+// coalesced load.  Selected vertices are appended, in no
 // particular order, to their job's segment of list (list + v_off[job],
 // jcount[job] entries; jcount zero before the launch).  Thread 0 also clears
 // the worklist counters and the round's move counter, which only the round
@@ -260,6 +260,29 @@ __device__ __forceinline__ void append_selected(bool sel, int32_t j, int32_t v, 
 // Rows longer than this are checked by k_select_long, one wave per vertex.
 constexpr int32_t kSelectShortRow = 32;
 
+// Candidate bitmask words read per wave by k_select_list.
+constexpr int kSelectWords = 8;
+
+// Position of the k-th (0-based) set bit of w (k < popcount(w)).
+__device__ __forceinline__ int nth_set_bit(uint64_t w, int k) {
+  int bit = 0;
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) {
+    const int pc = popc64((w >> bit) & ((1ull << s) - 1));
+    if (pc <= k) {
+      k -= pc;
+      bit += s;
+    }
+  }
+  return bit;
+}
+
+// A candidate whose move fits its job's cap is selected when no smaller-id
+// neighbour is also movable.  Each wave reads kSelectWords
+// words of the bitmask with one load and deals their set bits out to its
+// lanes in id order, 64 candidates per pass, so the grid scales with the
+// bitmask and the lanes with the candidates.  Rows longer than
+// kSelectShortRow go to longv for k_select_long.
 __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ vw,
@@ -274,42 +297,72 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
                                                      int32_t* __restrict__ list, int32_t* __restrict__ jcount,
                                                      int32_t* __restrict__ longv, int32_t* __restrict__ longcnt,
                                                      int32_t* __restrict__ reset, int32_t* __restrict__ active) {
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (v == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     reset[0] = 0;
     reset[1] = 0;
     active[0] = 0;
   }
-  const bool valid = v < n;
-  const uint64_t cw = valid ? cand[v >> 6] : 0;  // one word per wave
-  bool sel = false, lng = false;
-  int32_t j = 0;
-  if ((cw >> (v & 63)) & 1ull) {
-    j = vjob ? vjob[v] : 0;
-    const int32_t* jpw = pw + pw_off[j];
-    const long long jc = cap[j];
-    if ((long long)jpw[dest[v]] + vw[v] <= jc) {
-      const int32_t s = xadj[v], e = xadj[v + 1];
-      if (e - s > kSelectShortRow) {
-        lng = true;
-      } else {
-        sel = true;
-        for (int32_t x = s; x < e; ++x) {
-          const int32_t u = adj[x];
-          if (u < v && gain[u] > 0 && (long long)jpw[dest[u]] + vw[u] <= jc) { sel = false; break; }
+  const int lane = lane_id();
+  const int64_t nwords = (int64_t(n) + 63) >> 6;
+  const int64_t w0 = ((int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6) * kSelectWords;
+  uint64_t word = 0;
+  if (lane < kSelectWords && w0 + lane < nwords) word = cand[w0 + lane];
+  const int incl = wave_incl_scan(popc64(word));  // candidates in words w0 .. w0 + lane
+  const int total = __shfl(incl, kSelectWords - 1, 64);
+  for (int base = 0; base < total; base += 64) {
+    const int r = base + lane;  // this lane's candidate rank in the wave
+    // word holding rank r: the number of words whose running count is <= r
+    int wi = 0;
+#pragma unroll
+    for (int s = kSelectWords / 2; s >= 1; s >>= 1) {
+      const int x = __shfl(incl, wi + s - 1, 64);
+      if (x <= r) wi += s;
+    }
+    const int before = __shfl(incl, wi > 0 ? wi - 1 : 0, 64);
+    const uint64_t wd = __shfl(word, wi < kSelectWords ? wi : 0, 64);
+    bool sel = false, lng = false;
+    int32_t j = 0, v = 0;
+    if (r < total) {
+      v = int32_t(((w0 + wi) << 6) + nth_set_bit(wd, r - (wi > 0 ? before : 0)));
+      j = vjob ? vjob[v] : 0;
+      const int32_t* jpw = pw + pw_off[j];
+      const long long jc = cap[j];
+      if ((long long)jpw[dest[v]] + vw[v] <= jc) {
+        const int32_t s = xadj[v], e = xadj[v + 1];
+        if (e - s > kSelectShortRow) {
+          lng = true;
+        } else {
+          // four neighbours per step with independent loads (adj, then
+          // gain/dest/vw, then the part weight)
+          sel = true;
+          for (int32_t x = s; x < e && sel; x += 4) {
+            int32_t u[4], gu[4], du[4], wu[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[i] = x + i < e ? adj[x + i] : -1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const bool lo = u[i] >= 0 && u[i] < v;
+              gu[i] = lo ? gain[u[i]] : 0;
+              du[i] = lo ? dest[u[i]] : 0;
+              wu[i] = lo ? vw[u[i]] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (gu[i] > 0 && (long long)jpw[du[i]] + wu[i] <= jc) sel = false;
+          }
         }
       }
     }
+    const uint64_t lm = ballot(lng);
+    if (lm) {
+      const int leader = __ffsll(static_cast<long long>(lm)) - 1;
+      int lb = 0;
+      if (lane == leader) lb = atomicAdd(longcnt, popc64(lm));
+      lb = __shfl(lb, leader, 64);
+      if (lng) longv[lb + popc64(lm & lanemask_lt())] = v;
+    }
+    append_selected(sel, j, v, v_off, list, jcount);
   }
-  const uint64_t lm = ballot(lng);
-  if (lm) {
-    const int leader = __ffsll(static_cast<long long>(lm)) - 1;
-    int base = 0;
-    if (lane_id() == leader) base = atomicAdd(longcnt, popc64(lm));
-    base = __shfl(base, leader, 64);
-    if (lng) longv[base + popc64(lm & lanemask_lt())] = int32_t(v);
-  }
-  append_selected(sel, j, int32_t(v), v_off, list, jcount);
 }
 
 // Movable vertices with rows longer than kSelectShortRow: one wave checks 64
@@ -355,7 +408,8 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_of
                        const uint64_t* cand, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
                        int32_t* reset, int32_t* active, hipStream_t st) {
   if (g.n <= 0) return 0;
-  k_select_list<<<grid_for(g.n, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, vjob, v_off, gain, dest, pw, pw_off, cap,
+  const int64_t waves = ((int64_t(g.n) + 63) / 64 + kSelectWords - 1) / kSelectWords;
+  k_select_list<<<grid_for(waves * 64, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, vjob, v_off, gain, dest, pw, pw_off, cap,
                                                     cand, list, jcount, longv, longcnt, reset, active);
   unsigned grid = grid_for(g.n, 256);
   if (grid > 1024) grid = 1024;
@@ -520,36 +574,65 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   lds_bitonic(R.b, B);
   const int pos = balance_prefix(R.b, dpw, kk[j], B, 0xffffffffull, part, dest, gain, vw, pw + pw_off[j], cap[j],
                                  nullptr, nullptr, &cut[j]);
-  // 4. apply moves 0..pos (balance_prefix read part[] before its last barrier)
+  // 4. apply moves 0..pos (balance_prefix read part[] before its last barrier):
+  // the movers and their neighbours are items [0, tot) -- mover m owns
+  // [istart[m], istart[m] + deg + 1), first itself, then its row -- dealt out
+  // to the block in passes of kApplyPer items per thread.  An item's vertex
+  // joins the gain worklist if its update flag was clear; the winners of a
+  // pass are appended with one global atomic for the whole block.
   const bool mv = t <= pos;
-  const int32_t v = mv ? R.b.sv[t] : 0;
-  int32_t e = 0, es = 0, deg = 0;
+  int32_t es = 0, deg = 0;
   if (mv) {
+    const int32_t v = R.b.sv[t];
     part[v] = dest[v];
     es = xadj[v];
     deg = xadj[v + 1] - es;
   }
-  bool act = mv;
-  e = -1;  // -1: the mover itself, then its neighbours
-  while (ballot(act) != 0) {
-    int32_t u = 0;
-    bool app = false;
-    if (act) {
-      u = e < 0 ? v : adj[es + e];
-      app = atomicExch(&upd[u], 1) == 0;
+  int tot;
+  const int ib = block_excl_scan_1024(mv ? deg + 1 : 0, &tot, R.ws);
+  int32_t* istart = R.b.sw;     // free after balance_prefix
+  int32_t* rstart = R.b.cover;  // likewise
+  if (mv) {
+    istart[t] = ib;
+    rstart[t] = es;
+  }
+  __syncthreads();
+  constexpr int kApplyPer = 8;
+  for (int pb = 0; pb < tot; pb += kApplyPer * kBalThreads) {
+    const int a0 = pb + kApplyPer * t;
+    int32_t us[kApplyPer];
+    uint32_t win = 0;
+    if (a0 < tot) {
+      int lo = 0, hi = pos;  // last mover m with istart[m] <= a0
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (istart[mid] <= a0) lo = mid; else hi = mid - 1;
+      }
+      int m = lo;
+#pragma unroll
+      for (int i = 0; i < kApplyPer; ++i) {
+        const int e2 = a0 + i;
+        us[i] = -1;
+        if (e2 < tot) {
+          while (m < pos && istart[m + 1] <= e2) ++m;
+          const int off = e2 - istart[m];
+          us[i] = off == 0 ? R.b.sv[m] : adj[rstart[m] + off - 1];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kApplyPer; ++i)
+        if (us[i] >= 0 && atomicExch(&upd[us[i]], 1) == 0) win |= 1u << i;
     }
-    const uint64_t am = ballot(app);
-    if (am) {
-      const int leader = __ffsll(static_cast<long long>(am)) - 1;
-      int wbase = 0;
-      if (lane_id() == leader) wbase = atomicAdd(&wlc[0], popc64(am));
-      wbase = __shfl(wbase, leader, 64);
-      if (app) wl[wbase + popc64(am & lanemask_lt())] = u;
-    }
-    if (act) {
-      ++e;
-      act = e < deg;
-    }
+    int wtot;
+    const int wex = block_excl_scan_1024(__popc(win), &wtot, R.ws);
+    if (t == 0 && wtot > 0) R.misc[0] = atomicAdd(&wlc[0], wtot);
+    __syncthreads();
+    const int wbase = R.misc[0] + wex;
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i)
+      if ((win >> i) & 1u) wl[wbase + r++] = us[i];
+    __syncthreads();  // R.misc[0] is reused by the next pass
   }
   if (t == 0) {
     rounds[j] += 1;
