@@ -2,15 +2,19 @@
 // (find_mv_wgt_sequence + scan + find_balance_sequence, gkway/uncoarsen.hpp:
 // 239-288, 37-80, plus the pw/cut update of apply_sequence_move, :305-316).
 //
-// Instead of the reference's dense k x B matrix of weight deltas, each move
-// gives two events, (source part, position, -w) and (destination part,
-// position, +w).  The 2B events are sorted by (part, position) in LDS and a
-// segmented prefix sum gives each part's weight change after each of its
-// events.  A part stays above the cap from such an event up to its next
-// event; those position intervals are marked in a difference array, and the
-// answer is the last position no interval covers.  The cost is O(B log B),
-// independent of k.  Shared by the single-graph kernel (refine.hip) and the
-// batched kernel with one workgroup per graph (batch.hip).
+// Instead of the reference's dense k x B matrix of weight deltas: a part can
+// only rise above the cap inside the sequence if its weight plus everything
+// the sequence moves into it exceeds the cap.  Only such parts ("risky"; few
+// once the partition is near balance) are examined.  Each move gives events
+// (source part, position, -w) and (destination part, position, +w); the
+// events of risky parts are sorted by (part, position) in LDS and a segmented
+// prefix sum gives each part's weight change after each of its events.  A
+// part stays above the cap from such an event up to its next event; those
+// position intervals are marked in a difference array, and the answer is the
+// last position no interval covers.  The cost is O(B log B), independent of
+// k apart from two passes over the part weights.  Shared by the single-graph
+// kernel (refine.hip) and the batched kernel with one workgroup per graph
+// (batch.hip).
 #pragma once
 #include "dev_common.hpp"
 
@@ -19,12 +23,16 @@ namespace gkd {
 constexpr int kBalThreads = 1024;
 constexpr int kBalMax = 1024;
 
+constexpr int kBalMaxK = 8192;  // parts
+
 struct BalanceLds {
   uint64_t sk[kBalMax];  // move keys; then the 2 * kBalMax event keys (uint32)
   int32_t sv[kBalMax], sw[kBalMax];
   int32_t cover[kBalMax + 1];
+  unsigned long long risk[kBalMaxK / 64];
   int32_t ws[kBalThreads / 64 + 1];
   int32_t wmax[kBalThreads / 64];
+  int32_t nev;
   long long sgain[kBalThreads / 64];
 };
 
@@ -139,7 +147,7 @@ __device__ inline void lds_bitonic(BalanceLds& L, int32_t count) {
 // Largest balanced prefix of the B sorted moves in L.sk (vertex id in the
 // bits of idmask): the last position pos such that, after moves 0..pos, no
 // part whose weight grew is above cap.  dpw: k ints of LDS (k parts, k <=
-// 8192).  Fills L.sv, writes pos to *pos_out (-1 when no position
+// kBalMaxK).  Fills L.sv, writes pos to *pos_out (-1 when no position
 // qualifies) and, when moves != null, the moved vertices to moves[0..pos];
 // advances pw and subtracts the prefix's gains from *cut.  Returns pos to
 // every thread.
@@ -148,63 +156,95 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
                                      const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
                                      int32_t* __restrict__ pw, long long cap, int32_t* __restrict__ moves,
                                      int32_t* __restrict__ pos_out, long long* __restrict__ cut) {
-  (void)k;
   const int t = threadIdx.x;
   const int lane = lane_id();
   const int wave = t >> 6;
-  // 1. event keys (part << 11 | position << 1 | kind), kind 0 = leave, 1 = enter
-  int32_t g = 0;
-  uint32_t e0 = ~0u, e1 = ~0u;
+  // 1. the moves, and each part's inflow over the whole sequence
+  for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
+  for (int32_t i = t; i < (k + 63) / 64; i += kBalThreads) L.risk[i] = 0;
+  if (t == 0) {
+    L.nev = 0;
+    L.cover[kBalMax] = 0;
+  }
+  L.cover[t] = 0;
+  int32_t g = 0, src = 0, dst = 0, w = 0;
   if (t < B) {
     const int32_t v = int32_t(L.sk[t] & idmask);
     L.sv[t] = v;
-    L.sw[t] = vw[v];
+    w = vw[v];
+    L.sw[t] = w;
     g = gain[v];
-    e0 = (uint32_t(part[v]) << 11) | (uint32_t(t) << 1);
-    e1 = (uint32_t(dest[v]) << 11) | (uint32_t(t) << 1) | 1u;
+    src = part[v];
+    dst = dest[v];
   }
-  L.cover[t] = 0;
-  if (t == 0) L.cover[kBalMax] = 0;
   __syncthreads();
+  if (t < B) atomicAdd(&dpw[dst], w);
+  __syncthreads();
+  // 2. risky parts: pw + inflow > cap (the only ones that can be above the
+  // cap at some position with a positive change)
+  for (int32_t q = t; q < k; q += kBalThreads) {
+    const int32_t in = dpw[q];
+    if (in > 0 && (long long)pw[q] + in > cap) atomicOr(&L.risk[q >> 6], 1ull << (q & 63));
+    dpw[q] = 0;
+  }
+  __syncthreads();
+  // 3. event keys of the risky parts (part << 11 | position << 1 | kind),
+  // kind 0 = leave, 1 = enter; slots in any order, sorted next
   uint32_t* ev = reinterpret_cast<uint32_t*>(L.sk);
-  ev[2 * t] = e0;
-  ev[2 * t + 1] = e1;
-  __syncthreads();
-  int P = 128;
-  while (P < 2 * B) P <<= 1;
-  bitonic_lds(ev, P);
-  // 2. thread t holds sorted events 2t and 2t + 1; per-part inclusive sums
-  int32_t qa = 0, qb = 0, ja = 0, jb = 0, xa = 0, xb = 0;
   if (t < B) {
-    const uint32_t ea = ev[2 * t], eb = ev[2 * t + 1];
+    const bool rs = (L.risk[src >> 6] >> (src & 63)) & 1ull;
+    const bool rd = (L.risk[dst >> 6] >> (dst & 63)) & 1ull;
+    if (rs || rd) {
+      int slot = atomicAdd(&L.nev, int(rs) + int(rd));
+      if (rs) ev[slot++] = (uint32_t(src) << 11) | (uint32_t(t) << 1);
+      if (rd) ev[slot] = (uint32_t(dst) << 11) | (uint32_t(t) << 1) | 1u;
+    }
+  }
+  __syncthreads();
+  const int nev = L.nev;
+  int P = 128;
+  while (P < nev) P <<= 1;
+  for (int i = nev + t; i < P; i += kBalThreads) ev[i] = ~0u;
+  __syncthreads();
+  if (nev > 0) bitonic_lds(ev, P);
+  // 4. thread t holds sorted events 2t and 2t + 1; per-part inclusive sums
+  const bool ha = 2 * t < nev, hb = 2 * t + 1 < nev;
+  int32_t qa = 0, qb = -1, ja = 0, jb = 0, xa = 0, xb = 0;
+  if (ha) {
+    const uint32_t ea = ev[2 * t];
     qa = int32_t(ea >> 11);
-    qb = int32_t(eb >> 11);
     ja = int32_t((ea >> 1) & 1023u);
-    jb = int32_t((eb >> 1) & 1023u);
     xa = (ea & 1u) ? L.sw[ja] : -L.sw[ja];
+  }
+  if (hb) {
+    const uint32_t eb = ev[2 * t + 1];
+    qb = int32_t(eb >> 11);
+    jb = int32_t((eb >> 1) & 1023u);
     xb = (eb & 1u) ? L.sw[jb] : -L.sw[jb];
   }
   int tot;
   const int ex = block_excl_scan_1024(xa + xb, &tot, L.ws);
   const int pa = ex + xa, pb = pa + xb;
-  if (t < B) {
-    if (t == 0 || int32_t(ev[2 * t - 1] >> 11) != qa) dpw[qa] = ex;  // first event of part qa
-    if (qb != qa) dpw[qb] = pa;
-  }
+  const bool heada = ha && (t == 0 || int32_t(ev[2 * t - 1] >> 11) != qa);
+  const bool headb = hb && qb != qa;
+  if (heada) dpw[qa] = ex;  // prefix before the part's first event
+  if (headb) dpw[qb] = pa;
   __syncthreads();
-  // 3. part q is above the cap from its event at j up to its next event
-  int32_t sa = 0, sb = 0, na = B, nb = B;
-  if (t < B) {
-    sa = pa - dpw[qa];
-    sb = pb - dpw[qb];
-    if (qb == qa) na = jb;
-    if (t + 1 < B) {
-      const uint32_t ec = ev[2 * t + 2];
-      if (int32_t(ec >> 11) == qb) nb = int32_t((ec >> 1) & 1023u);
-    }
+  // 5. a risky part is above the cap from its event at j up to its next event
+  if (ha) {
+    const int32_t sa = pa - dpw[qa];
+    const int32_t na = (hb && qb == qa) ? jb : B;
     if (ja < na && sa > 0 && (long long)pw[qa] + sa > cap) {
       atomicAdd(&L.cover[ja], 1);
       atomicAdd(&L.cover[na], -1);
+    }
+  }
+  if (hb) {
+    const int32_t sb = pb - dpw[qb];
+    int32_t nb = B;
+    if (2 * t + 2 < nev) {
+      const uint32_t ec = ev[2 * t + 2];
+      if (int32_t(ec >> 11) == qb) nb = int32_t((ec >> 1) & 1023u);
     }
     if (jb < nb && sb > 0 && (long long)pw[qb] + sb > cap) {
       atomicAdd(&L.cover[jb], 1);
@@ -212,7 +252,9 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
     }
   }
   __syncthreads();
-  // 4. pos = last position no interval covers
+  if (heada) dpw[qa] = 0;
+  if (headb) dpw[qb] = 0;
+  // 6. pos = last position no interval covers
   const int c = t < B ? L.cover[t] : 0;
   const int cov = block_excl_scan_1024(c, &tot, L.ws) + c;
   const int cand = wave_max((t < B && cov == 0) ? t : -1);
@@ -221,19 +263,21 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   int pos = -1;
 #pragma unroll
   for (int w2 = 0; w2 < kBalThreads / 64; ++w2) pos = max(pos, L.wmax[w2]);
-  // 5. part weights after the prefix: the last event of each part at or before pos
-  if (t < B) {
-    if (ja <= pos && na > pos) pw[qa] += sa;
-    if (jb <= pos && nb > pos) pw[qb] += sb;
-  }
+  // 7. apply the prefix to the part weights, moves and cut
   long long gsum = 0;
   if (t <= pos) {
+    atomicAdd(&dpw[src], -w);
+    atomicAdd(&dpw[dst], w);
     if (moves) moves[t] = L.sv[t];
     gsum = g;
   }
   gsum = wave_sum64(gsum);
   if (lane == 0) L.sgain[wave] = gsum;
   __syncthreads();
+  for (int32_t q = t; q < k; q += kBalThreads) {
+    const int32_t d = dpw[q];
+    if (d != 0) pw[q] += d;
+  }
   if (t == 0) {
     long long s = 0;
     for (int w2 = 0; w2 < kBalThreads / 64; ++w2) s += L.sgain[w2];

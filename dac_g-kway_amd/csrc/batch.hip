@@ -23,14 +23,34 @@ namespace gkd {
 
 static inline int hip_status() { return hipGetLastError() == hipSuccess ? 0 : GK_ERR_HIP; }
 
-// Index j with off[j] <= i < off[j+1] (off is nondecreasing, off[0] = 0).
-__device__ __forceinline__ int32_t job_of(const int32_t* __restrict__ off, int32_t nj, int64_t i) {
-  int32_t lo = 0, hi = nj;  // answer in [lo, hi)
+// Index j in [lo, hi) with off[j] <= i < off[j+1] (off is nondecreasing;
+// the caller knows off[lo] <= i < off[hi]).
+__device__ __forceinline__ int32_t job_in(const int32_t* __restrict__ off, int32_t lo, int32_t hi, int64_t i) {
   while (hi - lo > 1) {
     const int32_t mid = (lo + hi) >> 1;
     if (off[mid] <= i) lo = mid; else hi = mid;
   }
   return lo;
+}
+
+__device__ __forceinline__ int32_t job_of(const int32_t* __restrict__ off, int32_t nj, int64_t i) {
+  return job_in(off, 0, nj, i);
+}
+
+// Descriptor-driven copies: a block covers kPackPer * 256 consecutive items
+// (vertices or entries) and looks up the jobs of its first and last item
+// once; items of a block inside one job (nearly all) then need no per-item
+// search.
+constexpr int kPackPer = 8;
+
+__device__ __forceinline__ void block_jobs(const int32_t* __restrict__ off, int32_t nj, int64_t total, int64_t i0,
+                                           int32_t* sj) {
+  if (threadIdx.x == 0) {
+    const int64_t last = (i0 + int64_t(kPackPer) * 256 < total ? i0 + int64_t(kPackPer) * 256 : total) - 1;
+    sj[0] = job_of(off, nj, i0);
+    sj[1] = job_in(off, sj[0], nj, last);
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------- pack
@@ -40,29 +60,55 @@ __global__ void __launch_bounds__(256) k_pack_vert(int32_t nj, const PackSrc* __
                                                    const int32_t* __restrict__ c_off, int32_t ntot,
                                                    int32_t* __restrict__ xadj, int32_t* __restrict__ vw,
                                                    int32_t* __restrict__ vjob, int32_t* __restrict__ cmap) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= ntot) return;
-  const int32_t j = job_of(v_off, nj, i);
-  const PackSrc s = src[j];
-  const int32_t li = int32_t(i - v_off[j]);
-  xadj[i] = e_off[j] + (s.xadj[li] - s.xbase);
-  vw[i] = s.vw[li];
-  if (vjob) vjob[i] = j;
-  // No source map: the coarse side has the fine level's ids (identity).
-  if (cmap) cmap[i] = c_off[j] + (s.cmap ? s.cmap[li] - s.cbase : li);
+  __shared__ int32_t sj[2];
+  const int64_t i0 = int64_t(blockIdx.x) * kPackPer * 256;
+  block_jobs(v_off, nj, ntot, i0, sj);
+  for (int r = 0; r < kPackPer; ++r) {
+    const int64_t i = i0 + r * 256 + threadIdx.x;
+    if (i >= ntot) break;
+    const int32_t j = sj[0] == sj[1] ? sj[0] : job_in(v_off, sj[0], sj[1] + 1, i);
+    const PackSrc& s = src[j];
+    const int32_t li = int32_t(i - v_off[j]);
+    xadj[i] = e_off[j] + (s.xadj[li] - s.xbase);
+    vw[i] = s.vw[li];
+    if (vjob) vjob[i] = j;
+    // No source map: the coarse side has the fine level's ids (identity).
+    if (cmap) cmap[i] = c_off[j] + (s.cmap ? s.cmap[li] - s.cbase : li);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_pack_edge(int32_t nj, const PackSrc* __restrict__ src,
                                                    const int32_t* __restrict__ v_off,
                                                    const int32_t* __restrict__ e_off, int32_t etot,
                                                    int32_t* __restrict__ adj, int32_t* __restrict__ w) {
-  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (e >= etot) return;
-  const int32_t j = job_of(e_off, nj, e);
-  const PackSrc s = src[j];
-  const int32_t le = int32_t(e - e_off[j]);
-  adj[e] = v_off[j] + (s.adj[le] - s.abase);
-  w[e] = s.w[le];
+  __shared__ int32_t sj[2];
+  const int64_t i0 = int64_t(blockIdx.x) * kPackPer * 256;
+  block_jobs(e_off, nj, etot, i0, sj);
+  if (sj[0] == sj[1]) {
+    const int32_t j = sj[0];
+    const int32_t* __restrict__ sa = src[j].adj;
+    const int32_t* __restrict__ sw = src[j].w;
+    const int32_t shift = v_off[j] - src[j].abase;
+    const int32_t eo = e_off[j];
+#pragma unroll
+    for (int r = 0; r < kPackPer; ++r) {
+      const int64_t e = i0 + r * 256 + threadIdx.x;
+      if (e < etot) {
+        adj[e] = sa[e - eo] + shift;
+        w[e] = sw[e - eo];
+      }
+    }
+    return;
+  }
+  for (int r = 0; r < kPackPer; ++r) {
+    const int64_t e = i0 + r * 256 + threadIdx.x;
+    if (e >= etot) break;
+    const int32_t j = job_in(e_off, sj[0], sj[1] + 1, e);
+    const PackSrc& s = src[j];
+    const int32_t le = int32_t(e - e_off[j]);
+    adj[e] = v_off[j] + (s.adj[le] - s.abase);
+    w[e] = s.w[le];
+  }
 }
 
 // Coarse-side arrays of a refinement pack (partition and boundary flags of
@@ -71,16 +117,21 @@ __global__ void __launch_bounds__(256) k_pack_coarse(int32_t nj, const PackSrc* 
                                                      const int32_t* __restrict__ c_off, int32_t ctot,
                                                      int32_t* __restrict__ cpart, int32_t* __restrict__ cbnd,
                                                      int32_t* __restrict__ cgain, int32_t* __restrict__ cdest) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= ctot) return;
-  const int32_t j = job_of(c_off, nj, i);
-  const int32_t li = int32_t(i - c_off[j]);
-  const PackSrc& s = src[j];
-  cpart[i] = s.cpart[li];
-  cbnd[i] = s.cbnd[li];
-  if (cgain) {
-    cgain[i] = s.cgain ? s.cgain[li] : 0;
-    cdest[i] = s.cgain ? s.cdest[li] : -1;
+  __shared__ int32_t sj[2];
+  const int64_t i0 = int64_t(blockIdx.x) * kPackPer * 256;
+  block_jobs(c_off, nj, ctot, i0, sj);
+  for (int r = 0; r < kPackPer; ++r) {
+    const int64_t i = i0 + r * 256 + threadIdx.x;
+    if (i >= ctot) break;
+    const int32_t j = sj[0] == sj[1] ? sj[0] : job_in(c_off, sj[0], sj[1] + 1, i);
+    const int32_t li = int32_t(i - c_off[j]);
+    const PackSrc& s = src[j];
+    cpart[i] = s.cpart[li];
+    cbnd[i] = s.cbnd[li];
+    if (cgain) {
+      cgain[i] = s.cgain ? s.cgain[li] : 0;
+      cdest[i] = s.cgain ? s.cdest[li] : -1;
+    }
   }
 }
 
@@ -88,12 +139,13 @@ int launch_pack(int32_t nj, const PackSrc* src, const int32_t* v_off, const int3
                 int32_t ntot, int32_t etot, int32_t ctot, int32_t* xadj, int32_t* adj, int32_t* w, int32_t* vw,
                 int32_t* vjob, int32_t* cmap, int32_t* cpart, int32_t* cbnd, int32_t* cgain, int32_t* cdest,
                 hipStream_t st) {
+  constexpr int kTile = kPackPer * 256;
   if (ntot > 0)
-    k_pack_vert<<<grid_for(ntot, 256), 256, 0, st>>>(nj, src, v_off, e_off, c_off, ntot, xadj, vw, vjob, cmap);
+    k_pack_vert<<<grid_for(ntot, kTile), 256, 0, st>>>(nj, src, v_off, e_off, c_off, ntot, xadj, vw, vjob, cmap);
   fill<int32_t>(xadj + ntot, etot, 1, st);
-  if (etot > 0) k_pack_edge<<<grid_for(etot, 256), 256, 0, st>>>(nj, src, v_off, e_off, etot, adj, w);
+  if (etot > 0) k_pack_edge<<<grid_for(etot, kTile), 256, 0, st>>>(nj, src, v_off, e_off, etot, adj, w);
   if (cpart && ctot > 0)
-    k_pack_coarse<<<grid_for(ctot, 256), 256, 0, st>>>(nj, src, c_off, ctot, cpart, cbnd, cgain, cdest);
+    k_pack_coarse<<<grid_for(ctot, kTile), 256, 0, st>>>(nj, src, c_off, ctot, cpart, cbnd, cgain, cdest);
   return hip_status();
 }
 
@@ -470,23 +522,40 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     int32_t* __restrict__ rounds, long long* __restrict__ moved, const int32_t* __restrict__ list,
     int32_t* __restrict__ jcount, uint32_t* __restrict__ hkey, int32_t* __restrict__ upd, int32_t* __restrict__ wl,
     int32_t* __restrict__ wlc, int32_t* __restrict__ lastcnt, int32_t* __restrict__ active,
-    int32_t* __restrict__ longcnt) {
+    int32_t* __restrict__ longcnt, uint64_t* __restrict__ cand) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
   const int32_t j = blockIdx.x;
   if (t == 0) {
-    R.misc[3] = jcount[j];
+    const int32_t c = jcount[j];
+    R.misc[3] = c;
+    R.misc[2] = lastcnt[j];
     jcount[j] = 0;
+    lastcnt[j] = c;
+    if (c) atomicAdd(active, c);
     if (j == 0) *longcnt = 0;  // the selection kernels of this round are done
   }
   __syncthreads();
   const int32_t count = R.misc[3];
-  if (t == 0) {
-    lastcnt[j] = count;
-    if (count) atomicAdd(active, count);
+  if (count == 0) {
+    // Nothing selected: the job's state (partition, gains, part weights) is
+    // unchanged, so every later round of this level selects nothing either.
+    // The first time, its candidate bits are dropped so that the selection
+    // stops re-checking them (with one job the rounds end at the next check
+    // anyway).
+    if (R.misc[2] != 0 && gridDim.x > 1) {
+      const int64_t a = v_off[j], b = v_off[j + 1];
+      for (int64_t w = (a >> 6) + t; b > a && w <= ((b - 1) >> 6); w += kBalThreads) {
+        const int64_t lo = (a > (w << 6) ? a : (w << 6)) - (w << 6);
+        const int64_t hi = (b < (w << 6) + 64 ? b : (w << 6) + 64) - (w << 6);
+        const uint64_t m = (hi - lo == 64) ? ~0ull : (((1ull << (hi - lo)) - 1) << lo);
+        if (m == ~0ull) cand[w] = 0;
+        else atomicAnd(reinterpret_cast<unsigned long long*>(&cand[w]), (unsigned long long)~m);
+      }
+    }
+    return;
   }
-  if (count == 0) return;
   const int32_t vb = v_off[j];
   const int32_t* seg = list + vb;
   const int32_t B = count < M ? count : M;
@@ -644,12 +713,12 @@ int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, co
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
                    int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const int32_t* list,
                    int32_t* jcount, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
-                   int32_t* active, int32_t* longcnt, hipStream_t st) {
+                   int32_t* active, int32_t* longcnt, uint64_t* cand, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
   k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, M, kk, cap, pw_off, g.xadj, g.adj, part, dest,
                                                              gain, g.vw, pw, cut, rounds, moved, list, jcount, hkey,
-                                                             upd, wl, wlc, lastcnt, active, longcnt);
+                                                             upd, wl, wlc, lastcnt, active, longcnt, cand);
   return hip_status();
 }
 

@@ -401,6 +401,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       gkd::fill<uint64_t>(cand.p, 0ull, (int64_t(g.n) + 63) / 64 + 1, st);
       gkd::fill<int32_t>(dvoff.p + 1, g.n, 1, st);
       gkd::fill<int32_t>(drounds.p, 0, 1, st);
+      gkd::fill<int32_t>(lastcnt.p, 1, 1, st);
       gkd::fill<long long>(dmoved.p, 0, 1, st);
       // The projected boundary becomes the first gain worklist.
       gkd::fill<int32_t>(hcnt.p, 0, 2, st);
@@ -423,7 +424,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
           GK_STAGE(ctx, GK_ST_BALANCE, 0,
                    gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p,
                                        pw.p, dcut.p, drounds.p, dmoved.p, list.p, jcount.p, hkey.p, upd.p, wl.p,
-                                       hcnt.p, lastcnt.p, active.p, longcnt.p, st));
+                                       hcnt.p, lastcnt.p, active.p, longcnt.p, cand.p, st));
           GK_STAGE(ctx, GK_ST_GAIN, 0,
                    gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p,
                                        dest.p, work.p, wl_items, cand.p, st));

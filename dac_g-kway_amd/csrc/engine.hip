@@ -590,6 +590,7 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(active.alloc(P_, 1));
   GK_TRY(drounds.alloc(P_, J));
   GK_TRY(lastcnt.alloc(P_, J));
+  gkd::fill<int32_t>(lastcnt.p, 1, J, st_);
   GK_TRY(jcount.alloc(P_, J));
   GK_TRY(dmax.alloc(P_, 1));
   GK_TRY(longv.alloc(P_, N));
@@ -625,9 +626,13 @@ int Engine::refine_step(bool* worked) {
   // move count every kRoundsPerCheck rounds.  A round in which no job selects
   // a move changes nothing, so rounds past convergence are harmless.  After
   // kMaxRounds rounds the step ends; jobs still moving resume their level in
-  // the next step, so one slow job does not hold back the others.
+  // the next step, so one slow job does not hold back the others.  The fixed
+  // cost of a step (pack, projection, candidate set, a host sync) grows with
+  // the pack, so bigger packs run more rounds per step: 16 per 16 Mi
+  // vertices + entries, at most 256.
   constexpr int kRoundsPerCheck = 4;
-  constexpr int kMaxRounds = 16;
+  const int64_t pack_items = int64_t(N) + int64_t(g.nnz);
+  const int kMaxRounds = static_cast<int>(std::min<int64_t>(256, 16 * std::max<int64_t>(1, pack_items >> 24)));
   const double tr0 = now_ms();
   for (int done_rounds = 0; done_rounds < kMaxRounds; done_rounds += kRoundsPerCheck) {
     rounds_ += kRoundsPerCheck;
@@ -638,7 +643,7 @@ int Engine::refine_step(bool* worked) {
       GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
                gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
                                    dpw.p, dcut.p, drounds.p, dmoved.p, list.p, jcount.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                   lastcnt.p, active.p, longcnt.p, st_));
+                                   lastcnt.p, active.p, longcnt.p, cand.p, st_));
       GK_STAGE(ctx_, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
                                    nullptr, wl_items, cand.p, st_));
