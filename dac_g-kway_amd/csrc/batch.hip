@@ -272,11 +272,11 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // (gkway/uncoarsen.hpp:111-173) with the cap and part weights of the vertex's
 // job, over the candidate bitmask kept by the gain kernels (boundary vertices
 // with positive gain), so vertices outside the mask cost one bit of a
-// coalesced load.  Selected vertices are appended, in no
-// particular order, to their job's segment of list (list + v_off[job],
-// jcount[job] entries; jcount zero before the launch).  Thread 0 also clears
-// the worklist counters and the round's move counter, which only the round
-// kernel adds to.  vjob == null: one job.
+// coalesced load.  Two kernels: k_select_fit turns the candidates into the
+// movable set (move fits the cap), k_select_list keeps the movable vertices
+// with no smaller-id movable neighbour.  Selected vertices are appended, in
+// no particular order, to their job's segment of list (list + v_off[job],
+// jcount[job] entries; jcount zero before the launch).  vjob == null: one job.
 // Appends the selected vertices of one wave to their jobs' list segments:
 // the selected lanes form runs of equal job id (ids do not decrease along the
 // wave) and the first lane of each run reserves the run's slots with one
@@ -329,78 +329,108 @@ __device__ __forceinline__ int nth_set_bit(uint64_t w, int k) {
   return bit;
 }
 
-// A candidate whose move fits its job's cap is selected when no smaller-id
-// neighbour is also movable.  Each wave reads kSelectWords
-// words of the bitmask with one load and deals their set bits out to its
-// lanes in id order, 64 candidates per pass, so the grid scales with the
-// bitmask and the lanes with the candidates.  Rows longer than
-// kSelectShortRow go to longv for k_select_long.
-__global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* __restrict__ xadj,
-                                                     const int32_t* __restrict__ adj,
-                                                     const int32_t* __restrict__ vw,
-                                                     const int32_t* __restrict__ vjob,
-                                                     const int32_t* __restrict__ v_off,
-                                                     const int32_t* __restrict__ gain,
-                                                     const int32_t* __restrict__ dest,
-                                                     const int32_t* __restrict__ pw,
-                                                     const int32_t* __restrict__ pw_off,
-                                                     const long long* __restrict__ cap,
-                                                     const uint64_t* __restrict__ cand,
-                                                     int32_t* __restrict__ list, int32_t* __restrict__ jcount,
-                                                     int32_t* __restrict__ longv, int32_t* __restrict__ longcnt,
-                                                     int32_t* __restrict__ reset, int32_t* __restrict__ active) {
+// The wave's bit of rank r among the set bits of its kSelectWords words: lane
+// l holds word l and the running count incl of words 0..l.  Every lane
+// calls it (wave shuffles); the result is meaningful for r < the total.
+__device__ __forceinline__ void deal_bit(int incl, uint64_t word, int r, int* wi_out, int* bit_out) {
+  int wi = 0;
+#pragma unroll
+  for (int s = kSelectWords / 2; s >= 1; s >>= 1) {
+    const int x = __shfl(incl, wi + s - 1, 64);
+    if (x <= r) wi += s;
+  }
+  const int before = __shfl(incl, wi > 0 ? wi - 1 : 0, 64);
+  const uint64_t wd = __shfl(word, wi < kSelectWords ? wi : 0, 64);
+  *wi_out = wi;
+  *bit_out = nth_set_bit(wd, r - (wi > 0 ? before : 0));
+}
+
+// Movable set of a round: the candidates (boundary, positive gain) whose move
+// fits their job's cap, as a bitmask.  Each wave reads kSelectWords words of
+// the candidate mask with one load and deals their set bits out to its lanes,
+// 64 per pass; the movable bits are gathered in LDS and written back for
+// every word.  Thread 0 also clears the worklist counters and the round's
+// move counter, which only the round kernel adds to.
+__global__ void __launch_bounds__(256) k_select_fit(int32_t n, const int32_t* __restrict__ vw,
+                                                    const int32_t* __restrict__ vjob,
+                                                    const int32_t* __restrict__ dest,
+                                                    const int32_t* __restrict__ pw,
+                                                    const int32_t* __restrict__ pw_off,
+                                                    const long long* __restrict__ cap,
+                                                    const uint64_t* __restrict__ cand,
+                                                    uint64_t* __restrict__ movable, int32_t* __restrict__ reset,
+                                                    int32_t* __restrict__ active) {
+  __shared__ unsigned long long mw[4][kSelectWords];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     reset[0] = 0;
     reset[1] = 0;
     active[0] = 0;
   }
   const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  if (lane < kSelectWords) mw[wv][lane] = 0;
+  __syncthreads();
   const int64_t nwords = (int64_t(n) + 63) >> 6;
   const int64_t w0 = ((int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6) * kSelectWords;
   uint64_t word = 0;
   if (lane < kSelectWords && w0 + lane < nwords) word = cand[w0 + lane];
-  const int incl = wave_incl_scan(popc64(word));  // candidates in words w0 .. w0 + lane
+  const int incl = wave_incl_scan(popc64(word));
   const int total = __shfl(incl, kSelectWords - 1, 64);
   for (int base = 0; base < total; base += 64) {
-    const int r = base + lane;  // this lane's candidate rank in the wave
-    // word holding rank r: the number of words whose running count is <= r
-    int wi = 0;
-#pragma unroll
-    for (int s = kSelectWords / 2; s >= 1; s >>= 1) {
-      const int x = __shfl(incl, wi + s - 1, 64);
-      if (x <= r) wi += s;
+    const int r = base + lane;
+    int wi, bit;
+    deal_bit(incl, word, r, &wi, &bit);
+    if (r < total) {
+      const int32_t v = int32_t(((w0 + wi) << 6) + bit);
+      const int32_t j = vjob ? vjob[v] : 0;
+      if ((long long)pw[pw_off[j] + dest[v]] + vw[v] <= cap[j]) atomicOr(&mw[wv][wi], 1ull << bit);
     }
-    const int before = __shfl(incl, wi > 0 ? wi - 1 : 0, 64);
-    const uint64_t wd = __shfl(word, wi < kSelectWords ? wi : 0, 64);
+  }
+  __syncthreads();
+  if (lane < kSelectWords && w0 + lane < nwords) movable[w0 + lane] = mw[wv][lane];
+}
+
+// Selection proper: a movable vertex is selected when no smaller-id
+// neighbour is movable (one bit per neighbour).  Same dealing of the movable
+// mask's bits to lanes as k_select_fit.  Rows longer than kSelectShortRow go
+// to longv for k_select_long.
+__global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* __restrict__ xadj,
+                                                     const int32_t* __restrict__ adj,
+                                                     const int32_t* __restrict__ vjob,
+                                                     const int32_t* __restrict__ v_off,
+                                                     const uint64_t* __restrict__ movable,
+                                                     int32_t* __restrict__ list, int32_t* __restrict__ jcount,
+                                                     int32_t* __restrict__ longv, int32_t* __restrict__ longcnt) {
+  const int lane = lane_id();
+  const int64_t nwords = (int64_t(n) + 63) >> 6;
+  const int64_t w0 = ((int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6) * kSelectWords;
+  uint64_t word = 0;
+  if (lane < kSelectWords && w0 + lane < nwords) word = movable[w0 + lane];
+  const int incl = wave_incl_scan(popc64(word));
+  const int total = __shfl(incl, kSelectWords - 1, 64);
+  for (int base = 0; base < total; base += 64) {
+    const int r = base + lane;
+    int wi, bit;
+    deal_bit(incl, word, r, &wi, &bit);
     bool sel = false, lng = false;
     int32_t j = 0, v = 0;
     if (r < total) {
-      v = int32_t(((w0 + wi) << 6) + nth_set_bit(wd, r - (wi > 0 ? before : 0)));
+      v = int32_t(((w0 + wi) << 6) + bit);
       j = vjob ? vjob[v] : 0;
-      const int32_t* jpw = pw + pw_off[j];
-      const long long jc = cap[j];
-      if ((long long)jpw[dest[v]] + vw[v] <= jc) {
-        const int32_t s = xadj[v], e = xadj[v + 1];
-        if (e - s > kSelectShortRow) {
-          lng = true;
-        } else {
-          // four neighbours per step with independent loads (adj, then
-          // gain/dest/vw, then the part weight)
-          sel = true;
-          for (int32_t x = s; x < e && sel; x += 4) {
-            int32_t u[4], gu[4], du[4], wu[4];
+      const int32_t s = xadj[v], e = xadj[v + 1];
+      if (e - s > kSelectShortRow) {
+        lng = true;
+      } else {
+        // eight neighbours per step: their ids, then their movable bits
+        sel = true;
+        for (int32_t x = s; x < e && sel; x += 8) {
+          int32_t u[8];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) u[i] = x + i < e ? adj[x + i] : -1;
+          for (int i = 0; i < 8; ++i) u[i] = x + i < e ? adj[x + i] : v;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const bool lo = u[i] >= 0 && u[i] < v;
-              gu[i] = lo ? gain[u[i]] : 0;
-              du[i] = lo ? dest[u[i]] : 0;
-              wu[i] = lo ? vw[u[i]] : 0;
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-              if (gu[i] > 0 && (long long)jpw[du[i]] + wu[i] <= jc) sel = false;
+          for (int i = 0; i < 8; ++i) {
+            const uint64_t mwd = movable[u[i] >> 6];
+            if (u[i] < v && ((mwd >> (u[i] & 63)) & 1ull)) sel = false;
           }
         }
       }
@@ -423,14 +453,9 @@ __global__ void __launch_bounds__(256) k_select_long(const int32_t* __restrict__
                                                      const int32_t* __restrict__ longcnt,
                                                      const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
-                                                     const int32_t* __restrict__ vw,
                                                      const int32_t* __restrict__ vjob,
                                                      const int32_t* __restrict__ v_off,
-                                                     const int32_t* __restrict__ gain,
-                                                     const int32_t* __restrict__ dest,
-                                                     const int32_t* __restrict__ pw,
-                                                     const int32_t* __restrict__ pw_off,
-                                                     const long long* __restrict__ cap,
+                                                     const uint64_t* __restrict__ movable,
                                                      int32_t* __restrict__ list, int32_t* __restrict__ jcount) {
   const int32_t cnt = *longcnt;
   const int lane = lane_id();
@@ -438,8 +463,6 @@ __global__ void __launch_bounds__(256) k_select_long(const int32_t* __restrict__
   for (int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); i < cnt; i += nw) {
     const int32_t v = longv[i];
     const int32_t j = vjob ? vjob[v] : 0;
-    const int32_t* jpw = pw + pw_off[j];
-    const long long jc = cap[j];
     const int32_t s = xadj[v], e = xadj[v + 1];
     bool blocked = false;
     for (int32_t x0 = s; x0 < e && !blocked; x0 += 64) {
@@ -447,7 +470,7 @@ __global__ void __launch_bounds__(256) k_select_long(const int32_t* __restrict__
       bool b = false;
       if (x < e) {
         const int32_t u = adj[x];
-        b = u < v && gain[u] > 0 && (long long)jpw[dest[u]] + vw[u] <= jc;
+        b = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
       }
       blocked = ballot(b) != 0;
     }
@@ -455,18 +478,18 @@ __global__ void __launch_bounds__(256) k_select_long(const int32_t* __restrict__
   }
 }
 
-int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* gain,
-                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                       const uint64_t* cand, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
+int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* dest,
+                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const uint64_t* cand,
+                       uint64_t* movable, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
                        int32_t* reset, int32_t* active, hipStream_t st) {
   if (g.n <= 0) return 0;
   const int64_t waves = ((int64_t(g.n) + 63) / 64 + kSelectWords - 1) / kSelectWords;
-  k_select_list<<<grid_for(waves * 64, 256), 256, 0, st>>>(g.n, g.xadj, g.adj, g.vw, vjob, v_off, gain, dest, pw, pw_off, cap,
-                                                    cand, list, jcount, longv, longcnt, reset, active);
-  unsigned grid = grid_for(g.n, 256);
-  if (grid > 1024) grid = 1024;
-  k_select_long<<<grid, 256, 0, st>>>(longv, longcnt, g.xadj, g.adj, g.vw, vjob, v_off, gain, dest, pw, pw_off, cap,
-                                      list, jcount);
+  const unsigned grid = grid_for(waves * 64, 256);
+  k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, movable, reset, active);
+  k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, v_off, movable, list, jcount, longv, longcnt);
+  unsigned lgrid = grid_for(g.n, 256);
+  if (lgrid > 1024) lgrid = 1024;
+  k_select_long<<<lgrid, 256, 0, st>>>(longv, longcnt, g.xadj, g.adj, vjob, v_off, movable, list, jcount);
   return hip_status();
 }
 

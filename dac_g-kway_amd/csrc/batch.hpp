@@ -60,7 +60,8 @@ int launch_job_eoffsets(int32_t nj, const int32_t* c_off, const int32_t* cxadj, 
 int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, int32_t n, const int32_t* part,
                        const int32_t* bnd, hipStream_t st);
 // Refinement round of the batched form without host round trips: selection
-// over the candidate bitmask (kept by the gain kernels) appended to per-job
+// over the candidate bitmask (kept by the gain kernels; movable: (n + 63) / 64
+// words of scratch for the movable set) appended to per-job
 // segments of list, then one workgroup per job (best min(count, M) moves,
 // LDS sort, balanced prefix, apply).  *active receives the number of selected
 // moves of the round (0 once every job has converged) and lastcnt[j] job j's
@@ -69,9 +70,9 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // cannot change).  jcount[j] and *longcnt (zero before the first round) carry
 // counts from the selection to the round kernel, which clears them; longv (n
 // ints) holds the movable vertices with long rows.  vjob == null: one job.
-int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* gain,
-                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                       const uint64_t* cand, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
+int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* dest,
+                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const uint64_t* cand,
+                       uint64_t* movable, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
                        int32_t* reset, int32_t* active, hipStream_t st);
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, const int32_t* kk, const long long* cap,
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,

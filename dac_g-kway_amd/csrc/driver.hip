@@ -347,7 +347,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, jcount, active, drounds, lastcnt, dvoff, dpw_off, dkk, dmax,
         longv, longcnt;
     Buf<long long> dcap, dmoved;
-    Buf<uint64_t> cand;
+    Buf<uint64_t> cand, movable;
     Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
     GK_TRY(work.alloc(P, 8));
@@ -360,6 +360,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(list.alloc(P, n0));
     GK_TRY(hkey.alloc(P, n0));
     GK_TRY(cand.alloc(P, (int64_t(n0) + 63) / 64 + 1));
+    GK_TRY(movable.alloc(P, (int64_t(n0) + 63) / 64 + 1));
     GK_TRY(hcnt.alloc(P, 2));  // [0] gain worklist count, [1] heavy-row count
     GK_TRY(jcount.alloc(P, 1));
     GK_TRY(active.alloc(P, 1));
@@ -419,7 +420,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       while (true) {
         for (int r = 0; r < kRoundsPerCheck; ++r) {
           GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
-                   gkd::launch_select_list(g, nullptr, dvoff.p, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, cand.p,
+                   gkd::launch_select_list(g, nullptr, dvoff.p, dest.p, pw.p, dpw_off.p, dcap.p, cand.p, movable.p,
                                            list.p, jcount.p, longv.p, longcnt.p, hcnt.p, active.p, st));
           GK_STAGE(ctx, GK_ST_BALANCE, 0,
                    gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p,

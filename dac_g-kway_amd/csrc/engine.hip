@@ -563,7 +563,7 @@ int Engine::refine_step(bool* worked) {
   Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, jcount, dmax, longv,
       longcnt;
   Buf<long long> dcap, dcut, dmoved;
-  Buf<uint64_t> cand;
+  Buf<uint64_t> cand, movable;
   Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
@@ -585,6 +585,7 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(heavy.alloc(P_, N));
   GK_TRY(list.alloc(P_, N));
   GK_TRY(cand.alloc(P_, (int64_t(N) + 63) / 64 + 1));
+  GK_TRY(movable.alloc(P_, (int64_t(N) + 63) / 64 + 1));
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
@@ -638,7 +639,7 @@ int Engine::refine_step(bool* worked) {
     rounds_ += kRoundsPerCheck;
     for (int r = 0; r < kRoundsPerCheck; ++r) {
       GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
-               gkd::launch_select_list(g, rs.vjob.p, F.dv_off.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, cand.p,
+               gkd::launch_select_list(g, rs.vjob.p, F.dv_off.p, dest.p, dpw.p, dpw_off.p, dcap.p, cand.p, movable.p,
                                        list.p, jcount.p, longv.p, longcnt.p, hcnt.p, active.p, st_));
       GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
                gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
