@@ -63,8 +63,23 @@ def stage_table(gk, stats_list, steps, log_it=True):
     return stage
 
 
-def roofline_of(stage, workload_tag):
-    dom = max(stage, key=lambda k_: stage[k_][0])
+def dominant_stage(gk, L, ctx, stage_step, off):
+    """Runs one step with every stage timed, returns the stage with the most
+    device time and restricts the context's stage timing to it."""
+    if off:
+        return None
+    st = stage_step()
+    dom = max(range(len(gk.STAGE_NAMES)), key=lambda i: st.stage_ms[i])
+    L.gk_ctx_set_stage_mask(ctx.handle, 1 << dom)
+    log(f"dominant stage: {gk.STAGE_NAMES[dom]} ({st.stage_ms[dom]:.3f} ms in a timed-stage step)")
+    return gk.STAGE_NAMES[dom]
+
+
+def roofline_of(stage, workload_tag, dom=None):
+    """Roofline of the dominant stage: its algorithmic bytes per call over its
+    average event-timed duration."""
+    if dom is None:
+        dom = max(stage, key=lambda k_: stage[k_][0])
     ms, cnt, by = stage[dom]
     if ms <= 0:
         return None
@@ -117,7 +132,7 @@ def run_batch(args, gk, torch):
     pk.v_off = v_off.ctypes.data_as(I32P)
     pk.e_off = e_off.ctypes.data_as(I32P)
     G = len(graphs)
-    prm = (gk.Params * G)(*[gk.params(k) for k in ks])
+    prm = (gk.Params * G)(*[gk.params(k, flags=0 if args.no_stage_time else gk.GK_FLAG_STAGE_TIME) for k in ks])
     prm_stage = (gk.Params * G)(*[gk.params(k, flags=gk.GK_FLAG_STAGE_TIME) for k in ks])
     L = gk.lib()
     ctx = gk.Context(0)
@@ -133,6 +148,7 @@ def run_batch(args, gk, torch):
         a = time.perf_counter()
         step()
         log(f"warmup {i}: {(time.perf_counter() - a) * 1e3:.1f} ms")
+    dom = dominant_stage(gk, L, ctx, lambda: step(prm_stage)[0], args.no_stage_time)
     torch.cuda.synchronize()
     times, runs = [], []
     t0 = time.perf_counter()
@@ -153,9 +169,11 @@ def run_batch(args, gk, torch):
     within_cap = all(s.max_part_wgt <= s.cap for s in last)
     stage, roofline = None, None
     if not args.no_stage_time:
+        roofline = roofline_of(stage_table(gk, [r[0] for r in runs], args.steps, log_it=False),
+                               f"batch_g{G}_seed{args.seed}", dom)
+        L.gk_ctx_set_stage_mask(ctx.handle, 0xFFFFFFFF)
         sruns = [step(prm_stage) for _ in range(args.stage_steps)]
         stage = stage_table(gk, [r[0] for r in sruns], args.stage_steps, log_it=True)
-        roofline = roofline_of(stage, f"batch_g{G}_seed{args.seed}")
 
     # The plain path on a sample of the same graphs: speed and bit-exactness.
     idx = list(range(0, G, max(1, G // args.plain_sample)))[:args.plain_sample]
@@ -431,10 +449,12 @@ def main():
     L = gk.lib()
 
     ctx = gk.Context(0)
-    p = gk.params(args.k)
-    # Stage timing records HIP events around every stage, which adds a gap
-    # before each kernel; the timed steps run without it and the stage table
-    # comes from separate steps.
+    # Stage timing brackets a stage's launches with HIP events on the work
+    # stream, and each event adds a gap before the next kernel.  The timed
+    # steps therefore time only the dominant stage (the roofline kernel),
+    # picked from one all-stage step before them; the full stage table comes
+    # from separate steps after them.
+    p = gk.params(args.k, flags=0 if args.no_stage_time else gk.GK_FLAG_STAGE_TIME)
     p_stage = gk.params(args.k, flags=gk.GK_FLAG_STAGE_TIME)
 
     def step(pp=p):
@@ -448,6 +468,7 @@ def main():
     for i in range(args.warmup):
         st = step()
         log(f"warmup {i}: {st.t_total_ms:.1f} ms cut={st.cut}")
+    dom = dominant_stage(gk, L, ctx, lambda: step(p_stage), args.no_stage_time)
 
     torch.cuda.synchronize()
     times, stats = [], []
@@ -465,13 +486,15 @@ def main():
     deterministic = len(cuts) == 1
     st = stats[-1]
 
-    # Per-stage device time (HIP events on the work stream) over separate
-    # steps with stage timing.
+    # Roofline from the dominant stage's events over the timed steps; the
+    # stage table from separate all-stage steps.
     stage, roofline = None, None
     if not args.no_stage_time:
+        roofline = roofline_of(stage_table(gk, stats, args.steps, log_it=False),
+                               f"{args.workload}_n{args.n}_k{args.k}", dom)
+        L.gk_ctx_set_stage_mask(ctx.handle, 0xFFFFFFFF)
         sstats = [step(p_stage) for _ in range(args.stage_steps)]
         stage = stage_table(gk, sstats, args.stage_steps, log_it=True)
-        roofline = roofline_of(stage, f"{args.workload}_n{args.n}_k{args.k}")
 
     m = g.m
     ms_step = total / args.steps * 1e3

@@ -165,6 +165,7 @@ def lib():
         L.gk_coarsen_floor.restype = c_int32
         L.gk_ctx_create.argtypes = [c_int, POINTER(c_void_p)]
         L.gk_ctx_destroy.argtypes = [c_void_p]
+        L.gk_ctx_set_stage_mask.argtypes = [c_void_p, ctypes.c_uint32]
         L.gk_partition.argtypes = [c_void_p, POINTER(Csr), POINTER(Params), I32P, POINTER(Stats)]
         L.gk_partition_device.argtypes = [c_void_p, POINTER(Csr), POINTER(Params), I32P,
                                           POINTER(Stats), c_void_p]

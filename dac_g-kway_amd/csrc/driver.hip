@@ -552,6 +552,12 @@ int gk_ctx_create(int device, gk_ctx** out) {
   return GK_OK;
 }
 
+int gk_ctx_set_stage_mask(gk_ctx* c, uint32_t mask) {
+  if (!c) return GK_ERR_ARG;
+  c->clock.set_mask(mask);
+  return GK_OK;
+}
+
 int gk_ctx_destroy(gk_ctx* c) {
   if (!c) return GK_OK;
   DeviceGuard dg(c->device);

@@ -142,13 +142,14 @@ class StageClock {
     for (auto& v : recs_) v.clear();
     used_ = 0;
   }
+  void set_mask(uint32_t m) { mask_ = m; }
   void begin(int id) {
-    if (!on_) return;
+    if (!on_ || !((mask_ >> id) & 1u)) return;
     cur_[id] = take();
     (void)hipEventRecord(cur_[id], st_);
   }
   void end(int id) {
-    if (!on_) return;
+    if (!on_ || !((mask_ >> id) & 1u)) return;
     hipEvent_t e = take();
     (void)hipEventRecord(e, st_);
     recs_[id].emplace_back(cur_[id], e);
@@ -178,6 +179,7 @@ class StageClock {
     return pool_[used_++];
   }
   bool on_ = false;
+  uint32_t mask_ = ~0u;
   hipStream_t st_ = nullptr;
   std::vector<hipEvent_t> pool_;
   size_t used_ = 0;

@@ -150,6 +150,10 @@ typedef struct gk_stats {
 typedef struct gk_ctx gk_ctx;
 int gk_ctx_create(int device, gk_ctx** out);
 int gk_ctx_destroy(gk_ctx* ctx);
+/* Stages that GK_FLAG_STAGE_TIME brackets with HIP events: bit s = stage s
+   (GK_ST_*); default all.  Timing one stage keeps the event overhead out of
+   the others.  No counterpart in the reference (it has no stage timers). */
+int gk_ctx_set_stage_mask(gk_ctx* ctx, uint32_t mask);
 const char* gk_strerror(int status);
 const char* gk_last_error(void);
 
