@@ -76,6 +76,7 @@ class Heap {
   void reset(int32_t n) {
     pos_.assign(n, -1);
     h_.clear();
+    h_.reserve(n);
   }
   bool empty() const { return h_.empty(); }
   bool contains(int32_t v) const { return pos_[v] >= 0; }
@@ -166,38 +167,51 @@ G coarsen_hem(const G& g, Rng& rng, int64_t maxvw, std::vector<int32_t>& cmap) {
     if (match[v] != v) cmap[match[v]] = nc;
     ++nc;
   }
+  // Coarse rows in coarse-id order: coarse vertex cv is first reached at its
+  // smaller member v (min(v, match[v])); its row lists the neighbours of v,
+  // then of match[v], in order of first appearance, weights summed.
   G c;
   c.n = nc;
   c.vw.assign(nc, 0);
   c.xadj.assign(nc + 1, 0);
   c.tw = g.tw;
-  std::vector<int32_t> members(2 * nc, -1);
-  for (int32_t v = 0; v < n; ++v) {
-    int32_t cv = cmap[v];
-    c.vw[cv] += g.vw[v];
-    if (members[2 * cv] < 0) members[2 * cv] = v; else members[2 * cv + 1] = v;
-  }
+  c.adj.resize(g.adj.size());
+  c.w.resize(g.adj.size());
   std::vector<int32_t> slot(nc, -1);
-  c.adj.reserve(g.adj.size());
-  c.w.reserve(g.adj.size());
-  for (int32_t cv = 0; cv < nc; ++cv) {
-    int32_t start = static_cast<int32_t>(c.adj.size());
-    for (int k = 0; k < 2; ++k) {
-      int32_t v = members[2 * cv + k];
-      if (v < 0) continue;
-      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
-        int32_t cu = cmap[g.adj[j]];
+  const int32_t* xa = g.xadj.data();
+  const int32_t* ad = g.adj.data();
+  const int32_t* wt = g.w.data();
+  const int32_t* cm = cmap.data();
+  int32_t* ca = c.adj.data();
+  int32_t* cw = c.w.data();
+  int32_t* sl = slot.data();
+  int32_t top = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    const int32_t m = match[v];
+    if (m < v) continue;  // second member: handled with the first
+    const int32_t cv = cm[v];
+    const int32_t start = top;
+    const int32_t mem[2] = {v, m};
+    for (int k = 0; k < (m != v ? 2 : 1); ++k) {
+      const int32_t x = mem[k];
+      c.vw[cv] += g.vw[x];
+      for (int32_t j = xa[x]; j < xa[x + 1]; ++j) {
+        const int32_t cu = cm[ad[j]];
         if (cu == cv) continue;
-        if (slot[cu] >= start) { c.w[slot[cu]] += g.w[j]; }
-        else {
-          slot[cu] = static_cast<int32_t>(c.adj.size());
-          c.adj.push_back(cu);
-          c.w.push_back(g.w[j]);
+        if (sl[cu] >= start) {
+          cw[sl[cu]] += wt[j];
+        } else {
+          sl[cu] = top;
+          ca[top] = cu;
+          cw[top] = wt[j];
+          ++top;
         }
       }
     }
-    c.xadj[cv + 1] = static_cast<int32_t>(c.adj.size());
+    c.xadj[cv + 1] = top;
   }
+  c.adj.resize(top);
+  c.w.resize(top);
   return c;
 }
 
@@ -231,12 +245,13 @@ void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int
   std::vector<uint8_t> locked(n);
   std::vector<int32_t> moved;
   const int32_t limit = std::max<int32_t>(25, std::min<int32_t>(n / 20, 300));
+  // Internal / external degrees: computed once, then kept exact through the
+  // moves of a pass and their rollback.
+  for (int32_t v = 0; v < n; ++v) {
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+      (side[g.adj[j]] == side[v] ? id[v] : ed[v]) += g.w[j];
+  }
   for (int pass = 0; pass < passes; ++pass) {
-    for (int32_t v = 0; v < n; ++v) {
-      id[v] = ed[v] = 0;
-      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
-        (side[g.adj[j]] == side[v] ? id[v] : ed[v]) += g.w[j];
-    }
     q[0].reset(n);
     q[1].reset(n);
     std::fill(locked.begin(), locked.end(), 0);
@@ -298,37 +313,58 @@ void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int
         break;
       }
     }
-    // Roll back past the best prefix.
+    // Roll back past the best prefix, newest first, undoing each move's
+    // degree updates.
     for (int32_t i = static_cast<int32_t>(moved.size()) - 1; i >= best_len; --i) {
-      int32_t v = moved[i];
-      int to = side[v], from = 1 - to;
+      const int32_t v = moved[i];
+      const int to = side[v], from = 1 - to;
+      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+        const int32_t u = g.adj[j];
+        const int32_t w = g.w[j];
+        if (side[u] == to) { id[u] -= w; ed[u] += w; }
+        else { id[u] += w; ed[u] -= w; }
+      }
+      std::swap(id[v], ed[v]);
       side[v] = static_cast<uint8_t>(from);
+      cur_pw[to] -= g.vw[v];
+      cur_pw[from] += g.vw[v];
     }
-    pw[0] = pw[1] = 0;
-    for (int32_t v = 0; v < n; ++v) pw[side[v]] += g.vw[v];
+    pw[0] = cur_pw[0];
+    pw[1] = cur_pw[1];
     bool improved = best_cut < cut || best_len > 0;
     cut = best_cut;
     if (!improved || best_len == 0) break;
   }
 }
 
+// Buffers of the growing trials of one bisection: the incident edge weight
+// of every vertex (computed once), side-0 connectivity and the heap.
+struct GrowWork {
+  std::vector<int64_t> inc, conn0;
+  Heap q;
+  explicit GrowWork(const G& g) : inc(g.n, 0) {
+    for (int32_t u = 0; u < g.n; ++u)
+      for (int32_t k = g.xadj[u]; k < g.xadj[u + 1]; ++k) inc[u] += g.w[k];
+  }
+};
+
 // Greedy graph growing from `seedv`: side 0 grows by max gain until it
-// reaches its target weight.
-void grow(const G& g, int32_t seedv, int64_t target0, std::vector<uint8_t>& side) {
+// reaches its target weight.  Returns the cut of the result (each added
+// vertex turns its edges to side 0 internal and the rest cut).
+int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vector<uint8_t>& side) {
   const int32_t n = g.n;
   side.assign(n, 1);
-  std::vector<int64_t> conn0(n, 0);
-  Heap q;
+  std::vector<int64_t>& conn0 = gw.conn0;
+  conn0.assign(n, 0);
+  const std::vector<int64_t>& inc = gw.inc;
+  Heap& q = gw.q;
   q.reset(n);
-  int64_t w0 = 0;
+  int64_t w0 = 0, cut = 0;
   int32_t next_unreached = 0;
-  // Total incident edge weight of every vertex.
-  std::vector<int64_t> inc(n, 0);
-  for (int32_t u = 0; u < n; ++u)
-    for (int32_t k = g.xadj[u]; k < g.xadj[u + 1]; ++k) inc[u] += g.w[k];
   auto add = [&](int32_t v) {
     side[v] = 0;
     w0 += g.vw[v];
+    cut += inc[v] - 2 * conn0[v];
     for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
       int32_t u = g.adj[j];
       if (side[u] == 0) continue;
@@ -352,6 +388,7 @@ void grow(const G& g, int32_t seedv, int64_t target0, std::vector<uint8_t>& side
     if (w0 + g.vw[v] > target0 && (w0 + g.vw[v] - target0) > (target0 - w0)) break;
     add(v);
   }
+  return cut;
 }
 
 // Multilevel bisection of g.  target0 = desired weight of side 0.
@@ -379,10 +416,10 @@ void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vect
   std::vector<uint8_t> best, trial;
   int64_t best_cut = INT64_MAX, best_ol = INT64_MAX;
   const int trials = cg.n <= 1 ? 1 : GKIP_TRIALS;
+  GrowWork gw(cg);
   for (int t = 0; t < trials; ++t) {
     int32_t s = static_cast<int32_t>(rng.below(static_cast<uint32_t>(std::max(1, cg.n))));
-    grow(cg, s, target0, trial);
-    int64_t c = cut2(cg, trial);
+    int64_t c = grow(cg, gw, s, target0, trial);
     fm2(cg, trial, bal, 4, c);
     int64_t pw[2] = {0, 0};
     for (int32_t v = 0; v < cg.n; ++v) pw[trial[v]] += cg.vw[v];
