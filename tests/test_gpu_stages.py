@@ -130,6 +130,44 @@ def test_contract_big_rows(gk, oracle, ctx):
     assert np.array_equal(host(cw)[: len(ow)], ow)
 
 
+# (k, moves, seed, lowest slack below the cap): the oracle's prefix ends at
+# 0, 79, 9, 3, 0, 6, 1, 9 for the tight cases, 501, 1012, 729, 3, 1023 for
+# the roomier ones, and nowhere (-1) for (8192, 1024, 14, -12).
+@pytest.mark.parametrize("k,nmoves,seed,lo", [(2, 1, 0, -60), (2, 300, 1, -60), (7, 1024, 2, -60), (64, 1024, 3, -60),
+                                              (64, 37, 4, -60), (1000, 1024, 5, -60), (8192, 1024, 6, -60),
+                                              (8192, 500, 7, -60), (64, 1024, 8, -600), (7, 1024, 10, -3000),
+                                              (64, 1024, 9, -700), (1000, 1024, 12, -30), (8192, 1024, 14, -12),
+                                              (16, 1024, 11, -1500)])
+def test_balance_adversarial(gk, oracle, ctx, k, nmoves, seed, lo):
+    """Balanced prefix on move lists the selection never produces: part
+    weights straddling the cap (some parts already above it), many moves into
+    the same parts, moves whose destination is their own part, k up to the
+    kernel's 8192.  Position and part weights must equal the oracle's."""
+    from _dev import DevGraph, dev, host
+    L = gk.lib()
+    rng = np.random.default_rng(seed)
+    n = 4096
+    xadj = np.arange(n + 1, dtype=np.int32)  # a ring: the balance stage reads only vertex weights
+    adj = ((np.arange(n) + 1) % n).astype(np.int32)
+    g = gk.Graph(xadj, adj, None, rng.integers(1, 20, size=n).astype(np.int32))
+    dg = DevGraph(g)
+    kk = min(k, 16) if seed % 2 else k  # odd seeds: few distinct parts, heavy collisions
+    part = rng.integers(0, kk, size=n).astype(np.int32)
+    dest = rng.integers(0, kk, size=n).astype(np.int32)
+    same = rng.random(n) < 0.05
+    dest[same] = part[same]
+    cap = 1000
+    pw = (cap + rng.integers(lo, 8, size=k)).astype(np.int32)
+    moves = rng.choice(n, size=nmoves, replace=False).astype(np.int32)
+    dpw = dev(pw)
+    pos = ctypes.c_int32(-7)
+    assert L.gk_k_balance(ctx.handle, dg.ref(), k, _I(dev(moves)), nmoves, _I(dev(part)), _I(dev(dest)),
+                          _I(dpw), cap, ctypes.cast(ctypes.pointer(pos), gk.I32P), None) == 0
+    opos, opw = oracle.balance(g, k, moves, part, dest, pw, cap)
+    assert pos.value == opos
+    assert np.array_equal(host(dpw), opw)
+
+
 @pytest.mark.parametrize("k", [2, 16, 256])
 def test_refinement_stages(gk, oracle, ctx, graph, k):
     from _dev import DevGraph, dev, host
