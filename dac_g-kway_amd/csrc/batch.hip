@@ -393,13 +393,19 @@ __global__ void __launch_bounds__(256) k_select_fit(int32_t n, const int32_t* __
 // Selection proper: a movable vertex is selected when no smaller-id
 // neighbour is movable (one bit per neighbour).  Same dealing of the movable
 // mask's bits to lanes as k_select_fit.  Rows longer than kSelectShortRow go
-// to longv for k_select_long.
+// to longv for k_select_long.  Only vertices with gain >= thr[job] are tested
+// here; the others are appended to the job's segment of dlist (dcount), and
+// the round kernel tests them only when fewer than M were selected (every
+// key above the threshold beats every key below it).
 __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ vjob,
                                                      const int32_t* __restrict__ v_off,
+                                                     const int32_t* __restrict__ gain,
+                                                     const int32_t* __restrict__ thr,
                                                      const uint64_t* __restrict__ movable,
                                                      int32_t* __restrict__ list, int32_t* __restrict__ jcount,
+                                                     int32_t* __restrict__ dlist, int32_t* __restrict__ dcount,
                                                      int32_t* __restrict__ longv, int32_t* __restrict__ longcnt) {
   const int lane = lane_id();
   const int64_t nwords = (int64_t(n) + 63) >> 6;
@@ -412,13 +418,15 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
     const int r = base + lane;
     int wi, bit;
     deal_bit(incl, word, r, &wi, &bit);
-    bool sel = false, lng = false;
+    bool sel = false, lng = false, defer = false;
     int32_t j = 0, v = 0;
     if (r < total) {
       v = int32_t(((w0 + wi) << 6) + bit);
       j = vjob ? vjob[v] : 0;
       const int32_t s = xadj[v], e = xadj[v + 1];
-      if (e - s > kSelectShortRow) {
+      if (gain[v] < thr[j]) {
+        defer = true;  // tested by the round kernel only if the job needs it
+      } else if (e - s > kSelectShortRow) {
         lng = true;
       } else {
         // eight neighbours per step: their ids, then their movable bits
@@ -444,6 +452,7 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
       if (lng) longv[lb + popc64(lm & lanemask_lt())] = v;
     }
     append_selected(sel, j, v, v_off, list, jcount);
+    append_selected(defer, j, v, v_off, dlist, dcount);
   }
 }
 
@@ -478,15 +487,17 @@ __global__ void __launch_bounds__(256) k_select_long(const int32_t* __restrict__
   }
 }
 
-int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* dest,
-                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const uint64_t* cand,
-                       uint64_t* movable, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
-                       int32_t* reset, int32_t* active, hipStream_t st) {
+int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* gain,
+                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
+                       const int32_t* thr, const uint64_t* cand, uint64_t* movable, int32_t* list, int32_t* jcount,
+                       int32_t* dlist, int32_t* dcount, int32_t* longv, int32_t* longcnt, int32_t* reset,
+                       int32_t* active, hipStream_t st) {
   if (g.n <= 0) return 0;
   const int64_t waves = ((int64_t(g.n) + 63) / 64 + kSelectWords - 1) / kSelectWords;
   const unsigned grid = grid_for(waves * 64, 256);
   k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, movable, reset, active);
-  k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, v_off, movable, list, jcount, longv, longcnt);
+  k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, v_off, gain, thr, movable, list, jcount, dlist,
+                                      dcount, longv, longcnt);
   unsigned lgrid = grid_for(g.n, 256);
   if (lgrid > 1024) lgrid = 1024;
   k_select_long<<<lgrid, 256, 0, st>>>(longv, longcnt, g.xadj, g.adj, vjob, v_off, movable, list, jcount);
@@ -497,7 +508,12 @@ struct RoundLds {
   BalanceLds b;  // b.sk doubles as the 2048-bin radix histogram
   int32_t ws[40];
   int32_t misc[4];
+  int32_t lz[4];  // deferred tests: selected count; next threshold's digit; next threshold
 };
+
+// Gain threshold of the next round's selection: aim at about
+// kLazyFactor * M tested vertices per job.
+constexpr int kLazyFactor = 2;
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
 #pragma unroll
@@ -542,32 +558,58 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     const int32_t* __restrict__ pw_off, const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj,
     int32_t* __restrict__ part, const int32_t* __restrict__ dest, const int32_t* __restrict__ gain,
     const int32_t* __restrict__ vw, int32_t* __restrict__ pw, long long* __restrict__ cut,
-    int32_t* __restrict__ rounds, long long* __restrict__ moved, const int32_t* __restrict__ list,
+    int32_t* __restrict__ rounds, long long* __restrict__ moved, int32_t* __restrict__ list,
     int32_t* __restrict__ jcount, uint32_t* __restrict__ hkey, int32_t* __restrict__ upd, int32_t* __restrict__ wl,
     int32_t* __restrict__ wlc, int32_t* __restrict__ lastcnt, int32_t* __restrict__ active,
-    int32_t* __restrict__ longcnt, uint64_t* __restrict__ cand) {
+    int32_t* __restrict__ longcnt, uint64_t* __restrict__ cand, const uint64_t* __restrict__ movable,
+    int32_t* __restrict__ thr, const int32_t* __restrict__ dlist, int32_t* __restrict__ dcount) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
   const int32_t j = blockIdx.x;
+  const int32_t vb = v_off[j];
   if (t == 0) {
-    const int32_t c = jcount[j];
-    R.misc[3] = c;
+    R.misc[3] = jcount[j];
     R.misc[2] = lastcnt[j];
+    R.misc[1] = dcount[j];
     jcount[j] = 0;
-    lastcnt[j] = c;
-    if (c) atomicAdd(active, c);
+    dcount[j] = 0;
     if (j == 0) *longcnt = 0;  // the selection kernels of this round are done
   }
   __syncthreads();
-  const int32_t count = R.misc[3];
+  int32_t count = R.misc[3];
+  const int32_t prev = R.misc[2], ndef = R.misc[1];
+  if (count < M && ndef > 0) {
+    // The threshold left fewer than M selected: test the deferred (lower
+    // gain) movable vertices here, so the list holds every selected vertex.
+    if (t == 0) R.lz[0] = count;
+    __syncthreads();
+    const int32_t* dl = dlist + vb;
+    for (int32_t i = t; i < ndef; i += kBalThreads) {
+      const int32_t v = dl[i];
+      bool blocked = false;
+      for (int32_t x = xadj[v]; x < xadj[v + 1] && !blocked; ++x) {
+        const int32_t u = adj[x];
+        blocked = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
+      }
+      if (!blocked) list[vb + atomicAdd(&R.lz[0], 1)] = v;
+    }
+    __syncthreads();
+    count = R.lz[0];
+  }
+  if (t == 0) {
+    lastcnt[j] = count;
+    if (count) atomicAdd(active, count);
+    R.lz[2] = 1;  // next threshold: test everything unless set below
+  }
   if (count == 0) {
+    if (t == 0) thr[j] = 1;
     // Nothing selected: the job's state (partition, gains, part weights) is
     // unchanged, so every later round of this level selects nothing either.
     // The first time, its candidate bits are dropped so that the selection
     // stops re-checking them (with one job the rounds end at the next check
     // anyway).
-    if (R.misc[2] != 0 && gridDim.x > 1) {
+    if (prev != 0 && gridDim.x > 1) {
       const int64_t a = v_off[j], b = v_off[j + 1];
       for (int64_t w = (a >> 6) + t; b > a && w <= ((b - 1) >> 6); w += kBalThreads) {
         const int64_t lo = (a > (w << 6) ? a : (w << 6)) - (w << 6);
@@ -579,7 +621,6 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     }
     return;
   }
-  const int32_t vb = v_off[j];
   const int32_t* seg = list + vb;
   const int32_t B = count < M ? count : M;
   // 2. the B best keys into L.sk[0, B)
@@ -647,7 +688,19 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
         R.misc[0] = 2 * t + 1;
         R.misc[1] = ex + h0;
       }
+      // First digit: also the digit holding rank kLazyFactor * M, for the
+      // next round's gain threshold.
+      const bool lazy = pmask == 0 && count > kLazyFactor * M;
+      const int32_t r2 = kLazyFactor * M;
+      if (lazy && ex < r2 && r2 <= ex + h0) R.lz[1] = 2 * t;
+      else if (lazy && ex + h0 < r2 && r2 <= ex + h0 + h1) R.lz[1] = 2 * t + 1;
       __syncthreads();
+      if (lazy && t == 0) {
+        // every key in digits <= lz[1] has a gain key <= hu, i.e. a gain >= 0x7fffffff - hu
+        const uint64_t top = ((uint64_t(R.lz[1]) + 1) << shift) - 1;
+        const uint64_t hu = uint64_t(hmin) + (top >> ib);
+        R.lz[2] = hu >= 0x7ffffffeull ? 1 : int32_t(0x7fffffffull - hu);
+      }
       need -= R.misc[1];
       prefix |= uint64_t(R.misc[0]) << shift;
       pmask |= 2047ull << shift;
@@ -727,6 +780,7 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     __syncthreads();  // R.misc[0] is reused by the next pass
   }
   if (t == 0) {
+    thr[j] = R.lz[2];
     rounds[j] += 1;
     moved[j] += pos + 1;
   }
@@ -734,14 +788,16 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
 
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, const int32_t* kk, const long long* cap,
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
-                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const int32_t* list,
+                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, int32_t* list,
                    int32_t* jcount, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
-                   int32_t* active, int32_t* longcnt, uint64_t* cand, hipStream_t st) {
+                   int32_t* active, int32_t* longcnt, uint64_t* cand, const uint64_t* movable, int32_t* thr,
+                   const int32_t* dlist, int32_t* dcount, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
   k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, M, kk, cap, pw_off, g.xadj, g.adj, part, dest,
                                                              gain, g.vw, pw, cut, rounds, moved, list, jcount, hkey,
-                                                             upd, wl, wlc, lastcnt, active, longcnt, cand);
+                                                             upd, wl, wlc, lastcnt, active, longcnt, cand, movable,
+                                                             thr, dlist, dcount);
   return hip_status();
 }
 

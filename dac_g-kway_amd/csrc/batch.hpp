@@ -67,18 +67,23 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // moves of the round (0 once every job has converged) and lastcnt[j] job j's
 // count (lastcnt nonzero before the first round: a job's candidate bits are
 // cleared in the first round that selects nothing for it, after which it
-// cannot change).  jcount[j] and *longcnt (zero before the first round) carry
+// cannot change).  thr[j] (1 before the first round) is the gain below which
+// the selection defers a job's movable vertices to dlist/dcount (n ints / one
+// per job, dcount zero before the first round); the round kernel tests them
+// when fewer than M were selected and sets the next round's threshold.  jcount[j] and *longcnt (zero before the first round) carry
 // counts from the selection to the round kernel, which clears them; longv (n
 // ints) holds the movable vertices with long rows.  vjob == null: one job.
-int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* dest,
-                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const uint64_t* cand,
-                       uint64_t* movable, int32_t* list, int32_t* jcount, int32_t* longv, int32_t* longcnt,
-                       int32_t* reset, int32_t* active, hipStream_t st);
+int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* gain,
+                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
+                       const int32_t* thr, const uint64_t* cand, uint64_t* movable, int32_t* list, int32_t* jcount,
+                       int32_t* dlist, int32_t* dcount, int32_t* longv, int32_t* longcnt, int32_t* reset,
+                       int32_t* active, hipStream_t st);
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, const int32_t* kk, const long long* cap,
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
-                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const int32_t* list,
+                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, int32_t* list,
                    int32_t* jcount, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
-                   int32_t* active, int32_t* longcnt, uint64_t* cand, hipStream_t st);
+                   int32_t* active, int32_t* longcnt, uint64_t* cand, const uint64_t* movable, int32_t* thr,
+                   const int32_t* dlist, int32_t* dcount, hipStream_t st);
 // *out = largest row length of g.
 int launch_max_deg(const DGraph& g, int32_t* out, hipStream_t st);
 // Final cut and part weights of the jobs covered by the chunk table: chunk c

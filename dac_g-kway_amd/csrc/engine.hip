@@ -586,6 +586,12 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(list.alloc(P_, N));
   GK_TRY(cand.alloc(P_, (int64_t(N) + 63) / 64 + 1));
   GK_TRY(movable.alloc(P_, (int64_t(N) + 63) / 64 + 1));
+  Buf<int32_t> dlist, dthr, dcount;
+  GK_TRY(dlist.alloc(P_, N));
+  GK_TRY(dthr.alloc(P_, J));
+  GK_TRY(dcount.alloc(P_, J));
+  gkd::fill<int32_t>(dthr.p, 1, J, st_);
+  gkd::fill<int32_t>(dcount.p, 0, J, st_);
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
@@ -639,12 +645,14 @@ int Engine::refine_step(bool* worked) {
     rounds_ += kRoundsPerCheck;
     for (int r = 0; r < kRoundsPerCheck; ++r) {
       GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
-               gkd::launch_select_list(g, rs.vjob.p, F.dv_off.p, dest.p, dpw.p, dpw_off.p, dcap.p, cand.p, movable.p,
-                                       list.p, jcount.p, longv.p, longcnt.p, hcnt.p, active.p, st_));
+               gkd::launch_select_list(g, rs.vjob.p, F.dv_off.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p,
+                                       cand.p, movable.p, list.p, jcount.p, dlist.p, dcount.p, longv.p, longcnt.p,
+                                       hcnt.p, active.p, st_));
       GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
                gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
                                    dpw.p, dcut.p, drounds.p, dmoved.p, list.p, jcount.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                   lastcnt.p, active.p, longcnt.p, cand.p, st_));
+                                   lastcnt.p, active.p, longcnt.p, cand.p, movable.p, dthr.p, dlist.p,
+                                   dcount.p, st_));
       GK_STAGE(ctx_, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
                                    nullptr, wl_items, cand.p, st_));
