@@ -272,48 +272,19 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // (gkway/uncoarsen.hpp:111-173) with the cap and part weights of the vertex's
 // job, over the candidate bitmask kept by the gain kernels (boundary vertices
 // with positive gain), so vertices outside the mask cost one bit of a
-// coalesced load.  Two kernels: k_select_fit turns the candidates into the
-// movable set (move fits the cap), k_select_list keeps the movable vertices
-// with no smaller-id movable neighbour.  Selected vertices are appended, in
-// no particular order, to their job's segment of list (list + v_off[job],
-// jcount[job] entries; jcount zero before the launch).  vjob == null: one job.
-// Appends the selected vertices of one wave to their jobs' list segments:
-// the selected lanes form runs of equal job id (ids do not decrease along the
-// wave) and the first lane of each run reserves the run's slots with one
-// atomic.  Every lane of the wave calls this.
-__device__ __forceinline__ void append_selected(bool sel, int32_t j, int32_t v, const int32_t* __restrict__ v_off,
-                                                int32_t* __restrict__ list, int32_t* __restrict__ jcount) {
-  const uint64_t m = ballot(sel);
-  if (m == 0) return;
-  const int lane = lane_id();
-  const uint64_t lt = lanemask_lt();
-  const uint64_t lower = m & lt;
-  const int p = lower ? 63 - __clzll(static_cast<long long>(lower)) : lane;
-  const int32_t jp = __shfl(j, p, 64);
-  const bool leader = sel && (lower == 0 || jp != j);
-  const uint64_t L = ballot(leader);
-  int base = 0;
-  if (leader) {
-    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-    const uint64_t rest = L & ~le;
-    const int q = rest ? __ffsll(static_cast<long long>(rest)) - 1 : 64;
-    const uint64_t upto = q == 64 ? ~0ull : ((1ull << q) - 1);
-    base = atomicAdd(&jcount[j], popc64(m & upto & ~((1ull << lane) - 1)));
-  }
-  const uint64_t lmask = L & (lt | (1ull << lane));
-  const int my = lmask ? 63 - __clzll(static_cast<long long>(lmask)) : lane;
-  base = __shfl(base, my, 64);
-  if (sel) {
-    const int off = popc64(m & lt) - popc64(m & ((1ull << my) - 1));
-    list[v_off[j] + base + off] = v;
-  }
-}
-
+// coalesced load.  k_select_fit turns the candidates into the movable set
+// (move fits the cap); k_select_list and k_select_long keep the movable
+// vertices with no smaller-id movable neighbour, as a bitmask; a scan turns
+// the bitmask into a list ordered by vertex id, in which every job's selected
+// vertices form one slice.  No kernel of the selection shares a counter
+// between waves (same-address atomics serialise in L2).  vjob == null: one
+// job.
+// Candidate bitmask words read per wave by k_select_fit / k_select_list, and
+// per 256-thread block.
+constexpr int kSelectWords = 8;
+constexpr int kSelectBlockWords = 4 * kSelectWords;
 // Rows longer than this are checked by k_select_long, one wave per vertex.
 constexpr int32_t kSelectShortRow = 32;
-
-// Candidate bitmask words read per wave by k_select_list.
-constexpr int kSelectWords = 8;
 
 // Position of the k-th (0-based) set bit of w (k < popcount(w)).
 __device__ __forceinline__ int nth_set_bit(uint64_t w, int k) {
@@ -392,22 +363,30 @@ __global__ void __launch_bounds__(256) k_select_fit(int32_t n, const int32_t* __
 
 // Selection proper: a movable vertex is selected when no smaller-id
 // neighbour is movable (one bit per neighbour).  Same dealing of the movable
-// mask's bits to lanes as k_select_fit.  Rows longer than kSelectShortRow go
-// to longv for k_select_long.  Only vertices with gain >= thr[job] are tested
-// here; the others are appended to the job's segment of dlist (dcount), and
-// the round kernel tests them only when fewer than M were selected (every
-// key above the threshold beats every key below it).
+// mask's bits to lanes as k_select_fit.  Only vertices with gain >= thr[job]
+// are tested; the others get a bit in defbits, and the round kernel tests
+// them only when fewer than M were selected (every key above the threshold
+// beats every key below it).  Rows longer than kSelectShortRow get a bit in
+// longbits for k_select_long.  Results are bitmask words (selbits, defbits,
+// longbits), all written, so no counter is shared between waves.
 __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ vjob,
-                                                     const int32_t* __restrict__ v_off,
                                                      const int32_t* __restrict__ gain,
                                                      const int32_t* __restrict__ thr,
                                                      const uint64_t* __restrict__ movable,
-                                                     int32_t* __restrict__ list, int32_t* __restrict__ jcount,
-                                                     int32_t* __restrict__ dlist, int32_t* __restrict__ dcount,
-                                                     int32_t* __restrict__ longv, int32_t* __restrict__ longcnt) {
+                                                     uint64_t* __restrict__ selbits, uint64_t* __restrict__ defbits,
+                                                     uint64_t* __restrict__ longbits, int32_t* __restrict__ bsum) {
+  __shared__ unsigned long long sw[4][3][kSelectWords];
+  __shared__ int32_t wsum[4];
   const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
+  if (lane < kSelectWords) {
+    sw[wv][0][lane] = 0;
+    sw[wv][1][lane] = 0;
+    sw[wv][2][lane] = 0;
+  }
+  __syncthreads();
   const int64_t nwords = (int64_t(n) + 63) >> 6;
   const int64_t w0 = ((int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6) * kSelectWords;
   uint64_t word = 0;
@@ -418,19 +397,18 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
     const int r = base + lane;
     int wi, bit;
     deal_bit(incl, word, r, &wi, &bit);
-    bool sel = false, lng = false, defer = false;
-    int32_t j = 0, v = 0;
     if (r < total) {
-      v = int32_t(((w0 + wi) << 6) + bit);
-      j = vjob ? vjob[v] : 0;
+      const int32_t v = int32_t(((w0 + wi) << 6) + bit);
+      const int32_t j = vjob ? vjob[v] : 0;
       const int32_t s = xadj[v], e = xadj[v + 1];
+      int which = -1;  // 0 selected, 1 deferred, 2 long row
       if (gain[v] < thr[j]) {
-        defer = true;  // tested by the round kernel only if the job needs it
+        which = 1;
       } else if (e - s > kSelectShortRow) {
-        lng = true;
+        which = 2;
       } else {
         // eight neighbours per step: their ids, then their movable bits
-        sel = true;
+        bool sel = true;
         for (int32_t x = s; x < e && sel; x += 8) {
           int32_t u[8];
 #pragma unroll
@@ -441,66 +419,132 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
             if (u[i] < v && ((mwd >> (u[i] & 63)) & 1ull)) sel = false;
           }
         }
+        if (sel) which = 0;
       }
+      if (which >= 0) atomicOr(&sw[wv][which][wi], 1ull << bit);
     }
-    const uint64_t lm = ballot(lng);
-    if (lm) {
-      const int leader = __ffsll(static_cast<long long>(lm)) - 1;
-      int lb = 0;
-      if (lane == leader) lb = atomicAdd(longcnt, popc64(lm));
-      lb = __shfl(lb, leader, 64);
-      if (lng) longv[lb + popc64(lm & lanemask_lt())] = v;
-    }
-    append_selected(sel, j, v, v_off, list, jcount);
-    append_selected(defer, j, v, v_off, dlist, dcount);
   }
+  __syncthreads();
+  int c = 0;
+  if (lane < kSelectWords && w0 + lane < nwords) {
+    selbits[w0 + lane] = sw[wv][0][lane];
+    defbits[w0 + lane] = sw[wv][1][lane];
+    longbits[w0 + lane] = sw[wv][2][lane];
+    c = popc64(sw[wv][0][lane]);
+  }
+  // selected vertices of the block's kSelectBlockWords words
+  c = wave_sum(c);
+  if (lane == 0) wsum[wv] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// Movable vertices with rows longer than kSelectShortRow: one wave checks 64
-// neighbours at a time.
-__global__ void __launch_bounds__(256) k_select_long(const int32_t* __restrict__ longv,
-                                                     const int32_t* __restrict__ longcnt,
+// Vertices with rows longer than kSelectShortRow: a wave per vertex, 64
+// neighbours at a time, over the words of longbits (grid-stride); selected
+// ones are OR-ed into selbits.
+__global__ void __launch_bounds__(256) k_select_long(int32_t n, const uint64_t* __restrict__ longbits,
                                                      const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
-                                                     const int32_t* __restrict__ vjob,
-                                                     const int32_t* __restrict__ v_off,
                                                      const uint64_t* __restrict__ movable,
-                                                     int32_t* __restrict__ list, int32_t* __restrict__ jcount) {
-  const int32_t cnt = *longcnt;
+                                                     uint64_t* __restrict__ selbits, int32_t* __restrict__ bsum) {
   const int lane = lane_id();
+  const int64_t nwords = (int64_t(n) + 63) >> 6;
   const int64_t nw = int64_t(gridDim.x) * 4;
-  for (int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); i < cnt; i += nw) {
-    const int32_t v = longv[i];
-    const int32_t j = vjob ? vjob[v] : 0;
-    const int32_t s = xadj[v], e = xadj[v + 1];
-    bool blocked = false;
-    for (int32_t x0 = s; x0 < e && !blocked; x0 += 64) {
-      const int32_t x = x0 + lane;
-      bool b = false;
-      if (x < e) {
-        const int32_t u = adj[x];
-        b = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
+  for (int64_t w = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); w < nwords; w += nw) {
+    uint64_t lb = longbits[w];
+    while (lb) {
+      const int bit = __ffsll(static_cast<long long>(lb)) - 1;
+      lb &= lb - 1;
+      const int32_t v = int32_t((w << 6) + bit);
+      const int32_t s = xadj[v], e = xadj[v + 1];
+      bool blocked = false;
+      for (int32_t x0 = s; x0 < e && !blocked; x0 += 64) {
+        const int32_t x = x0 + lane;
+        bool b = false;
+        if (x < e) {
+          const int32_t u = adj[x];
+          b = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
+        }
+        blocked = ballot(b) != 0;
       }
-      blocked = ballot(b) != 0;
+      if (!blocked && lane == 0) {
+        atomicOr(reinterpret_cast<unsigned long long*>(&selbits[w]), 1ull << bit);
+        atomicAdd(&bsum[w / kSelectBlockWords], 1);
+      }
     }
-    if (!blocked && lane == 0) list[v_off[j] + atomicAdd(&jcount[j], 1)] = v;
   }
 }
 
-int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* gain,
-                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                       const int32_t* thr, const uint64_t* cand, uint64_t* movable, int32_t* list, int32_t* jcount,
-                       int32_t* dlist, int32_t* dcount, int32_t* longv, int32_t* longcnt, int32_t* reset,
-                       int32_t* active, hipStream_t st) {
+// Ordered list of the selected vertices: every word's first list slot
+// (wpos) is the number of selected vertices before it.  Block b covers
+// kScatterWords words; its offset is the sum of the selection blocks' counts
+// (bsum) before its first word, then a block scan over its words.  One launch,
+// no shared counter.  wpos[nwords] receives the total.
+constexpr int kScatterWords = 2048;
+
+__global__ void __launch_bounds__(256) k_sel_scatter(int64_t nwords, const uint64_t* __restrict__ selbits,
+                                                     const int32_t* __restrict__ bsum, int32_t* __restrict__ wpos,
+                                                     int32_t* __restrict__ list) {
+  __shared__ int lds[8];
+  const int64_t wb = int64_t(blockIdx.x) * kScatterWords;
+  // offset: selection blocks [0, wb / kSelectBlockWords)
+  const int64_t nb = wb / kSelectBlockWords;
+  int off = 0;
+  for (int64_t i = threadIdx.x; i < nb; i += 256) off += bsum[i];
+  int tot;
+  (void)block_excl_scan_256(off, &tot, lds);
+  off = tot;
+  constexpr int kPer = kScatterWords / 256;
+  const int64_t w0 = wb + int64_t(threadIdx.x) * kPer;
+  uint64_t b[kPer];
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    b[i] = w0 + i < nwords ? selbits[w0 + i] : 0;
+    c += popc64(b[i]);
+  }
+  int r = off + block_excl_scan_256(c, &tot, lds);
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    if (w0 + i < nwords) wpos[w0 + i] = r;
+    uint64_t x = b[i];
+    while (x) {
+      const int bit = __ffsll(static_cast<long long>(x)) - 1;
+      x &= x - 1;
+      list[r++] = int32_t(((w0 + i) << 6) + bit);
+    }
+  }
+  if (wb + kScatterWords >= nwords && threadIdx.x == 255) wpos[nwords] = r;
+}
+
+// List slot of vertex x (the number of selected vertices before it).
+__device__ __forceinline__ int32_t sel_slot(const int32_t* __restrict__ wpos, const uint64_t* __restrict__ selbits,
+                                            int64_t x) {
+  const int64_t w = x >> 6;
+  const int b = int(x & 63);
+  return wpos[w] + (b ? popc64(selbits[w] & ((1ull << b) - 1)) : 0);
+}
+
+int select_scratch_ints(int64_t n) {
+  const int64_t nwords = (n + 63) / 64;
+  return static_cast<int>((nwords + kSelectBlockWords - 1) / kSelectBlockWords + 1);
+}
+
+int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
+                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const int32_t* thr,
+                       const uint64_t* cand, const SelectBufs& sb, int32_t* reset, int32_t* active,
+                       hipStream_t st) {
   if (g.n <= 0) return 0;
-  const int64_t waves = ((int64_t(g.n) + 63) / 64 + kSelectWords - 1) / kSelectWords;
+  const int64_t nwords = (int64_t(g.n) + 63) / 64;
+  const int64_t waves = (nwords + kSelectWords - 1) / kSelectWords;
   const unsigned grid = grid_for(waves * 64, 256);
-  k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, movable, reset, active);
-  k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, v_off, gain, thr, movable, list, jcount, dlist,
-                                      dcount, longv, longcnt);
-  unsigned lgrid = grid_for(g.n, 256);
+  k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable, reset, active);
+  k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, gain, thr, sb.movable, sb.selbits, sb.defbits,
+                                      sb.longbits, sb.scratch);
+  unsigned lgrid = grid_for(nwords * 64, 256);
   if (lgrid > 1024) lgrid = 1024;
-  k_select_long<<<lgrid, 256, 0, st>>>(longv, longcnt, g.xadj, g.adj, vjob, v_off, movable, list, jcount);
+  k_select_long<<<lgrid, 256, 0, st>>>(g.n, sb.longbits, g.xadj, g.adj, sb.movable, sb.selbits, sb.scratch);
+  k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list);
   return hip_status();
 }
 
@@ -543,11 +587,13 @@ __device__ __forceinline__ uint64_t move_key(const int32_t* __restrict__ gain, i
 }
 
 // One refinement round of one job per workgroup, after k_select_list:
-//   1. the job's selected vertices: list[v_off[j] ...], jcount[j] of them
-//      (jcount is cleared here for the next round);
+//   1. the job's selected vertices: its slice of the selection list (plus,
+//      when fewer than M and a gain threshold deferred some, those of the
+//      deferred vertices that pass the test);
 //   2. the min(count, M) best moves by (gain desc, id asc): all of them when
-//      count <= M, else an 8-pass radix select on the 64-bit key (gain key,
-//      id) -- keys are distinct, so exactly M are taken;
+//      count <= M, else an exact radix select on a compressed key -- keys are
+//      distinct, so exactly M are taken -- whose first digit also gives the
+//      gain threshold of the job's next round;
 //   3. LDS bitonic sort, balanced prefix (balance_prefix);
 //   4. apply of the prefix: part, update flags and the gain worklist
 //      (apply_sequence_move, gkway/uncoarsen.hpp:290-323).
@@ -558,44 +604,51 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     const int32_t* __restrict__ pw_off, const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj,
     int32_t* __restrict__ part, const int32_t* __restrict__ dest, const int32_t* __restrict__ gain,
     const int32_t* __restrict__ vw, int32_t* __restrict__ pw, long long* __restrict__ cut,
-    int32_t* __restrict__ rounds, long long* __restrict__ moved, int32_t* __restrict__ list,
-    int32_t* __restrict__ jcount, uint32_t* __restrict__ hkey, int32_t* __restrict__ upd, int32_t* __restrict__ wl,
-    int32_t* __restrict__ wlc, int32_t* __restrict__ lastcnt, int32_t* __restrict__ active,
-    int32_t* __restrict__ longcnt, uint64_t* __restrict__ cand, const uint64_t* __restrict__ movable,
-    int32_t* __restrict__ thr, const int32_t* __restrict__ dlist, int32_t* __restrict__ dcount) {
+    int32_t* __restrict__ rounds, long long* __restrict__ moved, const SelectBufs sb, int32_t* __restrict__ xlist,
+    uint32_t* __restrict__ hkey, int32_t* __restrict__ upd, int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
+    int32_t* __restrict__ lastcnt, int32_t* __restrict__ active, uint64_t* __restrict__ cand,
+    int32_t* __restrict__ thr) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
   const int32_t j = blockIdx.x;
-  const int32_t vb = v_off[j];
+  const int32_t vb = v_off[j], ve = v_off[j + 1];
   if (t == 0) {
-    R.misc[3] = jcount[j];
+    R.misc[3] = sel_slot(sb.wpos, sb.selbits, vb);
+    R.misc[1] = sel_slot(sb.wpos, sb.selbits, ve);
     R.misc[2] = lastcnt[j];
-    R.misc[1] = dcount[j];
-    jcount[j] = 0;
-    dcount[j] = 0;
-    if (j == 0) *longcnt = 0;  // the selection kernels of this round are done
   }
   __syncthreads();
-  int32_t count = R.misc[3];
-  const int32_t prev = R.misc[2], ndef = R.misc[1];
-  if (count < M && ndef > 0) {
+  const int32_t* seg = sb.list + R.misc[3];
+  int32_t count = R.misc[1] - R.misc[3];
+  const int32_t prev = R.misc[2];
+  if (count < M && thr[j] > 1) {
     // The threshold left fewer than M selected: test the deferred (lower
-    // gain) movable vertices here, so the list holds every selected vertex.
+    // gain) movable vertices of the job too, into the job's segment of xlist
+    // after the selected ones, so that it holds every selected vertex.
+    int32_t* xs = xlist + vb;
+    for (int32_t i = t; i < count; i += kBalThreads) xs[i] = seg[i];
     if (t == 0) R.lz[0] = count;
     __syncthreads();
-    const int32_t* dl = dlist + vb;
-    for (int32_t i = t; i < ndef; i += kBalThreads) {
-      const int32_t v = dl[i];
-      bool blocked = false;
-      for (int32_t x = xadj[v]; x < xadj[v + 1] && !blocked; ++x) {
-        const int32_t u = adj[x];
-        blocked = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
+    for (int64_t w = (int64_t(vb) >> 6) + t; ve > vb && w <= ((int64_t(ve) - 1) >> 6); w += kBalThreads) {
+      const int64_t lo = (vb > (w << 6) ? vb : (w << 6)) - (w << 6);
+      const int64_t hi = (ve < (w << 6) + 64 ? ve : (w << 6) + 64) - (w << 6);
+      uint64_t db = sb.defbits[w] & ((hi - lo == 64) ? ~0ull : (((1ull << (hi - lo)) - 1) << lo));
+      while (db) {
+        const int bit = __ffsll(static_cast<long long>(db)) - 1;
+        db &= db - 1;
+        const int32_t v = int32_t((w << 6) + bit);
+        bool blocked = false;
+        for (int32_t x = xadj[v]; x < xadj[v + 1] && !blocked; ++x) {
+          const int32_t u = adj[x];
+          blocked = u < v && ((sb.movable[u >> 6] >> (u & 63)) & 1ull);
+        }
+        if (!blocked) xs[atomicAdd(&R.lz[0], 1)] = v;
       }
-      if (!blocked) list[vb + atomicAdd(&R.lz[0], 1)] = v;
     }
     __syncthreads();
     count = R.lz[0];
+    seg = xs;
   }
   if (t == 0) {
     lastcnt[j] = count;
@@ -621,7 +674,6 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     }
     return;
   }
-  const int32_t* seg = list + vb;
   const int32_t B = count < M ? count : M;
   // 2. the B best keys into L.sk[0, B)
   if (count <= M) {
@@ -788,16 +840,14 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
 
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, const int32_t* kk, const long long* cap,
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
-                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, int32_t* list,
-                   int32_t* jcount, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
-                   int32_t* active, int32_t* longcnt, uint64_t* cand, const uint64_t* movable, int32_t* thr,
-                   const int32_t* dlist, int32_t* dcount, hipStream_t st) {
+                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const SelectBufs& sb,
+                   int32_t* xlist, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
+                   int32_t* active, uint64_t* cand, int32_t* thr, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
   k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, M, kk, cap, pw_off, g.xadj, g.adj, part, dest,
-                                                             gain, g.vw, pw, cut, rounds, moved, list, jcount, hkey,
-                                                             upd, wl, wlc, lastcnt, active, longcnt, cand, movable,
-                                                             thr, dlist, dcount);
+                                                             gain, g.vw, pw, cut, rounds, moved, sb, xlist, hkey,
+                                                             upd, wl, wlc, lastcnt, active, cand, thr);
   return hip_status();
 }
 

@@ -59,31 +59,42 @@ int launch_job_offsets(int32_t nj, const int32_t* v_off, const int32_t* order, c
 int launch_job_eoffsets(int32_t nj, const int32_t* c_off, const int32_t* cxadj, int32_t* ce_off, hipStream_t st);
 int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, int32_t n, const int32_t* part,
                        const int32_t* bnd, hipStream_t st);
-// Refinement round of the batched form without host round trips: selection
-// over the candidate bitmask (kept by the gain kernels; movable: (n + 63) / 64
-// words of scratch for the movable set) appended to per-job
-// segments of list, then one workgroup per job (best min(count, M) moves,
-// LDS sort, balanced prefix, apply).  *active receives the number of selected
-// moves of the round (0 once every job has converged) and lastcnt[j] job j's
-// count (lastcnt nonzero before the first round: a job's candidate bits are
-// cleared in the first round that selects nothing for it, after which it
-// cannot change).  thr[j] (1 before the first round) is the gain below which
-// the selection defers a job's movable vertices to dlist/dcount (n ints / one
-// per job, dcount zero before the first round); the round kernel tests them
-// when fewer than M were selected and sets the next round's threshold.  jcount[j] and *longcnt (zero before the first round) carry
-// counts from the selection to the round kernel, which clears them; longv (n
-// ints) holds the movable vertices with long rows.  vjob == null: one job.
-int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* v_off, const int32_t* gain,
-                       const int32_t* dest, const int32_t* pw, const int32_t* pw_off, const long long* cap,
-                       const int32_t* thr, const uint64_t* cand, uint64_t* movable, int32_t* list, int32_t* jcount,
-                       int32_t* dlist, int32_t* dcount, int32_t* longv, int32_t* longcnt, int32_t* reset,
-                       int32_t* active, hipStream_t st);
+// Buffers of the round selection (n vertices, nwords = (n + 63) / 64):
+// movable, selbits, defbits, longbits and wpos hold nwords + 1 entries, the
+// last word of selbits zero (filled once); list holds n ints; scratch holds
+// select_scratch_ints(n) ints (per-block selection counts).
+struct SelectBufs {
+  uint64_t* movable;
+  uint64_t* selbits;
+  uint64_t* defbits;
+  uint64_t* longbits;
+  int32_t* wpos;
+  int32_t* list;
+  int* scratch;
+};
+int select_scratch_ints(int64_t n);
+
+// Refinement round without host round trips.  launch_select_list: the
+// movable set (candidates whose move fits the job's cap), the selection
+// (no smaller-id movable neighbour) as a bitmask, and its ordered list; a
+// job's movable vertices with gain below thr[job] (1 before a level's first
+// round) are only marked in defbits.  launch_round_b: one workgroup per job
+// (best min(count, M) moves, LDS sort, balanced prefix, apply), which tests
+// the deferred vertices when fewer than M were selected (into the job's
+// segment of xlist, n ints) and sets the job's next threshold.  *active
+// receives the number of selected moves of the round (0 once every job has
+// converged) and lastcnt[j] job j's count (lastcnt nonzero before the first
+// round: a job's candidate bits are cleared in the first round that selects
+// nothing for it, after which it cannot change).  vjob == null: one job.
+int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
+                       const int32_t* pw, const int32_t* pw_off, const long long* cap, const int32_t* thr,
+                       const uint64_t* cand, const SelectBufs& sb, int32_t* reset, int32_t* active,
+                       hipStream_t st);
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, const int32_t* kk, const long long* cap,
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
-                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, int32_t* list,
-                   int32_t* jcount, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
-                   int32_t* active, int32_t* longcnt, uint64_t* cand, const uint64_t* movable, int32_t* thr,
-                   const int32_t* dlist, int32_t* dcount, hipStream_t st);
+                   int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const SelectBufs& sb,
+                   int32_t* xlist, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
+                   int32_t* active, uint64_t* cand, int32_t* thr, hipStream_t st);
 // *out = largest row length of g.
 int launch_max_deg(const DGraph& g, int32_t* out, hipStream_t st);
 // Final cut and part weights of the jobs covered by the chunk table: chunk c

@@ -344,11 +344,10 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   int32_t* oth_bnd = bndB.p;
   if (nl > 1) {
     const int32_t M = prm.max_moves;
-    Buf<int32_t> dlist, dthr, dcount;
-    Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, jcount, active, drounds, lastcnt, dvoff, dpw_off, dkk, dmax,
-        longv, longcnt;
+    Buf<int32_t> xlist, dthr, wpos, sscr;
+    Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, active, drounds, lastcnt, dvoff, dpw_off, dkk, dmax;
     Buf<long long> dcap, dmoved;
-    Buf<uint64_t> cand, movable;
+    Buf<uint64_t> cand, movable, selbits, defbits, longbits;
     Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
     GK_TRY(work.alloc(P, 8));
@@ -361,12 +360,16 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(list.alloc(P, n0));
     GK_TRY(hkey.alloc(P, n0));
     GK_TRY(cand.alloc(P, (int64_t(n0) + 63) / 64 + 1));
-    GK_TRY(movable.alloc(P, (int64_t(n0) + 63) / 64 + 1));
-    GK_TRY(dlist.alloc(P, n0));
+    const int64_t nwords0 = (int64_t(n0) + 63) / 64;
+    GK_TRY(movable.alloc(P, nwords0 + 1));
+    GK_TRY(selbits.alloc(P, nwords0 + 1));
+    GK_TRY(defbits.alloc(P, nwords0 + 1));
+    GK_TRY(longbits.alloc(P, nwords0 + 1));
+    GK_TRY(wpos.alloc(P, nwords0 + 1));
+    GK_TRY(sscr.alloc(P, gkd::select_scratch_ints(n0)));
+    GK_TRY(xlist.alloc(P, n0));
     GK_TRY(dthr.alloc(P, 1));
-    GK_TRY(dcount.alloc(P, 1));
     GK_TRY(hcnt.alloc(P, 2));  // [0] gain worklist count, [1] heavy-row count
-    GK_TRY(jcount.alloc(P, 1));
     GK_TRY(active.alloc(P, 1));
     GK_TRY(drounds.alloc(P, 1));
     GK_TRY(lastcnt.alloc(P, 1));
@@ -376,10 +379,8 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(dkk.alloc(P, 1));
     GK_TRY(dcap.alloc(P, 1));
     GK_TRY(dmax.alloc(P, 1));
-    GK_TRY(longv.alloc(P, n0));
-    GK_TRY(longcnt.alloc(P, 1));
-    gkd::fill<int32_t>(jcount.p, 0, 1, st);
-    gkd::fill<int32_t>(longcnt.p, 0, 1, st);
+    gkd::fill<uint64_t>(selbits.p, 0ull, nwords0 + 1, st);
+    const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
     gkd::fill<int32_t>(dvoff.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p + 1, k, 1, st);
@@ -408,7 +409,6 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       gkd::fill<int32_t>(drounds.p, 0, 1, st);
       gkd::fill<int32_t>(lastcnt.p, 1, 1, st);
       gkd::fill<int32_t>(dthr.p, 1, 1, st);
-      gkd::fill<int32_t>(dcount.p, 0, 1, st);
       gkd::fill<long long>(dmoved.p, 0, 1, st);
       // The projected boundary becomes the first gain worklist.
       gkd::fill<int32_t>(hcnt.p, 0, 2, st);
@@ -426,14 +426,12 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       while (true) {
         for (int r = 0; r < kRoundsPerCheck; ++r) {
           GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
-                   gkd::launch_select_list(g, nullptr, dvoff.p, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p,
-                                           cand.p, movable.p, list.p, jcount.p, dlist.p, dcount.p, longv.p,
-                                           longcnt.p, hcnt.p, active.p, st));
+                   gkd::launch_select_list(g, nullptr, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
+                                           hcnt.p, active.p, st));
           GK_STAGE(ctx, GK_ST_BALANCE, 0,
                    gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p,
-                                       pw.p, dcut.p, drounds.p, dmoved.p, list.p, jcount.p, hkey.p, upd.p, wl.p,
-                                       hcnt.p, lastcnt.p, active.p, longcnt.p, cand.p, movable.p, dthr.p,
-                                       dlist.p, dcount.p, st));
+                                       pw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p,
+                                       hcnt.p, lastcnt.p, active.p, cand.p, dthr.p, st));
           GK_STAGE(ctx, GK_ST_GAIN, 0,
                    gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p,
                                        dest.p, work.p, wl_items, cand.p, st));

@@ -560,10 +560,10 @@ int Engine::refine_step(bool* worked) {
   const Store& F = *pk;
   const DGraph g = F.graph();
   const int32_t N = g.n;
-  Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, jcount, dmax, longv,
-      longcnt;
+  Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, dmax, xlist, dthr, wpos,
+      sscr;
   Buf<long long> dcap, dcut, dmoved;
-  Buf<uint64_t> cand, movable;
+  Buf<uint64_t> cand, movable, selbits, defbits, longbits;
   Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
@@ -585,25 +585,25 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(heavy.alloc(P_, N));
   GK_TRY(list.alloc(P_, N));
   GK_TRY(cand.alloc(P_, (int64_t(N) + 63) / 64 + 1));
-  GK_TRY(movable.alloc(P_, (int64_t(N) + 63) / 64 + 1));
-  Buf<int32_t> dlist, dthr, dcount;
-  GK_TRY(dlist.alloc(P_, N));
+  const int64_t nwords = (int64_t(N) + 63) / 64;
+  GK_TRY(movable.alloc(P_, nwords + 1));
+  GK_TRY(selbits.alloc(P_, nwords + 1));
+  GK_TRY(defbits.alloc(P_, nwords + 1));
+  GK_TRY(longbits.alloc(P_, nwords + 1));
+  GK_TRY(wpos.alloc(P_, nwords + 1));
+  GK_TRY(sscr.alloc(P_, gkd::select_scratch_ints(N)));
+  GK_TRY(xlist.alloc(P_, N));
   GK_TRY(dthr.alloc(P_, J));
-  GK_TRY(dcount.alloc(P_, J));
   gkd::fill<int32_t>(dthr.p, 1, J, st_);
-  gkd::fill<int32_t>(dcount.p, 0, J, st_);
+  gkd::fill<uint64_t>(selbits.p, 0ull, nwords + 1, st_);
+  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
   GK_TRY(drounds.alloc(P_, J));
   GK_TRY(lastcnt.alloc(P_, J));
   gkd::fill<int32_t>(lastcnt.p, 1, J, st_);
-  GK_TRY(jcount.alloc(P_, J));
   GK_TRY(dmax.alloc(P_, 1));
-  GK_TRY(longv.alloc(P_, N));
-  GK_TRY(longcnt.alloc(P_, 1));
-  gkd::fill<int32_t>(jcount.p, 0, J, st_);
-  gkd::fill<int32_t>(longcnt.p, 0, 1, st_);
   GK_TRY(dmoved.alloc(P_, J));
   gkd::fill<int32_t>(drounds.p, 0, J, st_);
   gkd::fill<long long>(dmoved.p, 0, J, st_);
@@ -645,14 +645,12 @@ int Engine::refine_step(bool* worked) {
     rounds_ += kRoundsPerCheck;
     for (int r = 0; r < kRoundsPerCheck; ++r) {
       GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
-               gkd::launch_select_list(g, rs.vjob.p, F.dv_off.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p,
-                                       cand.p, movable.p, list.p, jcount.p, dlist.p, dcount.p, longv.p, longcnt.p,
+               gkd::launch_select_list(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
                                        hcnt.p, active.p, st_));
       GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
                gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
-                                   dpw.p, dcut.p, drounds.p, dmoved.p, list.p, jcount.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                   lastcnt.p, active.p, longcnt.p, cand.p, movable.p, dthr.p, dlist.p,
-                                   dcount.p, st_));
+                                   dpw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
+                                   lastcnt.p, active.p, cand.p, dthr.p, st_));
       GK_STAGE(ctx_, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
                                    nullptr, wl_items, cand.p, st_));
