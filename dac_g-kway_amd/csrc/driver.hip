@@ -350,8 +350,11 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     Buf<uint64_t> cand, movable, selbits, defbits, longbits;
     Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
-    GK_TRY(work.alloc(P, 8));
-    gkd::fill<unsigned long long>(work.p, 0ull, 8, st);
+    // gain statistics: a (vertices, entries) pair per block of the gain
+    // launches (grids of at most kGainSlots blocks), summed at the end
+    constexpr int kGainSlots = 2048;
+    GK_TRY(work.alloc(P, 2 * kGainSlots));
+    gkd::fill<unsigned long long>(work.p, 0ull, 2 * kGainSlots, st);
     GK_TRY(gain.alloc(P, n0));
     GK_TRY(dest.alloc(P, n0));
     GK_TRY(upd.alloc(P, n0));
@@ -460,20 +463,26 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       }
     }
     // Data-dependent bytes of the gain passes from the device work counters.
-    unsigned long long wk[8];
-    GK_TRY(fetch(ctx, st, {{wk, work.p, sizeof wk}}));
-    S.stage_bytes[GK_ST_GAIN] += int64_t(28 * wk[0] + 12 * wk[1]);
+    std::vector<unsigned long long> wk;
+    GK_TRY(to_host(st, wk, work.p, 2 * kGainSlots));
+    unsigned long long wv = 0, we = 0;
+    for (int i = 0; i < kGainSlots; ++i) {
+      wv += wk[2 * i];
+      we += wk[2 * i + 1];
+    }
+    S.stage_bytes[GK_ST_GAIN] += int64_t(28 * wv + 12 * we);
   }
   GK_HIP(hipStreamSynchronize(st));
   const auto t3 = clk::now();
 
   // ---------------------------------------------------------- final metrics
   {
-    Buf<unsigned long long> cut, pw64;
+    Buf<unsigned long long> cut, pw64, cscr;
     GK_TRY(cut.alloc(P, 1));
     GK_TRY(pw64.alloc(P, k));
+    GK_TRY(cscr.alloc(P, gkd::cut_scratch(L[0].g.n, k)));
     GK_STAGE(ctx, GK_ST_CUT, 16LL * n0 + 12LL * L[0].g.nnz,
-             gkd::launch_cut(L[0].g, k, cur_part, cut.p, pw64.p, st));
+             gkd::launch_cut(L[0].g, k, cur_part, cut.p, pw64.p, cscr.p, st));
     std::vector<unsigned long long> hpw64;
     unsigned long long hcut = 0;
     long long tracked = 0;
@@ -882,10 +891,11 @@ int gk_k_cut(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, int6
   DeviceGuard guard(ctx->device);
   hipStream_t st = pick(ctx, stream);
   DGraph g = dview(dg);
-  Buf<unsigned long long> c, p;
+  Buf<unsigned long long> c, p, cscr;
   GK_TRY(c.alloc(ctx->pool, 1));
   GK_TRY(p.alloc(ctx->pool, k));
-  GK_TRY(gkd::launch_cut(g, k, part, c.p, p.p, st));
+  GK_TRY(cscr.alloc(ctx->pool, gkd::cut_scratch(g.n, k)));
+  GK_TRY(gkd::launch_cut(g, k, part, c.p, p.p, cscr.p, st));
   unsigned long long hc = 0;
   GK_TRY(fetch(ctx, st, {{&hc, c.p, sizeof hc}}));
   *cut = static_cast<int64_t>(hc);

@@ -166,10 +166,18 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
     }
   }
   if (work) {
+    // Statistics: one slot pair per block (no two blocks of a launch share
+    // one, launches are stream-ordered), summed on the host.
+    __shared__ int ws[2][4];
     const int a = wave_sum(done_v), b = wave_sum(done_e);
-    if (lane_id() == 0 && a) {
-      atomicAdd(&work[0], (unsigned long long)a);
-      atomicAdd(&work[1], (unsigned long long)b);
+    if (lane_id() == 0) {
+      ws[0][threadIdx.x >> 6] = a;
+      ws[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      work[2 * blockIdx.x] += (unsigned long long)(ws[0][0] + ws[0][1] + ws[0][2] + ws[0][3]);
+      work[2 * blockIdx.x + 1] += (unsigned long long)(ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3]);
     }
   }
 }
@@ -190,6 +198,7 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
   extern __shared__ int32_t hist[];
   const int lane = lane_id();
   const int32_t cnt = wlc[1];
+  unsigned long long done_v = 0, done_e = 0;
   for (int32_t hi = blockIdx.x; hi < cnt; hi += gridDim.x) {
     const int32_t v = heavy[hi];
     for (int32_t q = lane; q < k; q += 64) hist[q] = 0;
@@ -224,12 +233,14 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
       else { gain[v] = 0; dest[v] = p; }
       if (cand) set_cand(cand, v, any && g > 0);
       upd[v] = 0;
-      if (work) {
-        atomicAdd(&work[0], 1ull);
-        atomicAdd(&work[1], (unsigned long long)(e - s));
-      }
+      done_v += 1;
+      done_e += (unsigned long long)(e - s);
     }
     __syncthreads();
+  }
+  if (work && lane == 0 && done_v) {
+    work[2 * blockIdx.x] += done_v;  // one slot pair per block, as in k_gain_wl
+    work[2 * blockIdx.x + 1] += done_e;
   }
 }
 
@@ -443,14 +454,29 @@ int launch_boundary(const DGraph& g, const int32_t* part, int32_t* bnd, hipStrea
 // gathered in an LDS histogram per block, then one global atomic per block
 // and part (a block's share of the total weight fits 32 bits, the total
 // being below 2^31); the cut reduced per wave in 64 bits.
+// Blocks of the cut kernel: at most 2048, and at most 4 Mi part-weight slots.
+inline unsigned cut_grid(int64_t n, int32_t k) {
+  unsigned grid = grid_for(n * 8, 256);
+  const int64_t cap = std::max<int64_t>(1, (int64_t(1) << 22) / std::max<int32_t>(k, 1));
+  if (grid > 2048) grid = 2048;
+  if (grid > cap) grid = static_cast<unsigned>(cap);
+  return grid;
+}
+
+int64_t cut_scratch(int64_t n, int32_t k) { return int64_t(cut_grid(n, k)) * (int64_t(k) + 1); }
+
+// Each block writes its share of the cut and its part-weight histogram to its
+// own slots (bcut[b], bpw[b * k + q]); k_cut_sum adds them up.  No counter is
+// shared between blocks.
 __global__ void __launch_bounds__(256) k_cut(int32_t n, int32_t k, const int32_t* __restrict__ xadj,
                                              const int32_t* __restrict__ adj,
                                              const int32_t* __restrict__ w,
                                              const int32_t* __restrict__ vw,
                                              const int32_t* __restrict__ part,
-                                             unsigned long long* __restrict__ cut,
-                                             unsigned long long* __restrict__ pw64) {
+                                             unsigned long long* __restrict__ bcut,
+                                             unsigned long long* __restrict__ bpw) {
   extern __shared__ unsigned int hist32[];
+  __shared__ long long wc[4];
   for (int32_t q = threadIdx.x; q < k; q += 256) hist32[q] = 0;
   __syncthreads();
   long long c = 0;
@@ -464,20 +490,47 @@ __global__ void __launch_bounds__(256) k_cut(int32_t n, int32_t k, const int32_t
     if (sub == 0) atomicAdd(&hist32[p], (unsigned int)vw[v]);
   }
   c = wave_sum64(c);
-  if (lane_id() == 0 && c != 0) atomicAdd(cut, (unsigned long long)c);
+  if (lane_id() == 0) wc[threadIdx.x >> 6] = c;
   __syncthreads();
-  for (int32_t q = threadIdx.x; q < k; q += 256)
-    if (hist32[q]) atomicAdd(&pw64[q], (unsigned long long)hist32[q]);
+  if (threadIdx.x == 0) bcut[blockIdx.x] = (unsigned long long)(wc[0] + wc[1] + wc[2] + wc[3]);
+  for (int32_t q = threadIdx.x; q < k; q += 256) bpw[int64_t(blockIdx.x) * k + q] = hist32[q];
+}
+
+// Block q < k: pw64[q] = sum of the blocks' slots for part q; block k: the
+// cut.
+__global__ void __launch_bounds__(256) k_cut_sum(int32_t nb, int32_t k, const unsigned long long* __restrict__ bcut,
+                                                 const unsigned long long* __restrict__ bpw,
+                                                 unsigned long long* __restrict__ cut,
+                                                 unsigned long long* __restrict__ pw64) {
+  __shared__ unsigned long long ws[4];
+  const int32_t q = blockIdx.x;
+  unsigned long long s = 0;
+  if (q < k) {
+    for (int32_t b = threadIdx.x; b < nb; b += 256) s += bpw[int64_t(b) * k + q];
+  } else {
+    for (int32_t b = threadIdx.x; b < nb; b += 256) s += bcut[b];
+  }
+  s = (unsigned long long)wave_sum64((long long)s);
+  if (lane_id() == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = ws[0] + ws[1] + ws[2] + ws[3];
+    if (q < k) pw64[q] = t; else *cut = t;
+  }
 }
 
 int launch_cut(const DGraph& g, int32_t k, const int32_t* part, unsigned long long* cut,
-               unsigned long long* pw64, hipStream_t st) {
-  fill<unsigned long long>(cut, 0ull, 1, st);
-  fill<unsigned long long>(pw64, 0ull, k, st);
-  if (g.n <= 0) return 0;
-  unsigned grid = grid_for(int64_t(g.n) * 8, 256);
-  if (grid > 2048) grid = 2048;
-  k_cut<<<grid, 256, sizeof(unsigned int) * k, st>>>(g.n, k, g.xadj, g.adj, g.w, g.vw, part, cut, pw64);
+               unsigned long long* pw64, unsigned long long* scratch, hipStream_t st) {
+  if (g.n <= 0) {
+    fill<unsigned long long>(cut, 0ull, 1, st);
+    fill<unsigned long long>(pw64, 0ull, k, st);
+    return hip_status();
+  }
+  const unsigned grid = cut_grid(g.n, k);
+  unsigned long long* bcut = scratch;
+  unsigned long long* bpw = scratch + grid;
+  k_cut<<<grid, 256, sizeof(unsigned int) * k, st>>>(g.n, k, g.xadj, g.adj, g.w, g.vw, part, bcut, bpw);
+  k_cut_sum<<<k + 1, 256, 0, st>>>(int32_t(grid), k, bcut, bpw, cut, pw64);
   return hip_status();
 }
 
