@@ -136,3 +136,11 @@ def test_batch_rejects_mixed_moves(gk, ctx):
     outs = (gk.I32P * n)(*[p.ctypes.data_as(gk.I32P) for p in parts])
     rc = gk.lib().gk_partition_batch(ctx.handle, n, csrs, prm, outs, None)
     assert rc == -1  # GK_ERR_ARG
+
+
+def test_batch_small_move_cap(gk, ctx):
+    # every round selects far more than M, so thresholds and their fallback
+    # run for many jobs at once
+    graphs, ks = mixed_set(gk, 24, 11)
+    parts, stats = ctx.partition_batch(graphs, ks, max_moves=24)
+    check_same(ctx, graphs, ks, parts, stats, max_moves=24)

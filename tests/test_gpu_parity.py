@@ -64,3 +64,15 @@ def test_dag_weighted(gk, oracle, ctx, n, layers, k):
 @pytest.mark.parametrize("n,k", [(30000, 8), (60000, 256)])
 def test_road(gk, oracle, ctx, n, k):
     compare(gk, oracle, ctx, gk.road(n, 2), k)
+
+
+# Small move caps: the best-M radix select, the per-job gain threshold of the
+# next round (set when more than 2M are selected) and its fallback (fewer than
+# M pass the threshold, the deferred vertices are tested by the round kernel)
+# all run in most rounds.
+@pytest.mark.parametrize("family,n,k,moves", [("delaunay", 20000, 16, 8), ("delaunay", 20000, 16, 37),
+                                              ("dag", 20000, 32, 16), ("dag", 50000, 256, 100),
+                                              ("road", 30000, 8, 64)])
+def test_small_move_cap(gk, oracle, ctx, family, n, k, moves):
+    g = {"delaunay": lambda: gk.delaunay(n, 5), "dag": lambda: gk.dag(n, 40, 3), "road": lambda: gk.road(n, 2)}[family]()
+    compare(gk, oracle, ctx, g, k, max_moves=moves)
