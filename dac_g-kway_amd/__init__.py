@@ -189,6 +189,9 @@ def lib():
         L.gk_graph_write.argtypes = [c_char_p, POINTER(Csr)]
         L.gk_graph_check.argtypes = [POINTER(Csr), c_int]
         L.gk_graph_free.argtypes = [POINTER(Csr)]
+        L.gk_graph_write_bin.argtypes = [c_char_p, POINTER(Csr)]
+        L.gk_graph_map_bin.argtypes = [c_char_p, c_int, POINTER(Csr), POINTER(c_void_p)]
+        L.gk_graph_unmap.argtypes = [c_void_p]
         L.gk_write_partition.argtypes = [c_char_p, I32P, c_int32]
         L.gk_partition_file.argtypes = [VP, c_char_p, c_char_p, c_int32, POINTER(Stats)]
         L.gk_partition_packed.argtypes = [VP, POINTER(PackedCsr), POINTER(Params), I32P,
@@ -263,6 +266,32 @@ def fnv1a(a, h=0xcbf29ce484222325) -> int:
     x = np.uint64(h)
     # vectorised-in-python is slow for big arrays; use the native helper.
     return int(gen().gkgen_fnv1a_i32(a.ctypes.data_as(I32P), len(a), h))
+
+
+def write_bin(g: "Graph", path):
+    """Binary CSR file (gk_graph_write_bin)."""
+    L = lib()
+    rc = L.gk_graph_write_bin(path.encode(), ctypes.byref(g.csr()))
+    if rc != GK_OK:
+        raise RuntimeError(f"gk_graph_write_bin: {L.gk_strerror(rc).decode()}: {L.gk_last_error().decode()}")
+
+
+def map_bin(path, verify=True) -> Graph:
+    """Maps a binary CSR file (gk_graph_map_bin) and returns a copy of it."""
+    L = lib()
+    c = Csr()
+    h = c_void_p()
+    rc = L.gk_graph_map_bin(path.encode(), 1 if verify else 0, ctypes.byref(c), ctypes.byref(h))
+    if rc != GK_OK:
+        raise RuntimeError(f"gk_graph_map_bin: {L.gk_strerror(rc).decode()}: {L.gk_last_error().decode()}")
+    n, nnz = c.n, c.nnz
+    try:
+        return Graph(np.ctypeslib.as_array(c.xadj, (n + 1,)).copy(),
+                     np.ctypeslib.as_array(c.adjncy, (max(nnz, 1),))[:nnz].copy() if nnz else np.zeros(0, np.int32),
+                     np.ctypeslib.as_array(c.adjwgt, (max(nnz, 1),))[:nnz].copy() if c.adjwgt else None,
+                     np.ctypeslib.as_array(c.vwgt, (max(n, 1),))[:n].copy() if c.vwgt else None)
+    finally:
+        L.gk_graph_unmap(h)
 
 
 def read_metis(path) -> Graph:

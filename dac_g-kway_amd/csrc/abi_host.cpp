@@ -65,12 +65,27 @@ int32_t gk_coarsen_floor(int32_t n0, int32_t k, int32_t policy, int32_t override
   return static_cast<int32_t>(r > INT32_MAX ? INT32_MAX : r);
 }
 
+__attribute__((visibility("hidden"))) bool gk_is_bin_csr(const char* path);  // csr_bin.cpp
+
 int gk_graph_read(const char* path, gk_csr* out) {
   if (!path || !out) { gkh::set_error("null argument"); return GK_ERR_ARG; }
   gkh::HostGraph g;
-  std::string err;
-  int rc = gkh::read_metis(path, g, err);
-  if (rc != GK_OK) { gkh::set_error(err); return rc; }
+  if (gk_is_bin_csr(path)) {
+    gk_csr m;
+    void* h = nullptr;
+    int rc = gk_graph_map_bin(path, 1, &m, &h);
+    if (rc != GK_OK) return rc;
+    g.n = m.n;
+    g.xadj.assign(m.xadj, m.xadj + m.n + 1);
+    g.adjncy.assign(m.adjncy, m.adjncy + m.nnz);
+    if (m.adjwgt) g.adjwgt.assign(m.adjwgt, m.adjwgt + m.nnz); else g.adjwgt.assign(m.nnz, 1);
+    if (m.vwgt) g.vwgt.assign(m.vwgt, m.vwgt + m.n); else g.vwgt.assign(m.n, 1);
+    gk_graph_unmap(h);
+  } else {
+    std::string err;
+    int rc = gkh::read_metis(path, g, err);
+    if (rc != GK_OK) { gkh::set_error(err); return rc; }
+  }
   const size_t nnz = g.adjncy.size();
   int32_t* xadj = static_cast<int32_t*>(malloc(sizeof(int32_t) * (g.n + 1)));
   int32_t* adj = static_cast<int32_t*>(malloc(sizeof(int32_t) * (nnz ? nnz : 1)));

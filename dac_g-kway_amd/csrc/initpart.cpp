@@ -133,11 +133,109 @@ class Heap {
   std::vector<int32_t> pos_;
 };
 
+// ------------------------------------------------------------------ fork-join
+// Process-wide worker threads for the independent pieces of the recursive
+// bisection (the trial bisections of a node and the two halves below it).
+// A thread waiting for its own tasks runs queued tasks meanwhile, so nested
+// fork-join never blocks a worker.  Results never depend on which thread runs
+// a task.
+class ForkJoin {
+ public:
+  struct Task {
+    std::function<void()> fn;
+    bool done = false;  // guarded by the pool mutex
+  };
+  static ForkJoin& get() {
+    static ForkJoin pool;
+    return pool;
+  }
+  int workers() const { return static_cast<int>(th_.size()); }
+  std::shared_ptr<Task> spawn(std::function<void()> fn) {
+    auto t = std::make_shared<Task>();
+    t->fn = std::move(fn);
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      q_.push_back(t);
+    }
+    cv_.notify_all();
+    return t;
+  }
+  // Runs queued tasks until t is done; sleeps when there is nothing to run.
+  void wait(const std::shared_ptr<Task>& t) {
+    while (true) {
+      std::shared_ptr<Task> o;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return t->done || !q_.empty(); });
+        if (t->done) return;
+        o = q_.back();  // newest first: the waiter's own, most likely
+        q_.pop_back();
+      }
+      run(*o);
+    }
+  }
+
+ private:
+  ForkJoin() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int n = static_cast<int>(hw ? std::min(hw, 16u) : 1u) - 1;
+    for (int i = 0; i < n; ++i)
+      th_.emplace_back([this] {
+        while (true) {
+          std::shared_ptr<Task> t;
+          {
+            std::unique_lock<std::mutex> lk(m_);
+            cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+            if (stop_ && q_.empty()) return;
+            t = q_.front();
+            q_.pop_front();
+          }
+          run(*t);
+        }
+      });
+  }
+  ~ForkJoin() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  void run(Task& t) {
+    t.fn();
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      t.done = true;
+    }
+    cv_.notify_all();
+  }
+  std::vector<std::thread> th_;
+  std::deque<std::shared_ptr<Task>> q_;
+  std::mutex m_;
+  std::condition_variable cv_;
+  bool stop_ = false;
+};
+
+// Runs fn(c) for c in [0, chunks): chunk 0 on the calling thread, the others
+// as pool tasks (the caller may itself be a pool task).
+template <class F>
+void par_chunks(int chunks, const F& fn) {
+  if (chunks <= 1) {
+    if (chunks == 1) fn(0);
+    return;
+  }
+  std::vector<std::shared_ptr<ForkJoin::Task>> ts;
+  for (int c = 1; c < chunks; ++c) ts.push_back(ForkJoin::get().spawn([&fn, c] { fn(c); }));
+  fn(0);
+  for (auto& t : ts) ForkJoin::get().wait(t);
+}
+
 // ------------------------------------------------------------------ coarsening
 // Heavy-edge matching in a seeded random visiting order.  Ties between equal
 // edge weights go to the lighter neighbour, then the lower id.  cmap gets the
 // coarse id of every vertex; returns the coarse graph.
-G coarsen_hem(const G& g, Rng& rng, int64_t maxvw, std::vector<int32_t>& cmap) {
+G coarsen_hem(const G& g, Rng& rng, int64_t maxvw, std::vector<int32_t>& cmap, bool par) {
   const int32_t n = g.n;
   std::vector<int32_t> perm(n);
   std::iota(perm.begin(), perm.end(), 0);
@@ -175,6 +273,57 @@ G coarsen_hem(const G& g, Rng& rng, int64_t maxvw, std::vector<int32_t>& cmap) {
   c.vw.assign(nc, 0);
   c.xadj.assign(nc + 1, 0);
   c.tw = g.tw;
+  const int chunks = par ? static_cast<int>(std::min<size_t>(8, g.adj.size() / 32768)) : 1;
+  if (chunks > 1) {
+    // Rows are independent: ranges of coarse ids are built on pool threads
+    // into their own buffers and concatenated (the same arrays as below).
+    std::vector<int32_t> firstv(nc);
+    for (int32_t v = 0; v < n; ++v)
+      if (match[v] >= v) firstv[cmap[v]] = v;
+    struct Piece {
+      std::vector<int32_t> adj, w;
+    };
+    std::vector<Piece> pieces(chunks);
+    par_chunks(chunks, [&](int ci) {
+      const int32_t lo = static_cast<int32_t>(int64_t(nc) * ci / chunks);
+      const int32_t hi = static_cast<int32_t>(int64_t(nc) * (ci + 1) / chunks);
+      Piece& pc = pieces[ci];
+      pc.adj.reserve(g.adj.size() / chunks + 1024);
+      pc.w.reserve(g.adj.size() / chunks + 1024);
+      std::vector<int32_t> slot(nc, -1);
+      for (int32_t cv = lo; cv < hi; ++cv) {
+        const int32_t v = firstv[cv], m = match[v];
+        const int32_t start = static_cast<int32_t>(pc.adj.size());
+        const int32_t mem[2] = {v, m};
+        for (int k = 0; k < (m != v ? 2 : 1); ++k) {
+          const int32_t x = mem[k];
+          c.vw[cv] += g.vw[x];
+          for (int32_t j = g.xadj[x]; j < g.xadj[x + 1]; ++j) {
+            const int32_t cu = cmap[g.adj[j]];
+            if (cu == cv) continue;
+            if (slot[cu] >= start) {
+              pc.w[slot[cu]] += g.w[j];
+            } else {
+              slot[cu] = static_cast<int32_t>(pc.adj.size());
+              pc.adj.push_back(cu);
+              pc.w.push_back(g.w[j]);
+            }
+          }
+        }
+        c.xadj[cv + 1] = static_cast<int32_t>(pc.adj.size()) - start;  // row length for now
+      }
+    });
+    for (int32_t cv = 0; cv < nc; ++cv) c.xadj[cv + 1] += c.xadj[cv];
+    c.adj.resize(c.xadj[nc]);
+    c.w.resize(c.xadj[nc]);
+    int64_t off = 0;
+    for (int ci = 0; ci < chunks; ++ci) {
+      std::copy(pieces[ci].adj.begin(), pieces[ci].adj.end(), c.adj.begin() + off);
+      std::copy(pieces[ci].w.begin(), pieces[ci].w.end(), c.w.begin() + off);
+      off += static_cast<int64_t>(pieces[ci].adj.size());
+    }
+    return c;
+  }
   c.adj.resize(g.adj.size());
   c.w.resize(g.adj.size());
   std::vector<int32_t> slot(nc, -1);
@@ -392,7 +541,7 @@ int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vect
 }
 
 // Multilevel bisection of g.  target0 = desired weight of side 0.
-void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vector<uint8_t>& side) {
+void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vector<uint8_t>& side, bool par) {
   Rng rng(seed);
   const int64_t t1 = g.tw - target0;
   Bal bal;
@@ -405,7 +554,7 @@ void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vect
   const int64_t maxvw = std::max<int64_t>(1, static_cast<int64_t>(1.5 * double(g.tw) / 60.0));
   while (cur->n > 120) {
     std::vector<int32_t> cm;
-    G c = coarsen_hem(*cur, rng, maxvw, cm);
+    G c = coarsen_hem(*cur, rng, maxvw, cm, par);
     if (c.n > 0.92 * cur->n) break;
     cmaps.push_back(std::move(cm));
     levels.push_back(std::move(c));
@@ -444,10 +593,17 @@ void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vect
 G induced(const G& g, const std::vector<uint8_t>& side, uint8_t s, std::vector<int32_t>& map_back) {
   std::vector<int32_t> loc(g.n, -1);
   map_back.clear();
+  int64_t ub = 0;  // entries of the kept rows: room for the induced rows
   for (int32_t v = 0; v < g.n; ++v)
-    if (side[v] == s) { loc[v] = static_cast<int32_t>(map_back.size()); map_back.push_back(v); }
+    if (side[v] == s) {
+      loc[v] = static_cast<int32_t>(map_back.size());
+      map_back.push_back(v);
+      ub += g.xadj[v + 1] - g.xadj[v];
+    }
   G h;
   h.n = static_cast<int32_t>(map_back.size());
+  h.adj.reserve(ub);
+  h.w.reserve(ub);
   h.xadj.assign(h.n + 1, 0);
   h.vw.resize(h.n);
   for (int32_t i = 0; i < h.n; ++i) {
@@ -464,90 +620,6 @@ G induced(const G& g, const std::vector<uint8_t>& side, uint8_t s, std::vector<i
   }
   return h;
 }
-
-// ------------------------------------------------------------------ fork-join
-// Process-wide worker threads for the independent pieces of the recursive
-// bisection (the trial bisections of a node and the two halves below it).
-// A thread waiting for its own tasks runs queued tasks meanwhile, so nested
-// fork-join never blocks a worker.  Results never depend on which thread runs
-// a task.
-class ForkJoin {
- public:
-  struct Task {
-    std::function<void()> fn;
-    bool done = false;  // guarded by the pool mutex
-  };
-  static ForkJoin& get() {
-    static ForkJoin pool;
-    return pool;
-  }
-  int workers() const { return static_cast<int>(th_.size()); }
-  std::shared_ptr<Task> spawn(std::function<void()> fn) {
-    auto t = std::make_shared<Task>();
-    t->fn = std::move(fn);
-    {
-      std::lock_guard<std::mutex> lk(m_);
-      q_.push_back(t);
-    }
-    cv_.notify_all();
-    return t;
-  }
-  // Runs queued tasks until t is done; sleeps when there is nothing to run.
-  void wait(const std::shared_ptr<Task>& t) {
-    while (true) {
-      std::shared_ptr<Task> o;
-      {
-        std::unique_lock<std::mutex> lk(m_);
-        cv_.wait(lk, [&] { return t->done || !q_.empty(); });
-        if (t->done) return;
-        o = q_.back();  // newest first: the waiter's own, most likely
-        q_.pop_back();
-      }
-      run(*o);
-    }
-  }
-
- private:
-  ForkJoin() {
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int n = static_cast<int>(hw ? std::min(hw, 16u) : 1u) - 1;
-    for (int i = 0; i < n; ++i)
-      th_.emplace_back([this] {
-        while (true) {
-          std::shared_ptr<Task> t;
-          {
-            std::unique_lock<std::mutex> lk(m_);
-            cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
-            if (stop_ && q_.empty()) return;
-            t = q_.front();
-            q_.pop_front();
-          }
-          run(*t);
-        }
-      });
-  }
-  ~ForkJoin() {
-    {
-      std::lock_guard<std::mutex> lk(m_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (auto& t : th_) t.join();
-  }
-  void run(Task& t) {
-    t.fn();
-    {
-      std::lock_guard<std::mutex> lk(m_);
-      t.done = true;
-    }
-    cv_.notify_all();
-  }
-  std::vector<std::thread> th_;
-  std::deque<std::shared_ptr<Task>> q_;
-  std::mutex m_;
-  std::condition_variable cv_;
-  bool stop_ = false;
-};
 
 struct RBCtx {
   double tol;
@@ -577,7 +649,7 @@ void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* ou
   std::vector<std::vector<uint8_t>> trial(ncuts);
   std::vector<int64_t> key_ol(ncuts), key_cut(ncuts);
   auto run_trial = [&](int c) {
-    ml_bisect(g, target0, ctx.tol, s + 0x51ED2701ull * uint64_t(c), trial[c]);
+    ml_bisect(g, target0, ctx.tol, s + 0x51ED2701ull * uint64_t(c), trial[c], ctx.parallel);
     int64_t pw0 = 0;
     for (int32_t v = 0; v < g.n; ++v) if (trial[c][v] == 0) pw0 += g.vw[v];
     const int64_t t1 = g.tw - target0;
@@ -599,8 +671,15 @@ void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* ou
     if (key_ol[c] < key_ol[bc] || (key_ol[c] == key_ol[bc] && key_cut[c] < key_cut[bc])) bc = c;
   side.swap(trial[bc]);
   std::vector<int32_t> mb0, mb1;
-  G g0 = induced(g, side, 0, mb0);
-  G g1 = induced(g, side, 1, mb1);
+  G g0, g1;
+  if (ctx.parallel && g.n > GKIP_SPAWN_MIN) {
+    auto t = ForkJoin::get().spawn([&] { g1 = induced(g, side, 1, mb1); });
+    g0 = induced(g, side, 0, mb0);
+    ForkJoin::get().wait(t);
+  } else {
+    g0 = induced(g, side, 0, mb0);
+    g1 = induced(g, side, 1, mb1);
+  }
   std::vector<int32_t> ids0(mb0.size()), ids1(mb1.size());
   for (size_t i = 0; i < mb0.size(); ++i) ids0[i] = ids[mb0[i]];
   for (size_t i = 0; i < mb1.size(); ++i) ids1[i] = ids[mb1[i]];

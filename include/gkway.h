@@ -252,6 +252,16 @@ int gk_graph_read(const char* path, gk_csr* out);
 void gk_graph_free(gk_csr* g);
 int gk_graph_write(const char* path, const gk_csr* g);
 int gk_graph_check(const gk_csr* g, int check_symmetry);
+/* Binary CSR files (this build's format, no counterpart in the reference,
+   whose only input is METIS text, gkway/graph.hpp:70-152): a 64-byte header
+   and 64-byte-aligned int32 sections, so a graph is mapped instead of parsed.
+   gk_graph_write_bin writes one; gk_graph_map_bin maps one read-only and
+   points out's arrays into the mapping (no copy; adjwgt / vwgt NULL when the
+   file has none; verify != 0 checks the checksum) until gk_graph_unmap.
+   gk_graph_read also accepts binary CSR files (copying, like METIS text). */
+int gk_graph_write_bin(const char* path, const gk_csr* g);
+int gk_graph_map_bin(const char* path, int verify, gk_csr* out, void** handle);
+void gk_graph_unmap(void* handle);
 int gk_write_partition(const char* path, const int32_t* part, int32_t n);
 
 /* ---- coarsest-level initial partitioner (host) -----------------------------

@@ -180,6 +180,74 @@ def test_write_read_round_trip(gk, tmp_path):
         assert np.array_equal(getattr(g, a), getattr(h, a))
 
 
+def _same_graph(a, b, weights=True):
+    assert (a.n, a.nnz) == (b.n, b.nnz)
+    assert np.array_equal(a.xadj, b.xadj) and np.array_equal(a.adjncy, b.adjncy)
+    if weights:
+        assert np.array_equal(a.adjwgt, b.adjwgt) and np.array_equal(a.vwgt, b.vwgt)
+
+
+def test_binary_csr_round_trip(gk, tmp_path):
+    # weighted graph: mapped without copying, and through gk_graph_read
+    g = gk.dag(3000, 12, 5)
+    path = str(tmp_path / "g.gkcsr")
+    gk.write_bin(g, path)
+    _same_graph(gk.map_bin(path), g)
+    _same_graph(gk.map_bin(path, verify=False), g)
+    _same_graph(gk.read_metis(path), g)
+    # the METIS sample graph, text -> binary -> back
+    s = gk.read_metis(os.path.join(GOLDEN, "delaunay_n11.graph"))
+    p2 = str(tmp_path / "s.gkcsr")
+    gk.write_bin(s, p2)
+    _same_graph(gk.read_metis(p2), s)
+    # sections start on 64-byte boundaries
+    raw = open(path, "rb").read()
+    assert raw[:5] == b"GKCSR" and len(raw) >= 64 + 4 * (g.n + 1) + 8 * g.nnz + 4 * g.n
+
+
+def test_binary_csr_without_weights(gk, tmp_path):
+    L = gk.lib()
+    x = np.array([0, 1, 3, 4], np.int32)
+    a = np.array([1, 0, 2, 1], np.int32)
+    c = gk.Csr()
+    c.n, c.nnz = 3, 4
+    c.xadj, c.adjncy = x.ctypes.data_as(gk.I32P), a.ctypes.data_as(gk.I32P)
+    path = str(tmp_path / "u.gkcsr")
+    assert L.gk_graph_write_bin(path.encode(), ctypes.byref(c)) == 0
+    m = gk.Csr()
+    h = ctypes.c_void_p()
+    assert L.gk_graph_map_bin(path.encode(), 1, ctypes.byref(m), ctypes.byref(h)) == 0
+    assert (m.n, m.nnz) == (3, 4) and not m.adjwgt and not m.vwgt
+    assert list(np.ctypeslib.as_array(m.adjncy, (4,))) == [1, 0, 2, 1]
+    L.gk_graph_unmap(h)
+    r = gk.read_metis(path)  # gk_graph_read fills unit weights
+    assert list(r.adjwgt) == [1, 1, 1, 1] and list(r.vwgt) == [1, 1, 1]
+
+
+def test_binary_csr_rejects_damage(gk, tmp_path):
+    g = gk.delaunay(500, 3)
+    path = str(tmp_path / "d.gkcsr")
+    gk.write_bin(g, path)
+    raw = bytearray(open(path, "rb").read())
+    flipped = bytearray(raw)
+    adj_off = (64 + 4 * (g.n + 1) + 63) // 64 * 64  # sections start on 64-byte boundaries
+    flipped[adj_off + 13] ^= 0x40  # inside adjncy
+    open(path, "wb").write(bytes(flipped))
+    with pytest.raises(RuntimeError, match="checksum"):
+        gk.map_bin(path)
+    gk.map_bin(path, verify=False)  # unchecked mapping still works
+    open(path, "wb").write(bytes(raw[: len(raw) - 40]))
+    with pytest.raises(RuntimeError, match="truncated"):
+        gk.map_bin(path)
+    bad = bytearray(raw)
+    bad[0] = ord("X")
+    open(path, "wb").write(bytes(bad))
+    with pytest.raises(RuntimeError, match="magic"):
+        gk.map_bin(path)
+    with pytest.raises(RuntimeError):
+        gk.map_bin(str(tmp_path / "missing.gkcsr"))
+
+
 def test_write_partition_format(gk, tmp_path):
     part = np.array([3, 0, 2, 1], np.int32)
     path = str(tmp_path / "x.out")
