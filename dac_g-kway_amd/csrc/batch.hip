@@ -449,27 +449,38 @@ __global__ void __launch_bounds__(256) k_select_long(int32_t n, const uint64_t* 
                                                      uint64_t* __restrict__ selbits, int32_t* __restrict__ bsum) {
   const int lane = lane_id();
   const int64_t nwords = (int64_t(n) + 63) >> 6;
-  const int64_t nw = int64_t(gridDim.x) * 4;
-  for (int64_t w = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); w < nwords; w += nw) {
-    uint64_t lb = longbits[w];
-    while (lb) {
-      const int bit = __ffsll(static_cast<long long>(lb)) - 1;
-      lb &= lb - 1;
-      const int32_t v = int32_t((w << 6) + bit);
-      const int32_t s = xadj[v], e = xadj[v + 1];
-      bool blocked = false;
-      for (int32_t x0 = s; x0 < e && !blocked; x0 += 64) {
-        const int32_t x = x0 + lane;
-        bool b = false;
-        if (x < e) {
-          const int32_t u = adj[x];
-          b = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
+  const int64_t nwaves = int64_t(gridDim.x) * 4;
+  const int64_t wave = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  // 64 words per wave and step, one per lane, interleaved over the waves (so
+  // dense runs of long rows spread over many waves); then the non-empty ones
+  for (int64_t base = wave; base < nwords; base += nwaves * 64) {
+    const int64_t wl = base + int64_t(lane) * nwaves;
+    const uint64_t mine = wl < nwords ? longbits[wl] : 0;
+    uint64_t nz = ballot(mine != 0);
+    while (nz) {
+      const int src = __ffsll(static_cast<long long>(nz)) - 1;
+      nz &= nz - 1;
+      const int64_t w = base + int64_t(src) * nwaves;
+      uint64_t lb = __shfl(mine, src, 64);
+      while (lb) {
+        const int bit = __ffsll(static_cast<long long>(lb)) - 1;
+        lb &= lb - 1;
+        const int32_t v = int32_t((w << 6) + bit);
+        const int32_t s = xadj[v], e = xadj[v + 1];
+        bool blocked = false;
+        for (int32_t x0 = s; x0 < e && !blocked; x0 += 64) {
+          const int32_t x = x0 + lane;
+          bool b = false;
+          if (x < e) {
+            const int32_t u = adj[x];
+            b = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
+          }
+          blocked = ballot(b) != 0;
         }
-        blocked = ballot(b) != 0;
-      }
-      if (!blocked && lane == 0) {
-        atomicOr(reinterpret_cast<unsigned long long*>(&selbits[w]), 1ull << bit);
-        atomicAdd(&bsum[w / kSelectBlockWords], 1);
+        if (!blocked && lane == 0) {
+          atomicOr(reinterpret_cast<unsigned long long*>(&selbits[w]), 1ull << bit);
+          atomicAdd(&bsum[w / kSelectBlockWords], 1);
+        }
       }
     }
   }
@@ -541,7 +552,7 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain
   k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable, reset, active);
   k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, gain, thr, sb.movable, sb.selbits, sb.defbits,
                                       sb.longbits, sb.scratch);
-  unsigned lgrid = grid_for(nwords * 64, 256);
+  unsigned lgrid = grid_for(nwords, 256);  // up to a wave per 64 words' worth of lanes
   if (lgrid > 1024) lgrid = 1024;
   k_select_long<<<lgrid, 256, 0, st>>>(g.n, sb.longbits, g.xadj, g.adj, sb.movable, sb.selbits, sb.scratch);
   k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list);
