@@ -337,7 +337,9 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   // ---------------------------------------------------------- uncoarsening
   // Rounds run back to back on the device (select_list, round_b with one
   // job, gain passes); the host reads the last round's move count every
-  // kRoundsPerCheck rounds.  A round with nothing selected changes nothing.
+  // kRoundsPerCheck rounds, every 8, 16, then 32 once a level has run 32
+  // (a check costs a copy and a host sync).  A round with nothing selected
+  // changes nothing.
   int32_t* cur_part = partA.p;
   int32_t* cur_bnd = bndA.p;
   int32_t* oth_part = partB.p;
@@ -426,8 +428,9 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       // A round updates at most M movers and their neighbours.
       const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
       const int64_t wl_items = std::min<int64_t>(g.n, int64_t(M) * (1 + 2 * avg_deg));
+      int per_check = kRoundsPerCheck, ran = 0;
       while (true) {
-        for (int r = 0; r < kRoundsPerCheck; ++r) {
+        for (int r = 0; r < per_check; ++r) {
           GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
                    gkd::launch_select_list(g, nullptr, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
                                            hcnt.p, active.p, st));
@@ -443,9 +446,11 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
                      gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p,
                                             cand.p, st));
         }
+        ran += per_check;
         int32_t live = 0;
         GK_TRY(fetch(ctx, st, {{&live, active.p, sizeof live}}));
         if (live == 0) break;
+        if (ran >= 32 && per_check < 32) per_check *= 2;
       }
       long long hcut = 0, hmoved = 0;
       int32_t hrounds = 0;

@@ -630,7 +630,8 @@ int Engine::refine_step(bool* worked) {
   const int64_t avg_deg = N > 0 ? (int64_t(g.nnz) + N - 1) / N : 0;
   const int64_t wl_items = std::min<int64_t>(N, int64_t(J) * M * (1 + 2 * avg_deg));
   // Rounds run back to back on the device; the host reads the last round's
-  // move count every kRoundsPerCheck rounds.  A round in which no job selects
+  // move count every kRoundsPerCheck rounds (every 8, 16, then 32 once the
+  // step has run 32: a check costs a copy and a host sync).  A round in which no job selects
   // a move changes nothing, so rounds past convergence are harmless.  After
   // kMaxRounds rounds the step ends; jobs still moving resume their level in
   // the next step, so one slow job does not hold back the others.  The fixed
@@ -641,9 +642,12 @@ int Engine::refine_step(bool* worked) {
   const int64_t pack_items = int64_t(N) + int64_t(g.nnz);
   const int kMaxRounds = static_cast<int>(std::min<int64_t>(256, 16 * std::max<int64_t>(1, pack_items >> 24)));
   const double tr0 = now_ms();
-  for (int done_rounds = 0; done_rounds < kMaxRounds; done_rounds += kRoundsPerCheck) {
-    rounds_ += kRoundsPerCheck;
-    for (int r = 0; r < kRoundsPerCheck; ++r) {
+  int done_rounds = 0, per_check = kRoundsPerCheck;
+  while (done_rounds < kMaxRounds) {
+    const int nr = std::min(per_check, kMaxRounds - done_rounds);
+    rounds_ += nr;
+    done_rounds += nr;
+    for (int r = 0; r < nr; ++r) {
       GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
                gkd::launch_select_list(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
                                        hcnt.p, active.p, st_));
@@ -662,6 +666,7 @@ int Engine::refine_step(bool* worked) {
     int32_t live = 0;
     GK_TRY(fetch(ctx_, st_, {{&live, active.p, sizeof live}}));
     if (live == 0) break;
+    if (done_rounds >= 32 && per_check < 32) per_check *= 2;
   }
   prof_ms_[5] += now_ms() - tr0;
   std::vector<long long> hcut(J), hmoved(J);
