@@ -317,12 +317,32 @@ __global__ void __launch_bounds__(256) k_contract_rows(const int32_t* __restrict
   const int sub = static_cast<int>(gt % S);
   if (ri >= nrows) return;
   const int32_t c = rows[ri];
-  // Locate this lane's entry in the concatenation of the member rows.
+  // Locate this lane's entry in the concatenation of the member rows: lane m
+  // loads member m's row start and length, a shuffle scan gives the offsets,
+  // and each lane finds its member among them (two dependent loads instead of
+  // two per member).  More members than lanes (max_group > S): member by member.
   int32_t j = -1;
-  {
+  const int32_t m0 = cptr[c], nm = cptr[c + 1] - m0;
+  if (nm <= S) {
+    int32_t ms = 0, md = 0;
+    if (sub < nm) {
+      const int32_t v = order[m0 + sub];
+      ms = xadj[v];
+      md = xadj[v + 1] - ms;
+    }
+    int32_t inc = md;
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+      const int32_t y = __shfl_up(inc, o, S);
+      if (sub >= o) inc += y;
+    }
+    for (int m = 0; m < nm; ++m) {
+      const int32_t im = __shfl(inc, m, S), sm = __shfl(ms, m, S), dm = __shfl(md, m, S);
+      if (sub < im && sub >= im - dm) j = sm + (sub - (im - dm));
+    }
+  } else {
     int32_t pre = 0;
-    const int32_t m1 = cptr[c + 1];
-    for (int32_t m = cptr[c]; m < m1; ++m) {
+    for (int32_t m = m0; m < m0 + nm; ++m) {
       const int32_t v = order[m];
       const int32_t s = xadj[v];
       const int32_t d = xadj[v + 1] - s;
