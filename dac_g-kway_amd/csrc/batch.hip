@@ -544,7 +544,7 @@ int select_scratch_ints(int64_t n) {
 int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
                        const int32_t* pw, const int32_t* pw_off, const long long* cap, const int32_t* thr,
                        const uint64_t* cand, const SelectBufs& sb, int32_t* reset, int32_t* active,
-                       hipStream_t st) {
+                       int32_t max_deg, hipStream_t st) {
   if (g.n <= 0) return 0;
   const int64_t nwords = (int64_t(g.n) + 63) / 64;
   const int64_t waves = (nwords + kSelectWords - 1) / kSelectWords;
@@ -552,9 +552,11 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain
   k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable, reset, active);
   k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, gain, thr, sb.movable, sb.selbits, sb.defbits,
                                       sb.longbits, sb.scratch);
-  unsigned lgrid = grid_for(nwords, 256);  // up to a wave per 64 words' worth of lanes
-  if (lgrid > 1024) lgrid = 1024;
-  k_select_long<<<lgrid, 256, 0, st>>>(g.n, sb.longbits, g.xadj, g.adj, sb.movable, sb.selbits, sb.scratch);
+  if (max_deg > kSelectShortRow) {  // no long rows at all: nothing to sweep
+    unsigned lgrid = grid_for(nwords, 256);  // up to a wave per 64 words' worth of lanes
+    if (lgrid > 1024) lgrid = 1024;
+    k_select_long<<<lgrid, 256, 0, st>>>(g.n, sb.longbits, g.xadj, g.adj, sb.movable, sb.selbits, sb.scratch);
+  }
   k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list);
   return hip_status();
 }

@@ -433,7 +433,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
         for (int r = 0; r < per_check; ++r) {
           GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
                    gkd::launch_select_list(g, nullptr, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
-                                           hcnt.p, active.p, st));
+                                           hcnt.p, active.p, maxdeg, st));
           GK_STAGE(ctx, GK_ST_BALANCE, 0,
                    gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p,
                                        pw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p,

@@ -650,7 +650,7 @@ int Engine::refine_step(bool* worked) {
     for (int r = 0; r < nr; ++r) {
       GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
                gkd::launch_select_list(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
-                                       hcnt.p, active.p, st_));
+                                       hcnt.p, active.p, maxdeg, st_));
       GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
                gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
                                    dpw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
