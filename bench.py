@@ -63,6 +63,12 @@ def stage_table(gk, stats_list, steps, log_it=True):
     return stage
 
 
+def launch_flags(gk, args):
+    """Flags that choose launch mechanics only (results are identical): rounds
+    replayed from a hipGraph, launch shapes picked by the autotuner."""
+    return (gk.GK_FLAG_GRAPH_ROUNDS if args.graph_rounds else 0) | (gk.GK_FLAG_AUTOTUNE if args.autotune else 0)
+
+
 def dominant_stage(gk, L, ctx, stage_step, off):
     """Runs one step with every stage timed, returns the stage with the most
     device time and restricts the context's stage timing to it."""
@@ -132,8 +138,9 @@ def run_batch(args, gk, torch):
     pk.v_off = v_off.ctypes.data_as(I32P)
     pk.e_off = e_off.ctypes.data_as(I32P)
     G = len(graphs)
-    prm = (gk.Params * G)(*[gk.params(k, flags=0 if args.no_stage_time else gk.GK_FLAG_STAGE_TIME) for k in ks])
-    prm_stage = (gk.Params * G)(*[gk.params(k, flags=gk.GK_FLAG_STAGE_TIME) for k in ks])
+    prm = (gk.Params * G)(*[gk.params(k, flags=launch_flags(gk, args) | (0 if args.no_stage_time else gk.GK_FLAG_STAGE_TIME))
+                            for k in ks])
+    prm_stage = (gk.Params * G)(*[gk.params(k, flags=launch_flags(gk, args) | gk.GK_FLAG_STAGE_TIME) for k in ks])
     L = gk.lib()
     ctx = gk.Context(0)
 
@@ -244,7 +251,9 @@ def run_batch(args, gk, torch):
         "config": {"workload": f"packed batch of {G} graphs (seed {args.seed}): n log-uniform in [500, 50000], "
                                f"even = Delaunay, odd = layered DAG; k in {{2,4,8,16,32}}",
                    "n_total": int(pg.n), "m_total": int(m_tot), "eps": 0.03,
-                   "window": "device packed CSR -> device partitions"},
+                   "window": "device packed CSR -> device partitions",
+                   "rounds": "hipGraph replay" if args.graph_rounds else "stream launches",
+                   "autotune": bool(args.autotune)},
         "plain_path_edges_per_s": round(plain_value, 1),
         "speedup_vs_plain": round(value / plain_value, 2),
         "worst_imbalance": round(worst_imb, 5),
@@ -406,6 +415,10 @@ def main():
     ap.add_argument("--check", type=int, default=24, help="serve form: requests re-run alone for bit-exactness")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline + bit-exact check)")
     ap.add_argument("--no-stage-time", action="store_true")
+    ap.add_argument("--graph-rounds", action="store_true",
+                    help="replay each level's refinement rounds from a captured hipGraph")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time candidate launch shapes of matching and contraction on first use")
     ap.add_argument("--stage-steps", type=int, default=3,
                     help="extra steps with per-stage HIP events for the stage table and roofline")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -454,8 +467,8 @@ def main():
     # steps therefore time only the dominant stage (the roofline kernel),
     # picked from one all-stage step before them; the full stage table comes
     # from separate steps after them.
-    p = gk.params(args.k, flags=0 if args.no_stage_time else gk.GK_FLAG_STAGE_TIME)
-    p_stage = gk.params(args.k, flags=gk.GK_FLAG_STAGE_TIME)
+    p = gk.params(args.k, flags=launch_flags(gk, args) | (0 if args.no_stage_time else gk.GK_FLAG_STAGE_TIME))
+    p_stage = gk.params(args.k, flags=launch_flags(gk, args) | gk.GK_FLAG_STAGE_TIME)
 
     def step(pp=p):
         st = gk.Stats()
@@ -537,7 +550,9 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded generator)",
         "config": {"workload": f"{args.workload} n={g.n} m={m} seed={args.seed}", "k": args.k,
-                   "eps": 0.03, "window": "device CSR -> device partition"},
+                   "eps": 0.03, "window": "device CSR -> device partition",
+                   "rounds": "hipGraph replay" if args.graph_rounds else "stream launches",
+                   "autotune": bool(args.autotune)},
         "cut": int(st.cut),
         "imbalance": round(st.imbalance, 5),
         "levels": int(st.nlevels),

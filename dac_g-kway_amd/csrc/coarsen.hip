@@ -11,11 +11,14 @@
 
 namespace gkd {
 
+// About one lane per two to four entries: measured on MI355X, 4 lanes beat 8
+// on meshes of average degree 4.5-6 (up to 24% at 2^24 vertices), 8 beat 16
+// at degree 7-16 and 16 beat 32 at degree 16-22.
 int lanes_for_degree(double avg) {
-  if (avg <= 3.0) return 4;
-  if (avg <= 6.5) return 8;
-  if (avg <= 14.0) return 16;
-  if (avg <= 28.0) return 32;
+  if (avg <= 8.0) return 4;
+  if (avg <= 16.0) return 8;
+  if (avg <= 32.0) return 16;
+  if (avg <= 64.0) return 32;
   return 64;
 }
 
@@ -25,11 +28,11 @@ static inline int hip_status() { return hipGetLastError() == hipSuccess ? 0 : GK
 // matching_with_score (gkway/coarsen.hpp:39-78) with S lanes per vertex: the
 // lanes stride over the row, then reduce the packed preference
 // (weight desc, neighbour degree asc, neighbour id asc) by xor-shuffles.
-template <int S>
-__global__ void __launch_bounds__(256) k_match(int32_t n, const int32_t* __restrict__ xadj,
-                                               const int32_t* __restrict__ adj,
-                                               const int32_t* __restrict__ w, int32_t* __restrict__ cand) {
-  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
+template <int S, int B>
+__global__ void __launch_bounds__(B) k_match(int32_t n, const int32_t* __restrict__ xadj,
+                                             const int32_t* __restrict__ adj,
+                                             const int32_t* __restrict__ w, int32_t* __restrict__ cand) {
+  const int64_t gt = int64_t(blockIdx.x) * B + threadIdx.x;
   const int32_t v = static_cast<int32_t>(gt / S);
   const int sub = static_cast<int>(gt % S);
   if (v >= n) return;
@@ -49,17 +52,26 @@ __global__ void __launch_bounds__(256) k_match(int32_t n, const int32_t* __restr
   if (sub == 0) cand[v] = bu;
 }
 
-int launch_match(const DGraph& g, int lanes, int32_t* cand, hipStream_t st) {
-  if (g.n <= 0) return 0;
-  const unsigned grid = grid_for(int64_t(g.n) * lanes, 256);
+template <int B>
+static void match_block(const DGraph& g, int lanes, int32_t* cand, hipStream_t st) {
+  const unsigned grid = grid_for(int64_t(g.n) * lanes, B);
   switch (lanes) {
-    case 1: k_match<1><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 2: k_match<2><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 4: k_match<4><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 8: k_match<8><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 16: k_match<16><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 32: k_match<32><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    default: k_match<64><<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 1: k_match<1, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 2: k_match<2, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 4: k_match<4, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 8: k_match<8, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 16: k_match<16, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 32: k_match<32, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    default: k_match<64, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+  }
+}
+
+int launch_match(const DGraph& g, int lanes, int block, int32_t* cand, hipStream_t st) {
+  if (g.n <= 0) return 0;
+  switch (block) {
+    case 128: match_block<128>(g, lanes, cand, st); break;
+    case 512: match_block<512>(g, lanes, cand, st); break;
+    default: match_block<256>(g, lanes, cand, st); break;
   }
   return hip_status();
 }
@@ -301,8 +313,8 @@ int launch_contract_bounds(const DGraph& g, int32_t nc, const int32_t* cptr, con
 // Rows whose upper bound fits S lanes: one entry per lane, bitonic sort by
 // coarse neighbour id across the S lanes, segmented sum of equal neighbours,
 // self entries dropped, run tails compacted with a ballot.
-template <int S>
-__global__ void __launch_bounds__(256) k_contract_rows(const int32_t* __restrict__ rows, int32_t nrows,
+template <int S, int B>
+__global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__ rows, int32_t nrows,
                                                        const int32_t* __restrict__ cptr,
                                                        const int32_t* __restrict__ order,
                                                        const int32_t* __restrict__ xadj,
@@ -312,7 +324,7 @@ __global__ void __launch_bounds__(256) k_contract_rows(const int32_t* __restrict
                                                        const int32_t* __restrict__ toff,
                                                        int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
                                                        int32_t* __restrict__ len) {
-  const int64_t gt = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t gt = int64_t(blockIdx.x) * B + threadIdx.x;
   const int64_t ri = gt / S;
   const int sub = static_cast<int>(gt % S);
   if (ri >= nrows) return;
@@ -386,16 +398,28 @@ __global__ void __launch_bounds__(256) k_contract_rows(const int32_t* __restrict
   if (sub == 0) len[c] = popc64(seg);
 }
 
-int launch_contract_rows(const DGraph& g, int lanes, const int32_t* rows, int32_t nrows,
+template <int B>
+static void contract_rows_block(const DGraph& g, int lanes, const int32_t* rows, int32_t nrows,
+                                const int32_t* cptr, const int32_t* order, const int32_t* cmap,
+                                const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len, hipStream_t st) {
+  const unsigned grid = grid_for(int64_t(nrows) * lanes, B);
+  switch (lanes) {
+    case 16: k_contract_rows<16, B><<<grid, B, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    case 32: k_contract_rows<32, B><<<grid, B, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    default: k_contract_rows<64, B><<<grid, B, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+  }
+}
+
+int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, int32_t nrows,
                          const int32_t* cptr, const int32_t* order, const int32_t* cmap,
                          const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len,
                          hipStream_t st) {
   if (nrows <= 0) return 0;
-  const unsigned grid = grid_for(int64_t(nrows) * lanes, 256);
-  switch (lanes) {
-    case 16: k_contract_rows<16><<<grid, 256, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
-    case 32: k_contract_rows<32><<<grid, 256, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
-    default: k_contract_rows<64><<<grid, 256, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+  switch (block) {
+    case 64: contract_rows_block<64>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
+    case 128: contract_rows_block<128>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
+    case 512: contract_rows_block<512>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
+    default: contract_rows_block<256>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
   }
   return hip_status();
 }

@@ -34,6 +34,11 @@ using namespace gkrt;
 
 namespace gkrt {
 
+// Smallest launches worth timing candidates for: matching over at least
+// 1 Mi entries, a contraction row class of at least 16 Ki rows.
+constexpr int64_t kTuneMinEntries = int64_t(1) << 20;
+constexpr int32_t kTuneMinRows = 1 << 14;
+
 // ---------------------------------------------------------------- stages
 // Grouping: pointer jumping to the root pair, then the component maximum.
 // Returns label/dist in the given buffers, the jump rounds in *rounds.
@@ -136,9 +141,26 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
   GK_TRY(len.alloc(P, nc));
   const int lanes[3] = {16, 32, 64};
   ctx->clock.begin(GK_ST_CONTRACT);
-  for (int cls = 0; cls < 3; ++cls)
-    GK_TRY(gkd::launch_contract_rows(g, lanes[cls], lists.p + int64_t(cls) * nc, hc[cls], cptr, order,
-                                     cmap, toff.p, tadj.p, tw.p, len.p, st));
+  for (int cls = 0; cls < 3; ++cls) {
+    const int32_t* rows = lists.p + int64_t(cls) * nc;
+    auto run = [&](const Tuner::Shape& x) {
+      return gkd::launch_contract_rows(g, x.lanes, x.block, rows, hc[cls], cptr, order, cmap, toff.p, tadj.p,
+                                       tw.p, len.p, st);
+    };
+    Tuner::Shape shape{lanes[cls], 256};
+    if (ctx->autotune && hc[cls] >= kTuneMinRows) {
+      // Block size per row class; the lanes of a class are fixed by its rows.
+      const uint64_t key = Tuner::key(1 + cls, g.n, g.nnz);
+      if (const Tuner::Shape* s = ctx->tuner.find(key)) {
+        shape = *s;
+      } else {
+        const int L = lanes[cls];
+        GK_TRY(ctx->tuner.tune(key, {{L, 256}, {L, 64}, {L, 128}, {L, 512}}, st, run, &shape));
+        continue;  // every candidate wrote the rows
+      }
+    }
+    GK_TRY(run(shape));
+  }
   ctx->clock.end(GK_ST_CONTRACT);
   count_stage(ctx, GK_ST_CONTRACT, 0);  // bytes added once the coarse entry count is known
   if (hc[3] > 0) {
@@ -190,6 +212,26 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
   return GK_OK;
 }
 
+int match_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t* cand) {
+  const int def = gkd::lanes_for_degree(g.n ? double(g.nnz) / g.n : 0.0);
+  Tuner::Shape shape{def, 256};
+  auto run = [&](const Tuner::Shape& x) { return gkd::launch_match(g, x.lanes, x.block, cand, st); };
+  if (ctx->autotune && g.nnz >= kTuneMinEntries) {
+    const uint64_t key = Tuner::key(0, g.n, g.nnz);
+    if (const Tuner::Shape* s = ctx->tuner.find(key)) {
+      shape = *s;
+    } else {
+      // Vertices per wavefront around the degree table's choice, three block sizes.
+      std::vector<Tuner::Shape> c;
+      for (int l : {def, def / 2, def * 2})
+        if (l >= 1 && l <= 64)
+          for (int b : {256, 128, 512}) c.push_back({l, b});
+      return ctx->tuner.tune(key, c, st, run, &shape);  // every candidate wrote cand
+    }
+  }
+  return run(shape);
+}
+
 int gain_lanes(const DGraph& g) {
   const double avg = g.n > 0 ? double(g.nnz) / g.n : 0.0;
   int s = 4;
@@ -213,9 +255,14 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   std::memset(&S, 0, sizeof S);
   ctx->cur = &S;
   ctx->clock.start((prm.flags & GK_FLAG_STAGE_TIME) != 0, st);
+  ctx->autotune = (prm.flags & GK_FLAG_AUTOTUNE) != 0;
   struct Reset {
     gk_ctx* c;
-    ~Reset() { c->cur = nullptr; c->clock.start(false, nullptr); }
+    ~Reset() {
+      c->cur = nullptr;
+      c->clock.start(false, nullptr);
+      c->autotune = false;
+    }
   } reset{ctx};
 
   std::vector<Level> L;
@@ -260,7 +307,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(label.alloc(P, n));
     GK_TRY(cmap.alloc(P, n));
     GK_STAGE(ctx, GK_ST_MATCH, 16LL * f.g.nnz + 8LL * n + 4,
-             gkd::launch_match(f.g, gkd::lanes_for_degree(n ? double(f.g.nnz) / n : 0.0), cand.p, st));
+             match_stage(ctx, st, f.g, cand.p));
     int32_t rounds = 0;
     GK_TRY(group_stage(ctx, st, n, cand.p, label.p, dist, &rounds));
     int32_t nc = 0;
@@ -428,23 +475,38 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       // A round updates at most M movers and their neighbours.
       const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
       const int64_t wl_items = std::min<int64_t>(g.n, int64_t(M) * (1 + 2 * avg_deg));
+      auto one_round = [&]() -> int {
+        GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
+                 gkd::launch_select_list(g, nullptr, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
+                                         hcnt.p, active.p, maxdeg, st));
+        GK_STAGE(ctx, GK_ST_BALANCE, 0,
+                 gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p, pw.p,
+                                     dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
+                                     lastcnt.p, active.p, cand.p, dthr.p, st));
+        GK_STAGE(ctx, GK_ST_GAIN, 0,
+                 gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
+                                     work.p, wl_items, cand.p, st));
+        if (heavy_rows)
+          GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
+                   gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p,
+                                          cand.p, st));
+        return GK_OK;
+      };
+      // GK_FLAG_GRAPH_ROUNDS: kRoundsPerCheck rounds are captured once per
+      // level and every check interval (a multiple of it) replays them.
+      RoundGraph rg;
+      if ((prm.flags & GK_FLAG_GRAPH_ROUNDS) && !ctx->clock.timed(GK_ST_SELECT) &&
+          !ctx->clock.timed(GK_ST_BALANCE) && !ctx->clock.timed(GK_ST_GAIN) && !ctx->clock.timed(GK_ST_GAIN_HEAVY))
+        GK_TRY(rg.capture(ctx, st, [&]() -> int {
+          for (int r = 0; r < kRoundsPerCheck; ++r) GK_TRY(one_round());
+          return GK_OK;
+        }));
       int per_check = kRoundsPerCheck, ran = 0;
       while (true) {
-        for (int r = 0; r < per_check; ++r) {
-          GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
-                   gkd::launch_select_list(g, nullptr, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
-                                           hcnt.p, active.p, maxdeg, st));
-          GK_STAGE(ctx, GK_ST_BALANCE, 0,
-                   gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p,
-                                       pw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p,
-                                       hcnt.p, lastcnt.p, active.p, cand.p, dthr.p, st));
-          GK_STAGE(ctx, GK_ST_GAIN, 0,
-                   gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p,
-                                       dest.p, work.p, wl_items, cand.p, st));
-          if (heavy_rows)
-            GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
-                     gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p,
-                                            cand.p, st));
+        if (rg.ready()) {
+          for (int r = 0; r < per_check; r += kRoundsPerCheck) GK_TRY(rg.replay(ctx, st));
+        } else {
+          for (int r = 0; r < per_check; ++r) GK_TRY(one_round());
         }
         ran += per_check;
         int32_t live = 0;
@@ -682,7 +744,7 @@ int gk_k_match(gk_ctx* ctx, const gk_csr* dg, int32_t* cand, void* stream) {
   DeviceGuard guard(ctx->device);
   hipStream_t st = pick(ctx, stream);
   DGraph g = dview(dg);
-  GK_TRY(gkd::launch_match(g, gkd::lanes_for_degree(g.n ? double(g.nnz) / g.n : 0.0), cand, st));
+  GK_TRY(gkd::launch_match(g, gkd::lanes_for_degree(g.n ? double(g.nnz) / g.n : 0.0), 256, cand, st));
   GK_HIP(hipStreamSynchronize(st));
   return GK_OK;
 }

@@ -288,7 +288,7 @@ int Engine::coarsen_step(bool* worked) {
   GK_TRY(label.alloc(P_, n));
   GK_TRY(cmap->alloc(P_, n));
   GK_STAGE(ctx_, GK_ST_MATCH, 16LL * g.nnz + 8LL * n + 4,
-           gkd::launch_match(g, gkd::lanes_for_degree(n ? double(g.nnz) / n : 0.0), cand.p, st_));
+           match_stage(ctx_, st_, g, cand.p));
   int32_t rounds = 0;
   GK_TRY(group_stage(ctx_, st_, n, cand.p, label.p, dist, &rounds));
   int32_t nc = 0;
@@ -641,27 +641,43 @@ int Engine::refine_step(bool* worked) {
   constexpr int kRoundsPerCheck = 4;
   const int64_t pack_items = int64_t(N) + int64_t(g.nnz);
   const int kMaxRounds = static_cast<int>(std::min<int64_t>(256, 16 * std::max<int64_t>(1, pack_items >> 24)));
+  auto one_round = [&]() -> int {
+    GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
+             gkd::launch_select_list(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
+                                     hcnt.p, active.p, maxdeg, st_));
+    GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
+             gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
+                                 dpw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
+                                 lastcnt.p, active.p, cand.p, dthr.p, st_));
+    GK_STAGE(ctx_, GK_ST_GAIN, 0,
+             gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
+                                 nullptr, wl_items, cand.p, st_));
+    if (heavy_rows)
+      GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
+               gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
+                                      cand.p, st_));
+    return GK_OK;
+  };
+  // With GK_FLAG_GRAPH_ROUNDS (and no round stage event-timed), the
+  // kRoundsPerCheck rounds are captured once and every check interval
+  // replays that graph (intervals are multiples of it).
+  RoundGraph rg;
+  if (graph_rounds_ && !ctx_->clock.timed(GK_ST_SELECT) && !ctx_->clock.timed(GK_ST_BALANCE) &&
+      !ctx_->clock.timed(GK_ST_GAIN) && !ctx_->clock.timed(GK_ST_GAIN_HEAVY))
+    GK_TRY(rg.capture(ctx_, st_, [&]() -> int {
+      for (int r = 0; r < kRoundsPerCheck; ++r) GK_TRY(one_round());
+      return GK_OK;
+    }));
   const double tr0 = now_ms();
   int done_rounds = 0, per_check = kRoundsPerCheck;
   while (done_rounds < kMaxRounds) {
     const int nr = std::min(per_check, kMaxRounds - done_rounds);
     rounds_ += nr;
     done_rounds += nr;
-    for (int r = 0; r < nr; ++r) {
-      GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
-               gkd::launch_select_list(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
-                                       hcnt.p, active.p, maxdeg, st_));
-      GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
-               gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
-                                   dpw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                   lastcnt.p, active.p, cand.p, dthr.p, st_));
-      GK_STAGE(ctx_, GK_ST_GAIN, 0,
-               gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                                   nullptr, wl_items, cand.p, st_));
-      if (heavy_rows)
-        GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
-                 gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
-                                        cand.p, st_));
+    if (rg.ready() && nr % kRoundsPerCheck == 0) {
+      for (int r = 0; r < nr; r += kRoundsPerCheck) GK_TRY(rg.replay(ctx_, st_));
+    } else {
+      for (int r = 0; r < nr; ++r) GK_TRY(one_round());
     }
     int32_t live = 0;
     GK_TRY(fetch(ctx_, st_, {{&live, active.p, sizeof live}}));
@@ -864,11 +880,13 @@ int run_packed(gk_ctx* ctx, hipStream_t st, const gk_packed_csr* pk, const gk_pa
   std::memset(&SB, 0, sizeof SB);
   ctx->cur = &SB;
   ctx->clock.start((prm[0].flags & GK_FLAG_STAGE_TIME) != 0, st);
+  ctx->autotune = (prm[0].flags & GK_FLAG_AUTOTUNE) != 0;
   struct Reset {
     gk_ctx* c;
     ~Reset() {
       c->cur = nullptr;
       c->clock.start(false, nullptr);
+      c->autotune = false;
     }
   } reset{ctx};
   // Level 0 of every graph is a slice of the caller's pack.
@@ -893,6 +911,7 @@ int run_packed(gk_ctx* ctx, hipStream_t st, const gk_packed_csr* pk, const gk_pa
     S0->vw = S0->bv.p;
   }
   Engine E(ctx, st, batch_workers(prm[0]));
+  E.set_graph_rounds((prm[0].flags & GK_FLAG_GRAPH_ROUNDS) != 0);
   GK_TRY(E.upload(S0->dv_off, S0->v_off.data(), S0->v_off.size()));
   std::vector<long long> T(G);
   {

@@ -144,6 +144,8 @@ class Engine {
     return GK_OK;
   }
   int download(std::initializer_list<Fetch> items);
+  // Refinement rounds replayed from a captured hipGraph (GK_FLAG_GRAPH_ROUNDS).
+  void set_graph_rounds(bool on) { graph_rounds_ = on; }
 
  private:
   static StoreP whole_store(const std::vector<const JLevel*>& lv);
@@ -165,6 +167,7 @@ class Engine {
   Pinned& up_;  // the context's staging arenas
   Pinned& dl_;
   size_t ndone_ = 0;
+  bool graph_rounds_ = false;
   int64_t steps_ = 0, rounds_ = 0;
   double prof_ms_[6] = {0, 0, 0, 0, 0, 0};  // host wall time per step phase
   std::vector<std::unique_ptr<Job>> jobs_;

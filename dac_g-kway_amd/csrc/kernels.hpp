@@ -20,7 +20,9 @@ struct DGraph {
 int lanes_for_degree(double avg_deg);
 
 // ---- coarsening (coarsen.hip)
-int launch_match(const DGraph& g, int lanes, int32_t* cand, hipStream_t st);
+// lanes per vertex (1..64) and block size (128, 256 or 512 threads) change
+// only the launch shape, never the result.
+int launch_match(const DGraph& g, int lanes, int block, int32_t* cand, hipStream_t st);
 // Pointer jumping over the candidate pseudo-forest.
 int launch_jump_init(int32_t n, const int32_t* cand, int32_t* next, int32_t* dist, hipStream_t st);
 // One Jacobi round; sets flags[round] = 1 when any pointer moved.  Skips all
@@ -45,8 +47,9 @@ int launch_cluster(int32_t n, const uint64_t* skeys, const int32_t* sids, int sb
 int launch_contract_bounds(const DGraph& g, int32_t nc, const int32_t* cptr, const int32_t* order,
                            int32_t* ub, int32_t* toff, int32_t* lists, int32_t* list_cnt,
                            int32_t* scratch, hipStream_t st);
-// Phase 2 for the rows of one class (lanes in {16, 32, 64}).
-int launch_contract_rows(const DGraph& g, int lanes, const int32_t* rows, int32_t nrows,
+// Phase 2 for the rows of one class (lanes in {16, 32, 64}); block size 64,
+// 128, 256 or 512 threads (launch shape only).
+int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, int32_t nrows,
                          const int32_t* cptr, const int32_t* order, const int32_t* cmap,
                          const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len,
                          hipStream_t st);

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -143,6 +146,7 @@ class StageClock {
     used_ = 0;
   }
   void set_mask(uint32_t m) { mask_ = m; }
+  bool timed(int id) const { return on_ && ((mask_ >> id) & 1u); }
   void begin(int id) {
     if (!on_ || !((mask_ >> id) & 1u)) return;
     cur_[id] = take();
@@ -208,6 +212,107 @@ class Pinned {
 
 }  // namespace gkrt
 
+namespace gkrt {
+// Launch shapes that do not change results (lanes per vertex or row, block
+// size), chosen by timing.  A key names a kernel family and a level's degree
+// profile: ceil(log2 n) and the average degree in half-octaves.  The first
+// launch with a new key warms the device, then times every candidate shape on
+// the real data, twice (each run writes the same output), and the fastest is
+// kept for the life of the context.
+class Tuner {
+ public:
+  struct Shape {
+    int lanes;
+    int block;
+  };
+  Tuner() = default;
+  Tuner(const Tuner&) = delete;
+  Tuner& operator=(const Tuner&) = delete;
+  ~Tuner() {
+    for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+  }
+  static uint64_t key(int family, int64_t n, int64_t nnz) {
+    int ln = 0;
+    while (ln < 62 && (int64_t(1) << ln) < n) ++ln;
+    const double avg = n > 0 ? double(nnz) / double(n) : 0.0;
+    const int la = avg > 1.0 ? static_cast<int>(std::floor(2.0 * std::log2(avg))) : 0;
+    return (uint64_t(family) << 32) | (uint64_t(ln) << 16) | uint64_t(la);
+  }
+  const Shape* find(uint64_t key) const {
+    auto it = best_.find(key);
+    return it == best_.end() ? nullptr : &it->second;
+  }
+  int tunings() const { return tunings_; }
+  // launch(shape) enqueues one run on st and returns a status.  Ties go to
+  // the earlier candidate.
+  template <class F>
+  int tune(uint64_t key, const std::vector<Shape>& cands, hipStream_t st, F&& launch, Shape* out) {
+    const size_t m = cands.size();
+    while (ev_.size() < 4 * m) {
+      hipEvent_t e;
+      GK_HIP(hipEventCreate(&e));
+      ev_.push_back(e);
+    }
+    // Warm the device first (clock ramp after idle, caches, address
+    // translation): the first candidate runs until two consecutive runs agree
+    // within 5%, at most 16 times.
+    float prev = -1.0f;
+    for (int r = 0; r < 16; ++r) {
+      GK_HIP(hipEventRecord(ev_[0], st));
+      GK_TRY(launch(cands[0]));
+      GK_HIP(hipEventRecord(ev_[1], st));
+      GK_HIP(hipEventSynchronize(ev_[1]));
+      float t = 0;
+      GK_HIP(hipEventElapsedTime(&t, ev_[0], ev_[1]));
+      if (prev > 0 && std::fabs(t - prev) <= 0.05f * prev) break;
+      prev = t;
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t i = 0; i < m; ++i) {
+        GK_HIP(hipEventRecord(ev_[4 * i + 2 * pass], st));
+        GK_TRY(launch(cands[i]));
+        GK_HIP(hipEventRecord(ev_[4 * i + 2 * pass + 1], st));
+      }
+    GK_HIP(hipEventSynchronize(ev_[4 * (m - 1) + 3]));
+    size_t bi = 0;
+    float bt = 0;
+    for (size_t i = 0; i < m; ++i) {
+      float t0 = 0, t1 = 0;
+      GK_HIP(hipEventElapsedTime(&t0, ev_[4 * i], ev_[4 * i + 1]));
+      GK_HIP(hipEventElapsedTime(&t1, ev_[4 * i + 2], ev_[4 * i + 3]));
+      const float t = std::min(t0, t1);
+      if (i == 0 || t < bt) {
+        bi = i;
+        bt = t;
+      }
+    }
+    // GK_TUNE_LOG=1: one line per tuning with every candidate's time
+    // (diagnostics only).
+    const char* lg = std::getenv("GK_TUNE_LOG");
+    if (lg && lg[0] == '1') {
+      std::fprintf(stderr, "[gk tune] family %u log2n %u avg2 %u:", unsigned(key >> 32), unsigned((key >> 16) & 0xffff),
+                   unsigned(key & 0xffff));
+      for (size_t i = 0; i < m; ++i) {
+        float t0 = 0, t1 = 0;
+        (void)hipEventElapsedTime(&t0, ev_[4 * i], ev_[4 * i + 1]);
+        (void)hipEventElapsedTime(&t1, ev_[4 * i + 2], ev_[4 * i + 3]);
+        std::fprintf(stderr, " %dx%d=%.1fus", cands[i].lanes, cands[i].block, 1e3 * std::min(t0, t1));
+      }
+      std::fprintf(stderr, " -> %dx%d\n", cands[bi].lanes, cands[bi].block);
+    }
+    best_[key] = cands[bi];
+    ++tunings_;
+    *out = cands[bi];
+    return GK_OK;
+  }
+
+ private:
+  std::unordered_map<uint64_t, Shape> best_;
+  std::vector<hipEvent_t> ev_;
+  int tunings_ = 0;
+};
+}  // namespace gkrt
+
 struct gk_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -216,6 +321,8 @@ struct gk_ctx {
   gkrt::StageClock clock;
   gk_stats* cur = nullptr;  // stats of the running partition (stage counters)
   gkrt::Pinned up, dl;      // pinned staging of the batched form (uploads / read-backs)
+  gkrt::Tuner tuner;        // launch shapes chosen by timing (GK_FLAG_AUTOTUNE)
+  bool autotune = false;    // set per call from GK_FLAG_AUTOTUNE
 };
 
 namespace gkrt {
@@ -236,6 +343,72 @@ inline void count_stage(gk_ctx* ctx, int id, int64_t bytes) {
   ctx->cur->stage_launches[id] += 1;
   ctx->cur->stage_bytes[id] += bytes;
 }
+
+// A fixed run of stream work -- the rounds of a refinement level between
+// two host checks -- captured once into a hipGraph and replayed with one
+// launch.  The captured calls do not run at capture; the stage counters that
+// they advanced are advanced again at every replay instead.
+class RoundGraph {
+ public:
+  RoundGraph() = default;
+  RoundGraph(const RoundGraph&) = delete;
+  RoundGraph& operator=(const RoundGraph&) = delete;
+  ~RoundGraph() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+  }
+  bool ready() const { return exec_ != nullptr; }
+  // body() enqueues the work on st and returns a status.
+  template <class F>
+  int capture(gk_ctx* ctx, hipStream_t st, F&& body) {
+    gk_stats* S = ctx->cur;
+    int32_t l0[GK_ST_COUNT] = {};
+    int64_t b0[GK_ST_COUNT] = {};
+    if (S)
+      for (int i = 0; i < GK_ST_COUNT; ++i) {
+        l0[i] = S->stage_launches[i];
+        b0[i] = S->stage_bytes[i];
+      }
+    GK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &gr);
+    if (S)
+      for (int i = 0; i < GK_ST_COUNT; ++i) {
+        launches_[i] = S->stage_launches[i] - l0[i];
+        bytes_[i] = S->stage_bytes[i] - b0[i];
+        S->stage_launches[i] = l0[i];
+        S->stage_bytes[i] = b0[i];
+      }
+    if (rc != GK_OK || ec != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      if (rc != GK_OK) return rc;
+      gkh::set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      return GK_ERR_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&exec_, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ei != hipSuccess) {
+      exec_ = nullptr;
+      gkh::set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      return GK_ERR_HIP;
+    }
+    return GK_OK;
+  }
+  int replay(gk_ctx* ctx, hipStream_t st) {
+    GK_HIP(hipGraphLaunch(exec_, st));
+    if (ctx->cur)
+      for (int i = 0; i < GK_ST_COUNT; ++i) {
+        ctx->cur->stage_launches[i] += launches_[i];
+        ctx->cur->stage_bytes[i] += bytes_[i];
+      }
+    return GK_OK;
+  }
+
+ private:
+  hipGraphExec_t exec_ = nullptr;
+  int32_t launches_[GK_ST_COUNT] = {};
+  int64_t bytes_[GK_ST_COUNT] = {};
+};
 
 // Copy small device values to host variables through pinned staging, then
 // synchronise the stream once.
@@ -280,6 +453,10 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const gkd::DGraph& g, int32_t nc
                    const int32_t* cptr, const int32_t* order, Level& c);
 // Lanes per vertex of the gain pass for this graph's average degree.
 int gain_lanes(const gkd::DGraph& g);
+// Candidate matching with the context's tuned launch shape (tuned on first
+// use of a degree profile when ctx->autotune is set; the degree table
+// otherwise).
+int match_stage(gk_ctx* ctx, hipStream_t st, const gkd::DGraph& g, int32_t* cand);
 
 struct Level {
   gkd::DGraph g;

@@ -87,6 +87,15 @@ typedef struct gk_params {
 #define GK_FLAG_TRACE 2      /* record per-level hashes in gk_stats.levels   */
 #define GK_FLAG_STAGE_TIME 4 /* bracket every launch with HIP events on the
                                 work stream; fill gk_stats.stage_*          */
+#define GK_FLAG_GRAPH_ROUNDS 8 /* capture a level's refinement rounds (four
+                                between two host checks) in a hipGraph once
+                                and replay it; results are identical.  Not
+                                used while a round stage is event-timed.    */
+#define GK_FLAG_AUTOTUNE 16  /* choose the launch shape of matching (lanes
+                                per vertex, block size) and contraction
+                                (block size) by timing candidates on first
+                                use of a degree profile; kept in the ctx.
+                                Results are identical.                      */
 
 /* Stage ids of gk_stats.stage_ms / stage_launches / stage_bytes. */
 #define GK_ST_MATCH 0      /* candidate matching                           */

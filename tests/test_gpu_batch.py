@@ -144,3 +144,23 @@ def test_batch_small_move_cap(gk, ctx):
     graphs, ks = mixed_set(gk, 24, 11)
     parts, stats = ctx.partition_batch(graphs, ks, max_moves=24)
     check_same(ctx, graphs, ks, parts, stats, max_moves=24)
+
+
+def test_batch_graph_rounds(gk, ctx):
+    # Rounds replayed from a captured hipGraph: each graph's result equals its
+    # eager single-graph run; two packed calls in a row (two captures), with
+    # eager single-graph runs in between.
+    graphs, ks = mixed_set(gk, 16, 9)
+    for rep in range(2):
+        print(f"graph-round batch rep {rep}: n={[g.n for g in graphs]}", flush=True)
+        parts, stats = ctx.partition_batch(graphs, ks, flags=gk.GK_FLAG_GRAPH_ROUNDS, max_moves=64)
+        check_same(ctx, graphs, ks, parts, stats, max_moves=64)
+
+
+def test_batch_autotune(gk, ctx):
+    # A pack of over 1 Mi entries is tuned on its first coarsening level; each
+    # graph's result still equals its plain run.
+    graphs, ks = mixed_set(gk, 24, 12, lo=20000, hi=60000)
+    print(f"autotune batch: n={[g.n for g in graphs]}", flush=True)
+    parts, stats = ctx.partition_batch(graphs, ks, flags=gk.GK_FLAG_AUTOTUNE)
+    check_same(ctx, graphs, ks, parts, stats)

@@ -76,3 +76,42 @@ def test_road(gk, oracle, ctx, n, k):
 def test_small_move_cap(gk, oracle, ctx, family, n, k, moves):
     g = {"delaunay": lambda: gk.delaunay(n, 5), "dag": lambda: gk.dag(n, 40, 3), "road": lambda: gk.road(n, 2)}[family]()
     compare(gk, oracle, ctx, g, k, max_moves=moves)
+
+
+# Refinement rounds replayed from a captured hipGraph: the same trace and
+# partition as the oracle, on levels with many replays (small move cap) and
+# across calls that re-capture the graph, with eager calls in between.
+@pytest.mark.parametrize("family,n,k,moves", [("delaunay", 60000, 64, 1024), ("dag", 20000, 32, 16),
+                                              ("road", 30000, 8, 64)])
+def test_graph_rounds(gk, oracle, ctx, family, n, k, moves):
+    g = {"delaunay": lambda: gk.delaunay(n, 3), "dag": lambda: gk.dag(n, 40, 3), "road": lambda: gk.road(n, 2)}[family]()
+    flags = gk.GK_FLAG_TRACE | gk.GK_FLAG_GRAPH_ROUNDS
+    for rep in range(2):
+        print(f"graph rounds rep {rep} n={g.n} k={k} moves={moves}", flush=True)
+        part, st = ctx.partition(g, k, flags=flags, max_moves=moves)
+        eager, se = ctx.partition(g, k, flags=gk.GK_FLAG_TRACE, max_moves=moves)
+        assert np.array_equal(part, eager)
+        for l in range(st.nlevels):
+            a, b = st.levels[l], se.levels[l]
+            assert (a.rounds, a.moves, a.cut_after, a.h_part) == (b.rounds, b.moves, b.cut_after, b.h_part), l
+        assert (st.cut, st.cut_tracked, st.max_part_wgt) == (se.cut, se.cut_tracked, se.max_part_wgt)
+    opart, _ = oracle.partition(g, k, max_moves=moves)
+    assert np.array_equal(part, opart)
+
+
+# Launch shapes chosen by the autotuner: identical results, on a first call
+# (candidates timed, each writing the output) and a second call (cached
+# shapes), for a graph large enough to be tuned (over 1 Mi entries, row
+# classes over 16 Ki rows).
+def test_autotune_same_result(gk, oracle, ctx):
+    g = gk.delaunay(300000, 11)
+    print(f"autotune n={g.n} nnz={g.nnz}", flush=True)
+    base, sb = ctx.partition(g, 64, flags=gk.GK_FLAG_TRACE)
+    for rep in range(2):
+        part, st = ctx.partition(g, 64, flags=gk.GK_FLAG_TRACE | gk.GK_FLAG_AUTOTUNE)
+        assert np.array_equal(part, base), rep
+        for l in range(st.nlevels):
+            a, b = st.levels[l], sb.levels[l]
+            assert (a.h_cand, a.h_cmap, a.h_coarse, a.h_part) == (b.h_cand, b.h_cmap, b.h_coarse, b.h_part), (rep, l)
+    opart, _ = oracle.partition(g, 64)
+    assert np.array_equal(base, opart)
