@@ -78,6 +78,19 @@ def test_small_move_cap(gk, oracle, ctx, family, n, k, moves):
     compare(gk, oracle, ctx, g, k, max_moves=moves)
 
 
+# Part counts that are not powers of two (uneven recursive splits) and large
+# k: the gain pass's LDS histogram path and balanced prefixes over thousands
+# of parts (k = 8192 is the largest the kernels take).
+@pytest.mark.parametrize("family,n,k", [("delaunay", 20000, 3), ("dag", 30000, 7), ("delaunay", 60000, 5),
+                                        ("road", 30000, 100), ("delaunay", 400000, 1000), ("dag", 300000, 3000),
+                                        ("road", 200000, 8192)])
+def test_any_k(gk, oracle, ctx, family, n, k):
+    g = {"delaunay": lambda: gk.delaunay(n, 4), "dag": lambda: gk.dag(n, min(100, n // 300), 4),
+         "road": lambda: gk.road(n, 4)}[family]()
+    st = compare(gk, oracle, ctx, g, k)
+    assert st.max_part_wgt <= st.cap
+
+
 # Refinement rounds replayed from a captured hipGraph: the same trace and
 # partition as the oracle, on levels with many replays (small move cap) and
 # across calls that re-capture the graph, with eager calls in between.
