@@ -248,6 +248,10 @@ class Tuner {
   template <class F>
   int tune(uint64_t key, const std::vector<Shape>& cands, hipStream_t st, F&& launch, Shape* out) {
     const size_t m = cands.size();
+    if (m == 0) {
+      gkh::set_error("autotuner: no candidate launch shape");
+      return GK_ERR_INTERNAL;
+    }
     while (ev_.size() < 4 * m) {
       hipEvent_t e;
       GK_HIP(hipEventCreate(&e));
