@@ -32,6 +32,23 @@ def test_library_exports_every_declared_symbol(header, lib):
     assert not missing, missing
 
 
+def test_header_constants_match_binding(gk):
+    # Flag bits, stage ids and coarsening policies: the Python binding's
+    # values are the header's.
+    txt = open(os.path.join(ROOT, "include", "gkway.h")).read()
+    defs = dict(re.findall(r"#define\s+(GK_(?:FLAG|ST|COARSEN)_[A-Z_]+)\s+(-?\d+)", txt))
+    assert {"GK_FLAG_CHECK", "GK_FLAG_TRACE", "GK_FLAG_STAGE_TIME", "GK_FLAG_GRAPH_ROUNDS",
+            "GK_FLAG_AUTOTUNE"} <= set(defs)
+    checked = 0
+    for name, val in defs.items():
+        if hasattr(gk, name):
+            assert getattr(gk, name) == int(val), name
+            checked += 1
+    assert checked >= 6
+    flags = [int(v) for k, v in defs.items() if k.startswith("GK_FLAG_")]
+    assert all(f & (f - 1) == 0 for f in flags) and len(set(flags)) == len(flags)
+
+
 def test_status_strings_and_defaults(gk):
     L = gk.lib()
     for code in range(0, -10, -1):
