@@ -343,26 +343,54 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   // ---------------------------------------------------------- initial partition
   Level& top = L.back();
   const int32_t nt = top.g.n;
-  std::vector<int32_t> hx, ha, hw, hv, hp(nt), hb(nt, 0);
-  GK_TRY(to_host(st, hx, top.g.xadj, int64_t(nt) + 1));
-  GK_TRY(to_host(st, ha, top.g.adj, top.g.nnz));
-  GK_TRY(to_host(st, hw, top.g.w, top.g.nnz));
-  GK_TRY(to_host(st, hv, top.g.vw, nt));
+  // Largest row of every level refined later (it decides whether a level
+  // needs the heavy-row gain kernel and the long-row selection pass): read
+  // back with the initial partition's upload, not one host sync per level.
+  Buf<int32_t> dmaxall;
+  std::vector<int32_t> hmaxdeg(nl, 0);
+  GK_TRY(dmaxall.alloc(P, nl));
+  for (int32_t l = 0; l + 1 < nl; ++l) GK_TRY(gkd::launch_max_deg(L[l].g, dmaxall.p + l, st));
+  // The coarsest graph comes back through the context's pinned staging in
+  // one piece (four copies, one host sync) and is partitioned from there.
+  const int64_t tn = top.g.nnz;
+  void* hdn = nullptr;
+  GK_TRY(ctx->dl.get(st, sizeof(int32_t) * (2 * size_t(nt) + 1 + 2 * size_t(tn)), &hdn));
+  int32_t* hx = static_cast<int32_t*>(hdn);
+  int32_t* ha = hx + nt + 1;
+  int32_t* hw = ha + tn;
+  int32_t* hv = hw + tn;
+  GK_HIP(hipMemcpyAsync(hx, top.g.xadj, sizeof(int32_t) * (size_t(nt) + 1), hipMemcpyDeviceToHost, st));
+  if (tn > 0) {
+    GK_HIP(hipMemcpyAsync(ha, top.g.adj, sizeof(int32_t) * size_t(tn), hipMemcpyDeviceToHost, st));
+    GK_HIP(hipMemcpyAsync(hw, top.g.w, sizeof(int32_t) * size_t(tn), hipMemcpyDeviceToHost, st));
+  }
+  if (nt > 0) GK_HIP(hipMemcpyAsync(hv, top.g.vw, sizeof(int32_t) * size_t(nt), hipMemcpyDeviceToHost, st));
+  GK_HIP(hipStreamSynchronize(st));
   gk_csr hc;
   hc.n = nt;
-  hc.nnz = top.g.nnz;
-  hc.xadj = hx.data();
-  hc.adjncy = ha.data();
-  hc.adjwgt = hw.data();
-  hc.vwgt = hv.data();
+  hc.nnz = tn;
+  hc.xadj = hx;
+  hc.adjncy = ha;
+  hc.adjwgt = hw;
+  hc.vwgt = hv;
+  // Initial part, boundary flags, part weights and cut go up through pinned
+  // staging as well.
+  void* hup = nullptr;
+  GK_TRY(ctx->up.get(st, sizeof(int32_t) * (2 * size_t(nt) + size_t(k)) + sizeof(long long), &hup));
+  long long* hcut0 = static_cast<long long*>(hup);
+  int32_t* hp = reinterpret_cast<int32_t*>(hcut0 + 1);
+  int32_t* hb = hp + nt;
+  int32_t* hpw = hb + nt;
   int64_t cut0 = 0;
-  GK_TRY(gk_initpart(&hc, k, cap, prm.seed, prm.initpart_threads, hp.data(), &cut0));
-  std::vector<int32_t> hpw(k, 0);
+  GK_TRY(gk_initpart(&hc, k, cap, prm.seed, prm.initpart_threads, hp, &cut0));
+  std::fill(hb, hb + nt, 0);
+  std::fill(hpw, hpw + k, 0);
   for (int32_t v = 0; v < nt; ++v) {
     hpw[hp[v]] += hv[v];
     for (int32_t j = hx[v]; j < hx[v + 1]; ++j)
       if (hp[ha[j]] != hp[v]) { hb[v] = 1; break; }
   }
+  *hcut0 = cut0;
   S.init_cut = cut0;
   Buf<int32_t> partA, partB, bndA, bndB, pw;
   GK_TRY(partA.alloc(P, n0));
@@ -372,13 +400,18 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   GK_TRY(pw.alloc(P, k));
   Buf<long long> dcut;
   GK_TRY(dcut.alloc(P, 1));
-  GK_HIP(hipMemcpyAsync(partA.p, hp.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, st));
-  GK_HIP(hipMemcpyAsync(bndA.p, hb.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, st));
-  GK_HIP(hipMemcpyAsync(pw.p, hpw.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
-  GK_HIP(hipMemcpyAsync(dcut.p, &cut0, sizeof(long long), hipMemcpyHostToDevice, st));
-  GK_HIP(hipStreamSynchronize(st));
+  if (nt > 0) {
+    GK_HIP(hipMemcpyAsync(partA.p, hp, sizeof(int32_t) * size_t(nt), hipMemcpyHostToDevice, st));
+    GK_HIP(hipMemcpyAsync(bndA.p, hb, sizeof(int32_t) * size_t(nt), hipMemcpyHostToDevice, st));
+  }
+  GK_HIP(hipMemcpyAsync(pw.p, hpw, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, st));
+  GK_HIP(hipMemcpyAsync(dcut.p, hcut0, sizeof(long long), hipMemcpyHostToDevice, st));
+  GK_TRY(fetch(ctx, st, {{hmaxdeg.data(), dmaxall.p, sizeof(int32_t) * size_t(nl)}}));
+  const uint64_t hpart_top = trace ? fnv_host(std::vector<int32_t>(hp, hp + nt)) : 0;
+  ctx->dl.drained();  // the stream is synchronised: both staging arenas are free again
+  ctx->up.drained();
   S.levels[nl - 1].cut_after = cut0;
-  if (trace) S.levels[nl - 1].h_part = fnv_host(hp);
+  if (trace) S.levels[nl - 1].h_part = hpart_top;
   const auto t2 = clk::now();
 
   // ---------------------------------------------------------- uncoarsening
@@ -394,7 +427,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   if (nl > 1) {
     const int32_t M = prm.max_moves;
     Buf<int32_t> xlist, dthr, wpos, sscr;
-    Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, active, drounds, lastcnt, dvoff, dpw_off, dkk, dmax;
+    Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, active, drounds, lastcnt, dvoff, dpw_off, dkk;
     Buf<long long> dcap, dmoved;
     Buf<uint64_t> cand, movable, selbits, defbits, longbits;
     Buf<uint32_t> hkey;
@@ -423,14 +456,16 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(dthr.alloc(P, 1));
     GK_TRY(hcnt.alloc(P, 2));  // [0] gain worklist count, [1] heavy-row count
     GK_TRY(active.alloc(P, 1));
-    GK_TRY(drounds.alloc(P, 1));
+    // Per-level rounds, moves and cut after the level, read back once at the end.
+    Buf<long long> dcutlv;
+    GK_TRY(drounds.alloc(P, nl));
     GK_TRY(lastcnt.alloc(P, 1));
-    GK_TRY(dmoved.alloc(P, 1));
+    GK_TRY(dmoved.alloc(P, nl));
+    GK_TRY(dcutlv.alloc(P, nl));
     GK_TRY(dvoff.alloc(P, 2));
     GK_TRY(dpw_off.alloc(P, 2));
     GK_TRY(dkk.alloc(P, 1));
     GK_TRY(dcap.alloc(P, 1));
-    GK_TRY(dmax.alloc(P, 1));
     gkd::fill<uint64_t>(selbits.p, 0ull, nwords0 + 1, st);
     const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
     gkd::fill<int32_t>(dvoff.p, 0, 1, st);
@@ -451,17 +486,15 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       const int lanes = gain_lanes(g);
       // Rows longer than the lane count go to the heavy-row kernel; without
       // any, its launches are skipped.
-      int32_t maxdeg = 0;
-      GK_TRY(gkd::launch_max_deg(g, dmax.p, st));
-      GK_TRY(fetch(ctx, st, {{&maxdeg, dmax.p, sizeof maxdeg}}));
+      const int32_t maxdeg = hmaxdeg[l];
       const bool heavy_rows = maxdeg > lanes;
       // After projection every gain is 0: no candidate yet.
       gkd::fill<uint64_t>(cand.p, 0ull, (int64_t(g.n) + 63) / 64 + 1, st);
       gkd::fill<int32_t>(dvoff.p + 1, g.n, 1, st);
-      gkd::fill<int32_t>(drounds.p, 0, 1, st);
+      gkd::fill<int32_t>(drounds.p + l, 0, 1, st);
       gkd::fill<int32_t>(lastcnt.p, 1, 1, st);
       gkd::fill<int32_t>(dthr.p, 1, 1, st);
-      gkd::fill<long long>(dmoved.p, 0, 1, st);
+      gkd::fill<long long>(dmoved.p + l, 0, 1, st);
       // The projected boundary becomes the first gain worklist.
       gkd::fill<int32_t>(hcnt.p, 0, 2, st);
       GK_STAGE(ctx, GK_ST_PROJECT, 4LL * g.n, gkd::launch_compact(g.n, upd.p, wl.p, hcnt.p, st));
@@ -481,8 +514,8 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
                                          hcnt.p, active.p, maxdeg, st));
         GK_STAGE(ctx, GK_ST_BALANCE, 0,
                  gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p, pw.p,
-                                     dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                     lastcnt.p, active.p, cand.p, dthr.p, st));
+                                     dcut.p, drounds.p + l, dmoved.p + l, sb, xlist.p, hkey.p, upd.p, wl.p,
+                                     hcnt.p, lastcnt.p, active.p, cand.p, dthr.p, st));
         GK_STAGE(ctx, GK_ST_GAIN, 0,
                  gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
                                      work.p, wl_items, cand.p, st));
@@ -514,20 +547,27 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
         if (live == 0) break;
         if (ran >= 32 && per_check < 32) per_check *= 2;
       }
-      long long hcut = 0, hmoved = 0;
-      int32_t hrounds = 0;
-      GK_TRY(fetch(ctx, st, {{&hcut, dcut.p, sizeof hcut}, {&hmoved, dmoved.p, sizeof hmoved},
-                             {&hrounds, drounds.p, sizeof hrounds}}));
-      // Moves: 24 bytes each in the round kernel and its apply.
-      S.stage_bytes[GK_ST_BALANCE] += 24LL * hmoved + 8LL * hmoved * avg_deg;
-      ls.rounds = hrounds;
-      ls.moves = hmoved;
-      ls.cut_after = hcut;
+      GK_HIP(hipMemcpyAsync(dcutlv.p + l, dcut.p, sizeof(long long), hipMemcpyDeviceToDevice, st));
       if (trace) {
         std::vector<int32_t> h;
         GK_TRY(to_host(st, h, cur_part, g.n));
         ls.h_part = fnv_host(h);
       }
+    }
+    std::vector<long long> hcutlv(nl), hmovedlv(nl);
+    std::vector<int32_t> hroundslv(nl);
+    GK_TRY(fetch(ctx, st, {{hcutlv.data(), dcutlv.p, sizeof(long long) * size_t(nl - 1)},
+                           {hmovedlv.data(), dmoved.p, sizeof(long long) * size_t(nl - 1)},
+                           {hroundslv.data(), drounds.p, sizeof(int32_t) * size_t(nl - 1)}}));
+    for (int32_t l = 0; l + 1 < nl; ++l) {
+      gk_level_stats& ls = S.levels[l];
+      const DGraph& g = L[l].g;
+      const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
+      // Moves: 24 bytes each in the round kernel and its apply.
+      S.stage_bytes[GK_ST_BALANCE] += 24LL * hmovedlv[l] + 8LL * hmovedlv[l] * avg_deg;
+      ls.rounds = hroundslv[l];
+      ls.moves = hmovedlv[l];
+      ls.cut_after = hcutlv[l];
     }
     // Data-dependent bytes of the gain passes from the device work counters.
     std::vector<unsigned long long> wk;
