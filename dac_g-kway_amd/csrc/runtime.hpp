@@ -358,7 +358,10 @@ class RoundGraph {
   RoundGraph(const RoundGraph&) = delete;
   RoundGraph& operator=(const RoundGraph&) = delete;
   ~RoundGraph() {
-    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (!exec_) return;
+    // Replays still queued (an error path) finish before the graph goes.
+    if (st_) (void)hipStreamSynchronize(st_);
+    (void)hipGraphExecDestroy(exec_);
   }
   bool ready() const { return exec_ != nullptr; }
   // body() enqueues the work on st and returns a status.
@@ -399,6 +402,7 @@ class RoundGraph {
     return GK_OK;
   }
   int replay(gk_ctx* ctx, hipStream_t st) {
+    st_ = st;
     GK_HIP(hipGraphLaunch(exec_, st));
     if (ctx->cur)
       for (int i = 0; i < GK_ST_COUNT; ++i) {
@@ -410,6 +414,7 @@ class RoundGraph {
 
  private:
   hipGraphExec_t exec_ = nullptr;
+  hipStream_t st_ = nullptr;  // stream of the last replay
   int32_t launches_[GK_ST_COUNT] = {};
   int64_t bytes_[GK_ST_COUNT] = {};
 };
