@@ -32,7 +32,6 @@ struct BalanceLds {
   unsigned long long risk[kBalMaxK / 64];
   int32_t ws[kBalThreads / 64 + 1];
   int32_t wmax[kBalThreads / 64];
-  int32_t nev;
   long long sgain[kBalThreads / 64];
 };
 
@@ -161,11 +160,7 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   const int wave = t >> 6;
   // 1. the moves, and each part's inflow over the whole sequence
   for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
-  for (int32_t i = t; i < (k + 63) / 64; i += kBalThreads) L.risk[i] = 0;
-  if (t == 0) {
-    L.nev = 0;
-    L.cover[kBalMax] = 0;
-  }
+  if (t == 0) L.cover[kBalMax] = 0;
   L.cover[t] = 0;
   int32_t g = 0, src = 0, dst = 0, w = 0;
   if (t < B) {
@@ -181,27 +176,34 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   if (t < B) atomicAdd(&dpw[dst], w);
   __syncthreads();
   // 2. risky parts: pw + inflow > cap (the only ones that can be above the
-  // cap at some position with a positive change)
-  for (int32_t q = t; q < k; q += kBalThreads) {
-    const int32_t in = dpw[q];
-    if (in > 0 && (long long)pw[q] + in > cap) atomicOr(&L.risk[q >> 6], 1ull << (q & 63));
-    dpw[q] = 0;
+  // cap at some position with a positive change); a wave writes the bitmask
+  // word of its 64 parts (no shared-word atomics)
+  for (int32_t q0 = wave * 64; q0 < k; q0 += kBalThreads) {
+    const int32_t q = q0 + lane;
+    bool r = false;
+    if (q < k) {
+      const int32_t in = dpw[q];
+      r = in > 0 && (long long)pw[q] + in > cap;
+      dpw[q] = 0;
+    }
+    const uint64_t m = ballot(r);
+    if (lane == 0) L.risk[q0 >> 6] = m;
   }
   __syncthreads();
   // 3. event keys of the risky parts (part << 11 | position << 1 | kind),
-  // kind 0 = leave, 1 = enter; slots in any order, sorted next
+  // kind 0 = leave, 1 = enter; slots from a block scan (keys are distinct,
+  // sorted next)
   uint32_t* ev = reinterpret_cast<uint32_t*>(L.sk);
+  bool rs = false, rd = false;
   if (t < B) {
-    const bool rs = (L.risk[src >> 6] >> (src & 63)) & 1ull;
-    const bool rd = (L.risk[dst >> 6] >> (dst & 63)) & 1ull;
-    if (rs || rd) {
-      int slot = atomicAdd(&L.nev, int(rs) + int(rd));
-      if (rs) ev[slot++] = (uint32_t(src) << 11) | (uint32_t(t) << 1);
-      if (rd) ev[slot] = (uint32_t(dst) << 11) | (uint32_t(t) << 1) | 1u;
-    }
+    rs = (L.risk[src >> 6] >> (src & 63)) & 1ull;
+    rd = (L.risk[dst >> 6] >> (dst & 63)) & 1ull;
   }
+  int nev;
+  int slot = block_excl_scan_1024(int(rs) + int(rd), &nev, L.ws);
+  if (rs) ev[slot++] = (uint32_t(src) << 11) | (uint32_t(t) << 1);
+  if (rd) ev[slot] = (uint32_t(dst) << 11) | (uint32_t(t) << 1) | 1u;
   __syncthreads();
-  const int nev = L.nev;
   int P = 128;
   while (P < nev) P <<= 1;
   for (int i = nev + t; i < P; i += kBalThreads) ev[i] = ~0u;
