@@ -749,9 +749,11 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
       if (ex < need && need <= ex + h0) {
         R.misc[0] = 2 * t;
         R.misc[1] = ex;
+        R.misc[3] = h0;
       } else if (ex + h0 < need && need <= ex + h0 + h1) {
         R.misc[0] = 2 * t + 1;
         R.misc[1] = ex + h0;
+        R.misc[3] = h1;
       }
       // First digit: also the digit holding rank kLazyFactor * M, for the
       // next round's gain threshold.
@@ -769,15 +771,21 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
       need -= R.misc[1];
       prefix |= uint64_t(R.misc[0]) << shift;
       pmask |= 2047ull << shift;
+      // Every key of the chosen bucket is needed: the B best are the keys
+      // whose digits so far are <= prefix, and the lower digits can stay
+      // unread.  (Only R.misc[2] and R.b.sk are written next; neither is read
+      // in this pass after its last barrier.)
+      if (need == R.misc[3]) break;
       __syncthreads();
     }
-    // prefix is now the B-th smallest compressed key; keys are distinct, so
-    // the keys <= prefix are exactly the B best.
+    // (key & pmask) <= prefix holds for exactly the B best keys: prefix is
+    // the B-th smallest key's leading digits and the keys are distinct.
     if (t == 0) R.misc[2] = 0;
     R.b.sk[t] = ~0ull;
     __syncthreads();
     for (int32_t i = t; i < count; i += kBalThreads)
-      if (ckey(i) <= prefix) R.b.sk[atomicAdd(&R.misc[2], 1)] = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
+      if ((ckey(i) & pmask) <= prefix)
+        R.b.sk[atomicAdd(&R.misc[2], 1)] = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
   }
   __syncthreads();
   // 3. sort and balanced prefix
