@@ -688,6 +688,7 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     return;
   }
   const int32_t B = count < M ? count : M;
+  bool narrow = false;  // keys already sorted as 32-bit compressed keys
   // 2. the B best keys into L.sk[0, B)
   if (count <= M) {
     R.b.sk[t] = t < count ? move_key(gain, seg[t]) : ~0ull;
@@ -780,16 +781,35 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     }
     // (key & pmask) <= prefix holds for exactly the B best keys: prefix is
     // the B-th smallest key's leading digits and the keys are distinct.
+    // Compressed keys of at most 32 bits are sorted as 32-bit words (half the
+    // shuffle and LDS traffic of the 64-bit keys) and expanded afterwards.
+    narrow = tb <= 32;
+    uint32_t* nk = reinterpret_cast<uint32_t*>(R.b.sv);  // free until balance_prefix
     if (t == 0) R.misc[2] = 0;
-    R.b.sk[t] = ~0ull;
+    if (narrow) nk[t] = 0xffffffffu;
+    else R.b.sk[t] = ~0ull;
     __syncthreads();
-    for (int32_t i = t; i < count; i += kBalThreads)
-      if ((ckey(i) & pmask) <= prefix)
-        R.b.sk[atomicAdd(&R.misc[2], 1)] = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
+    for (int32_t i = t; i < count; i += kBalThreads) {
+      const uint64_t ck = ckey(i);
+      if ((ck & pmask) <= prefix) {
+        const int slot = atomicAdd(&R.misc[2], 1);
+        if (narrow) nk[slot] = uint32_t(ck);
+        else R.b.sk[slot] = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
+      }
+    }
+    if (narrow) {
+      __syncthreads();
+      int P = 128;
+      while (P < B) P <<= 1;
+      bitonic_lds(nk, P);
+      const uint32_t ck = nk[t];
+      const uint32_t idm = uint32_t((1ull << ib) - 1);
+      R.b.sk[t] = t < B ? (uint64_t(hmin + (ck >> ib)) << 32) | uint64_t(uint32_t(vb) + (ck & idm)) : ~0ull;
+    }
   }
   __syncthreads();
   // 3. sort and balanced prefix
-  lds_bitonic(R.b, B);
+  if (!narrow) lds_bitonic(R.b, B);
   const int pos = balance_prefix(R.b, dpw, kk[j], B, 0xffffffffull, part, dest, gain, vw, pw + pw_off[j], cap[j],
                                  nullptr, nullptr, &cut[j]);
   // 4. apply moves 0..pos (balance_prefix read part[] before its last barrier):
