@@ -836,6 +836,11 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   }
   __syncthreads();
   constexpr int kApplyPer = 8;
+  // One workgroup (one job) owns the worklist count, which the selection
+  // zeroed: it appends at a running offset and stores the count at the end.
+  // Several jobs append with one global atomic per pass.
+  const bool sole = gridDim.x == 1;
+  int32_t wrun = 0;
   for (int pb = 0; pb < tot; pb += kApplyPer * kBalThreads) {
     const int a0 = pb + kApplyPer * t;
     int32_t us[kApplyPer];
@@ -863,7 +868,14 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     }
     int wtot;
     const int wex = block_excl_scan_1024(__popc(win), &wtot, R.ws);
-    if (t == 0 && wtot > 0) R.misc[0] = atomicAdd(&wlc[0], wtot);
+    if (t == 0) {
+      if (sole) {
+        R.misc[0] = wrun;
+        wrun += wtot;
+      } else if (wtot > 0) {
+        R.misc[0] = atomicAdd(&wlc[0], wtot);
+      }
+    }
     __syncthreads();
     const int wbase = R.misc[0] + wex;
     int r = 0;
@@ -873,6 +885,7 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     __syncthreads();  // R.misc[0] is reused by the next pass
   }
   if (t == 0) {
+    if (sole) wlc[0] = wrun;
     thr[j] = R.lz[2];
     rounds[j] += 1;
     moved[j] += pos + 1;
