@@ -495,8 +495,7 @@ constexpr int kScatterWords = 2048;
 
 __global__ void __launch_bounds__(256) k_sel_scatter(int64_t nwords, const uint64_t* __restrict__ selbits,
                                                      const int32_t* __restrict__ bsum, int32_t* __restrict__ wpos,
-                                                     const int32_t* __restrict__ gain, int32_t* __restrict__ list,
-                                                     uint32_t* __restrict__ lkey) {
+                                                     int32_t* __restrict__ list) {
   __shared__ int lds[8];
   const int64_t wb = int64_t(blockIdx.x) * kScatterWords;
   // offset: selection blocks [0, wb / kSelectBlockWords)
@@ -523,9 +522,7 @@ __global__ void __launch_bounds__(256) k_sel_scatter(int64_t nwords, const uint6
     while (x) {
       const int bit = __ffsll(static_cast<long long>(x)) - 1;
       x &= x - 1;
-      const int32_t v = int32_t(((w0 + i) << 6) + bit);
-      list[r] = v;
-      lkey[r++] = uint32_t(0x7fffffff - gain[v]);
+      list[r++] = int32_t(((w0 + i) << 6) + bit);
     }
   }
   if (wb + kScatterWords >= nwords && threadIdx.x == 255) wpos[nwords] = r;
@@ -560,8 +557,7 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain
     if (lgrid > 1024) lgrid = 1024;
     k_select_long<<<lgrid, 256, 0, st>>>(g.n, sb.longbits, g.xadj, g.adj, sb.movable, sb.selbits, sb.scratch);
   }
-  k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, gain,
-                                                                   sb.list, sb.lkey);
+  k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list);
   return hip_status();
 }
 
@@ -637,7 +633,6 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   }
   __syncthreads();
   const int32_t* seg = sb.list + R.misc[3];
-  const uint32_t* segk = sb.lkey + R.misc[3];  // gain keys of seg (null after the fallback)
   int32_t count = R.misc[1] - R.misc[3];
   const int32_t prev = R.misc[2];
   if (count < M && thr[j] > 1) {
@@ -667,7 +662,6 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     __syncthreads();
     count = R.lz[0];
     seg = xs;
-    segk = nullptr;
   }
   if (t == 0) {
     lastcnt[j] = count;
@@ -697,8 +691,7 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   bool narrow = false;  // keys already sorted as 32-bit compressed keys
   // 2. the B best keys into L.sk[0, B)
   if (count <= M) {
-    R.b.sk[t] = t < count ? (segk ? (uint64_t(segk[t]) << 32) | uint64_t(uint32_t(seg[t])) : move_key(gain, seg[t]))
-                          : ~0ull;
+    R.b.sk[t] = t < count ? move_key(gain, seg[t]) : ~0ull;
   } else {
     // Exact top-B by radix select on a compressed key: (gain key - its
     // minimum) above (id - first id of the job); the key is monotone in
@@ -708,7 +701,7 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     uint32_t* hk = hkey + vb;
     uint32_t lo = 0xffffffffu, hi = 0;
     for (int32_t i = t; i < count; i += kBalThreads) {
-      const uint32_t h = segk ? segk[i] : uint32_t(0x7fffffff - gain[seg[i]]);
+      const uint32_t h = uint32_t(0x7fffffff - gain[seg[i]]);
       hk[i] = h;
       lo = h < lo ? h : lo;
       hi = h > hi ? h : hi;

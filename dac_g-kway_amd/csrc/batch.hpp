@@ -61,8 +61,7 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
                        const int32_t* bnd, hipStream_t st);
 // Buffers of the round selection (n vertices, nwords = (n + 63) / 64):
 // movable, selbits, defbits, longbits and wpos hold nwords + 1 entries, the
-// last word of selbits zero (filled once); list holds n ints and lkey n
-// uints (the gain key 0x7fffffff - gain of each list entry); scratch holds
+// last word of selbits zero (filled once); list holds n ints; scratch holds
 // select_scratch_ints(n) ints (per-block selection counts).
 struct SelectBufs {
   uint64_t* movable;
@@ -71,7 +70,6 @@ struct SelectBufs {
   uint64_t* longbits;
   int32_t* wpos;
   int32_t* list;
-  uint32_t* lkey;
   int* scratch;
 };
 int select_scratch_ints(int64_t n);

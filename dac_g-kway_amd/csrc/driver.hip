@@ -430,7 +430,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, active, drounds, lastcnt, dvoff, dpw_off, dkk;
     Buf<long long> dcap, dmoved;
     Buf<uint64_t> cand, movable, selbits, defbits, longbits;
-    Buf<uint32_t> hkey, lkey;
+    Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
     // gain statistics: a (vertices, entries) pair per block of the gain
     // launches (grids of at most kGainSlots blocks), summed at the end
@@ -444,7 +444,6 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(wl.alloc(P, n0));
     GK_TRY(list.alloc(P, n0));
     GK_TRY(hkey.alloc(P, n0));
-    GK_TRY(lkey.alloc(P, n0));
     GK_TRY(cand.alloc(P, (int64_t(n0) + 63) / 64 + 1));
     const int64_t nwords0 = (int64_t(n0) + 63) / 64;
     GK_TRY(movable.alloc(P, nwords0 + 1));
@@ -468,7 +467,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(dkk.alloc(P, 1));
     GK_TRY(dcap.alloc(P, 1));
     gkd::fill<uint64_t>(selbits.p, 0ull, nwords0 + 1, st);
-    const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, lkey.p, sscr.p};
+    const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
     gkd::fill<int32_t>(dvoff.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p + 1, k, 1, st);

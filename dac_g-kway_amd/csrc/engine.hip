@@ -564,7 +564,7 @@ int Engine::refine_step(bool* worked) {
       sscr;
   Buf<long long> dcap, dcut, dmoved;
   Buf<uint64_t> cand, movable, selbits, defbits, longbits;
-  Buf<uint32_t> hkey, lkey;
+  Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
   auto gainp = std::make_shared<Buf<int32_t>>();
@@ -596,8 +596,7 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(dthr.alloc(P_, J));
   gkd::fill<int32_t>(dthr.p, 1, J, st_);
   gkd::fill<uint64_t>(selbits.p, 0ull, nwords + 1, st_);
-  GK_TRY(lkey.alloc(P_, N));
-  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, lkey.p, sscr.p};
+  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
