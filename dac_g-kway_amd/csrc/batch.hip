@@ -774,8 +774,8 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
       pmask |= 2047ull << shift;
       // Every key of the chosen bucket is needed: the B best are the keys
       // whose digits so far are <= prefix, and the lower digits can stay
-      // unread.  (Only R.misc[2] and R.b.sk are written next; neither is read
-      // in this pass after its last barrier.)
+      // unread.  (Only R.misc[2], R.b.sk and R.b.sv are written next; none
+      // of them is read in this pass after its last barrier.)
       if (need == R.misc[3]) break;
       __syncthreads();
     }
