@@ -33,6 +33,8 @@ struct BalanceLds {
   int32_t ws[kBalThreads / 64 + 1];
   int32_t wmax[kBalThreads / 64];
   long long sgain[kBalThreads / 64];
+  uint32_t ev2[2 * kBalMax];  // second buffer of the event sort
+  int32_t rcnt[256];          // event sort: element counts per (digit, wave)
 };
 
 // Exclusive scan of one int per thread over a kBalThreads block; *tot gets
@@ -143,6 +145,73 @@ __device__ inline void lds_bitonic(BalanceLds& L, int32_t count) {
   bitonic_lds(L.sk, P);
 }
 
+// Stable sort of the event keys a[0, n) (n <= 2 * kBalThreads) by their part
+// field, bits [11, 11 + pbits): LSD passes over 4-bit digits.  Thread t moves
+// elements 2t and 2t + 1; an element's rank among the equal digits of its
+// wave comes from ballot match masks, and the per-(digit, wave) counts in C
+// (256 ints), scanned in digit-major order, give the offsets.  The events
+// arrive in (position, kind) order, so the result is ordered by the whole
+// key.  Returns the buffer holding the result (a or b).  All threads call it,
+// after a barrier that follows the writes of a[]; it ends with a barrier.
+__device__ inline uint32_t* sort_events_by_part(uint32_t* a, uint32_t* b, int n, int pbits, int32_t* C) {
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = t >> 6;
+  const uint64_t lt = lanemask_lt(), le = lt | (1ull << lane);
+  for (int s = 11; s < 11 + pbits; s += 4) {
+    const bool v0 = 2 * t < n, v1 = 2 * t + 1 < n;
+    const uint32_t k0 = v0 ? a[2 * t] : 0u, k1 = v1 ? a[2 * t + 1] : 0u;
+    const int d0 = int((k0 >> s) & 15u), d1 = int((k1 >> s) & 15u);
+    uint64_t m0[4], m1[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      m0[i] = ballot(v0 && ((d0 >> i) & 1));
+      m1[i] = ballot(v1 && ((d1 >> i) & 1));
+    }
+    const uint64_t vm0 = ballot(v0), vm1 = ballot(v1);
+    // lanes whose first (second) element has digit x
+    auto same0 = [&](int x) {
+      uint64_t m = vm0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) m &= ((x >> i) & 1) ? m0[i] : ~m0[i];
+      return m;
+    };
+    auto same1 = [&](int x) {
+      uint64_t m = vm1;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) m &= ((x >> i) & 1) ? m1[i] : ~m1[i];
+      return m;
+    };
+    // wave order: lane by lane, first element before second
+    const int r0 = popc64(same0(d0) & lt) + popc64(same1(d0) & lt);
+    const int r1 = popc64(same0(d1) & le) + popc64(same1(d1) & lt);
+    if (lane < 16) C[lane * 16 + wave] = popc64(same0(lane)) + popc64(same1(lane));
+    __syncthreads();
+    if (wave == 0) {
+      int c[4], sum = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        c[i] = C[4 * lane + i];
+        sum += c[i];
+      }
+      int run = wave_incl_scan(sum) - sum;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        C[4 * lane + i] = run;
+        run += c[i];
+      }
+    }
+    __syncthreads();
+    if (v0) b[C[d0 * 16 + wave] + r0] = k0;
+    if (v1) b[C[d1 * 16 + wave] + r1] = k1;
+    __syncthreads();
+    uint32_t* x = a;
+    a = b;
+    b = x;
+  }
+  return a;
+}
+
 // Largest balanced prefix of the B sorted moves in L.sk (vertex id in the
 // bits of idmask): the last position pos such that, after moves 0..pos, no
 // part whose weight grew is above cap.  dpw: k ints of LDS (k parts, k <=
@@ -191,8 +260,8 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   }
   __syncthreads();
   // 3. event keys of the risky parts (part << 11 | position << 1 | kind),
-  // kind 0 = leave, 1 = enter; slots from a block scan (keys are distinct,
-  // sorted next)
+  // kind 0 = leave, 1 = enter; slots from a block scan, so in (position,
+  // kind) order; then a stable sort by part
   uint32_t* ev = reinterpret_cast<uint32_t*>(L.sk);
   bool rs = false, rd = false;
   if (t < B) {
@@ -204,11 +273,7 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   if (rs) ev[slot++] = (uint32_t(src) << 11) | (uint32_t(t) << 1);
   if (rd) ev[slot] = (uint32_t(dst) << 11) | (uint32_t(t) << 1) | 1u;
   __syncthreads();
-  int P = 128;
-  while (P < nev) P <<= 1;
-  for (int i = nev + t; i < P; i += kBalThreads) ev[i] = ~0u;
-  __syncthreads();
-  if (nev > 0) bitonic_lds(ev, P);
+  if (nev > 0) ev = sort_events_by_part(ev, L.ev2, nev, k > 1 ? 32 - __clz(k - 1) : 0, L.rcnt);
   // 4. thread t holds sorted events 2t and 2t + 1; per-part inclusive sums
   const bool ha = 2 * t < nev, hb = 2 * t + 1 < nev;
   int32_t qa = 0, qb = -1, ja = 0, jb = 0, xa = 0, xb = 0;
