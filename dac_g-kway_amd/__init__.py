@@ -67,7 +67,7 @@ class Stats(Structure):
                 ("t_total_ms", c_double), ("alg_bytes", c_int64),
                 ("levels", LevelStats * GK_MAX_LEVELS),
                 ("stage_ms", c_double * GK_ST_COUNT), ("stage_launches", c_int32 * GK_ST_COUNT),
-                ("stage_bytes", c_int64 * GK_ST_COUNT)]
+                ("stage_bytes", c_int64 * GK_ST_COUNT), ("round_paths", c_int64 * 3)]
 
     def stages(self):
         return {name: (self.stage_ms[i], self.stage_launches[i], self.stage_bytes[i])
@@ -92,7 +92,7 @@ class ServeStats(Structure):
                 ("queued", c_int64), ("inflight", c_int64), ("edges_done", c_int64),
                 ("steps", c_int64), ("footprint_now", c_int64), ("footprint_peak", c_int64),
                 ("device_peak", c_int64), ("p50_ms", c_double), ("p99_ms", c_double),
-                ("max_ms", c_double), ("edges_per_s", c_double)]
+                ("max_ms", c_double), ("edges_per_s", c_double), ("isolated", c_int64)]
 
 
 class GenGraph(Structure):
@@ -168,6 +168,7 @@ def lib():
         L.gk_ctx_create.argtypes = [c_int, POINTER(c_void_p)]
         L.gk_ctx_destroy.argtypes = [c_void_p]
         L.gk_ctx_set_stage_mask.argtypes = [c_void_p, ctypes.c_uint32]
+        L.gk_ctx_set_pool_limit.argtypes = [c_void_p, c_int64]
         L.gk_partition.argtypes = [c_void_p, POINTER(Csr), POINTER(Params), I32P, POINTER(Stats)]
         L.gk_partition_device.argtypes = [c_void_p, POINTER(Csr), POINTER(Params), I32P,
                                           POINTER(Stats), c_void_p]
@@ -187,6 +188,8 @@ def lib():
         L.gk_k_apply.argtypes = [VP, POINTER(Csr), I32P, c_int32, I32P, I32P, I32P, I32P,
                                  POINTER(c_int64), VP]
         L.gk_k_cut.argtypes = [VP, POINTER(Csr), c_int32, I32P, POINTER(c_int64), POINTER(c_int64), VP]
+        L.gk_k_round.argtypes = [VP, POINTER(Csr), c_int32, c_int64, c_int32, c_int32, I32P, I32P, I32P, I32P,
+                                 I32P, I32P, I32P, POINTER(c_int64), I32P, POINTER(c_int64), VP]
         L.gk_graph_read.argtypes = [c_char_p, POINTER(Csr)]
         L.gk_graph_write.argtypes = [c_char_p, POINTER(Csr)]
         L.gk_graph_check.argtypes = [POINTER(Csr), c_int]
@@ -379,8 +382,10 @@ class Context:
 class Server:
     """gk_serve_* request loop on its own gk_ctx (the server owns it)."""
 
-    def __init__(self, device=0, hbm_budget=0, max_inflight=0, workers=0):
+    def __init__(self, device=0, hbm_budget=0, max_inflight=0, workers=0, pool_limit=0):
         self._ctx = Context(device)
+        if pool_limit:
+            lib().gk_ctx_set_pool_limit(self._ctx.handle, int(pool_limit))
         cfg = ServeConfig(hbm_budget, max_inflight, workers)
         self._h = c_void_p()
         rc = lib().gk_serve_open(self._ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h))

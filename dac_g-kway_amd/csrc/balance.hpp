@@ -25,6 +25,15 @@ constexpr int kBalMax = 1024;
 
 constexpr int kBalMaxK = 8192;  // parts
 
+// sort_events_by_part ranks digits per wave with a 16 x 16 count table
+// (16 waves x 16 four-bit digits = 256 = 64 lanes x 4 in wave 0's scan);
+// event keys are part << 11 | position << 1 | kind, so positions need 10
+// bits and parts the remaining 21.
+static_assert(kBalThreads == 1024, "event sort assumes 16 waves of 64 lanes");
+static_assert(kBalMax <= 1024, "event key holds positions in 10 bits");
+static_assert(2 * kBalMax <= 2 * kBalThreads, "two events per thread");
+static_assert(kBalMaxK <= (1 << 21), "event key holds parts in 21 bits");
+
 struct BalanceLds {
   uint64_t sk[kBalMax];  // move keys; then the 2 * kBalMax event keys (uint32)
   int32_t sv[kBalMax], sw[kBalMax];

@@ -620,7 +620,7 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     int32_t* __restrict__ rounds, long long* __restrict__ moved, const SelectBufs sb, int32_t* __restrict__ xlist,
     uint32_t* __restrict__ hkey, int32_t* __restrict__ upd, int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
     int32_t* __restrict__ lastcnt, int32_t* __restrict__ active, uint64_t* __restrict__ cand,
-    int32_t* __restrict__ thr) {
+    int32_t* __restrict__ thr, long long* __restrict__ paths, int32_t* __restrict__ order) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
@@ -635,7 +635,9 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   const int32_t* seg = sb.list + R.misc[3];
   int32_t count = R.misc[1] - R.misc[3];
   const int32_t prev = R.misc[2];
-  if (count < M && thr[j] > 1) {
+  const bool deferred = count < M && thr[j] > 1;
+  if (paths && deferred && t == 0) paths[3 * j + 2] += 1;
+  if (deferred) {
     // The threshold left fewer than M selected: test the deferred (lower
     // gain) movable vertices of the job too, into the job's segment of xlist
     // after the selected ones, so that it holds every selected vertex.
@@ -812,6 +814,21 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   if (!narrow) lds_bitonic(R.b, B);
   const int pos = balance_prefix(R.b, dpw, kk[j], B, 0xffffffffull, part, dest, gain, vw, pw + pw_off[j], cap[j],
                                  nullptr, nullptr, &cut[j]);
+  // Optional trace of the round: which ordering path ran (top-M radix
+  // select; compressed keys wider than 32 bits) and the ordered moves with B
+  // and pos.  R.b.sv keeps the ordered vertices through the apply below.
+  if (paths && t == 0 && count > M) {
+    paths[3 * j] += 1;
+    if (!narrow) paths[3 * j + 1] += 1;
+  }
+  if (order) {
+    int32_t* o = order + int64_t(j) * (M + 2);
+    if (t < B) o[2 + t] = R.b.sv[t];
+    if (t == 0) {
+      o[0] = B;
+      o[1] = pos;
+    }
+  }
   // 4. apply moves 0..pos (balance_prefix read part[] before its last barrier):
   // the movers and their neighbours are items [0, tot) -- mover m owns
   // [istart[m], istart[m] + deg + 1), first itself, then its row -- dealt out
@@ -896,12 +913,14 @@ int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, co
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
                    int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const SelectBufs& sb,
                    int32_t* xlist, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
-                   int32_t* active, uint64_t* cand, int32_t* thr, hipStream_t st) {
+                   int32_t* active, uint64_t* cand, int32_t* thr, long long* paths, int32_t* order,
+                   hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
   k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, M, kk, cap, pw_off, g.xadj, g.adj, part, dest,
                                                              gain, g.vw, pw, cut, rounds, moved, sb, xlist, hkey,
-                                                             upd, wl, wlc, lastcnt, active, cand, thr);
+                                                             upd, wl, wlc, lastcnt, active, cand, thr, paths,
+                                                             order);
   return hip_status();
 }
 

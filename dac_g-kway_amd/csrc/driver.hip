@@ -428,10 +428,12 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     const int32_t M = prm.max_moves;
     Buf<int32_t> xlist, dthr, wpos, sscr;
     Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, active, drounds, lastcnt, dvoff, dpw_off, dkk;
-    Buf<long long> dcap, dmoved;
+    Buf<long long> dcap, dmoved, dpaths;
     Buf<uint64_t> cand, movable, selbits, defbits, longbits;
     Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
+    GK_TRY(dpaths.alloc(P, 3));
+    gkd::fill<long long>(dpaths.p, 0, 3, st);
     // gain statistics: a (vertices, entries) pair per block of the gain
     // launches (grids of at most kGainSlots blocks), summed at the end
     constexpr int kGainSlots = 2048;
@@ -515,7 +517,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
         GK_STAGE(ctx, GK_ST_BALANCE, 0,
                  gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p, pw.p,
                                      dcut.p, drounds.p + l, dmoved.p + l, sb, xlist.p, hkey.p, upd.p, wl.p,
-                                     hcnt.p, lastcnt.p, active.p, cand.p, dthr.p, st));
+                                     hcnt.p, lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr, st));
         GK_STAGE(ctx, GK_ST_GAIN, 0,
                  gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
                                      work.p, wl_items, cand.p, st));
@@ -556,9 +558,12 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     }
     std::vector<long long> hcutlv(nl), hmovedlv(nl);
     std::vector<int32_t> hroundslv(nl);
+    long long hpaths[3] = {0, 0, 0};
     GK_TRY(fetch(ctx, st, {{hcutlv.data(), dcutlv.p, sizeof(long long) * size_t(nl - 1)},
                            {hmovedlv.data(), dmoved.p, sizeof(long long) * size_t(nl - 1)},
-                           {hroundslv.data(), drounds.p, sizeof(int32_t) * size_t(nl - 1)}}));
+                           {hroundslv.data(), drounds.p, sizeof(int32_t) * size_t(nl - 1)},
+                           {hpaths, dpaths.p, sizeof hpaths}}));
+    for (int i = 0; i < 3; ++i) S.round_paths[i] = hpaths[i];
     for (int32_t l = 0; l + 1 < nl; ++l) {
       gk_level_stats& ls = S.levels[l];
       const DGraph& g = L[l].g;
@@ -677,6 +682,12 @@ int gk_ctx_create(int device, gk_ctx** out) {
 int gk_ctx_set_stage_mask(gk_ctx* c, uint32_t mask) {
   if (!c) return GK_ERR_ARG;
   c->clock.set_mask(mask);
+  return GK_OK;
+}
+
+int gk_ctx_set_pool_limit(gk_ctx* c, int64_t bytes) {
+  if (!c || bytes < 0) { gkh::set_error("bad argument"); return GK_ERR_ARG; }
+  c->pool.set_limit(bytes);
   return GK_OK;
 }
 
@@ -989,6 +1000,111 @@ int gk_k_apply(gk_ctx* ctx, const gk_csr* dg, const int32_t* moves, int32_t pos,
   GK_TRY(gkd::launch_apply(g, pos + 1, moves, ctl.p, dest, part, upd, wl.p, wlc.p, st));
   GK_HIP(hipStreamSynchronize(st));
   *gain_sum = s;
+  return GK_OK;
+}
+
+int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t max_moves, int32_t nrounds,
+               int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* pw, int32_t* nmoves,
+               int32_t* pos, int64_t* gain_sum, int32_t* order, int64_t* paths, void* stream) {
+  if (!ctx || !dg || !dg->adjwgt || !dg->vwgt || !part || !bnd || !gain || !dest || !pw || !gain_sum ||
+      k < 1 || k > kMaxK || max_moves < 1 || max_moves > 1024 || nrounds < 0) {
+    gkh::set_error("bad argument");
+    return GK_ERR_ARG;
+  }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  *gain_sum = 0;
+  if (paths) paths[0] = paths[1] = paths[2] = 0;
+  DGraph g = dview(dg);
+  if (g.n <= 0 || nrounds == 0) {
+    for (int32_t r = 0; r < nrounds; ++r) {
+      if (nmoves) nmoves[r] = 0;
+      if (pos) pos[r] = -1;
+    }
+    return GK_OK;
+  }
+  Pool& P = ctx->pool;
+  const int32_t n = g.n, M = max_moves;
+  const int64_t nwords = (int64_t(n) + 63) / 64;
+  // The round state of run_vcycle's refinement, for one job.
+  Buf<int32_t> upd, wl, heavy, hcnt, list, xlist, wpos, sscr, dthr, active, drounds, lastcnt, dvoff, dpw_off, dkk,
+      dorder, dmax;
+  Buf<long long> dcap, dcut, dmoved, dpaths;
+  Buf<uint64_t> cand, movable, selbits, defbits, longbits;
+  Buf<uint32_t> hkey;
+  GK_TRY(upd.alloc(P, n));
+  GK_TRY(wl.alloc(P, n));
+  GK_TRY(heavy.alloc(P, n));
+  GK_TRY(hcnt.alloc(P, 2));
+  GK_TRY(list.alloc(P, n));
+  GK_TRY(xlist.alloc(P, n));
+  GK_TRY(hkey.alloc(P, n));
+  GK_TRY(wpos.alloc(P, nwords + 1));
+  GK_TRY(sscr.alloc(P, gkd::select_scratch_ints(n)));
+  GK_TRY(cand.alloc(P, nwords + 1));
+  GK_TRY(movable.alloc(P, nwords + 1));
+  GK_TRY(selbits.alloc(P, nwords + 1));
+  GK_TRY(defbits.alloc(P, nwords + 1));
+  GK_TRY(longbits.alloc(P, nwords + 1));
+  GK_TRY(dthr.alloc(P, 1));
+  GK_TRY(active.alloc(P, 1));
+  GK_TRY(drounds.alloc(P, 1));
+  GK_TRY(lastcnt.alloc(P, 1));
+  GK_TRY(dvoff.alloc(P, 2));
+  GK_TRY(dpw_off.alloc(P, 2));
+  GK_TRY(dkk.alloc(P, 1));
+  GK_TRY(dorder.alloc(P, int64_t(M) + 2));
+  GK_TRY(dmax.alloc(P, 1));
+  GK_TRY(dcap.alloc(P, 1));
+  GK_TRY(dcut.alloc(P, 1));
+  GK_TRY(dmoved.alloc(P, 1));
+  GK_TRY(dpaths.alloc(P, 3));
+  gkd::fill<int32_t>(upd.p, 0, n, st);
+  gkd::fill<int32_t>(hcnt.p, 0, 2, st);
+  gkd::fill<uint64_t>(selbits.p, 0ull, nwords + 1, st);
+  gkd::fill<int32_t>(dthr.p, 1, 1, st);
+  gkd::fill<int32_t>(drounds.p, 0, 1, st);
+  gkd::fill<int32_t>(lastcnt.p, 1, 1, st);
+  gkd::fill<int32_t>(dvoff.p, 0, 1, st);
+  gkd::fill<int32_t>(dvoff.p + 1, n, 1, st);
+  gkd::fill<int32_t>(dpw_off.p, 0, 1, st);
+  gkd::fill<int32_t>(dpw_off.p + 1, k, 1, st);
+  gkd::fill<int32_t>(dkk.p, k, 1, st);
+  gkd::fill<long long>(dcap.p, static_cast<long long>(cap), 1, st);
+  gkd::fill<long long>(dcut.p, 0, 1, st);
+  gkd::fill<long long>(dmoved.p, 0, 1, st);
+  gkd::fill<long long>(dpaths.p, 0, 3, st);
+  GK_TRY(gkd::launch_cand_init(n, bnd, gain, cand.p, st));
+  GK_TRY(gkd::launch_max_deg(g, dmax.p, st));
+  int32_t maxdeg = 0;
+  GK_TRY(fetch(ctx, st, {{&maxdeg, dmax.p, sizeof maxdeg}}));
+  const int lanes = gain_lanes(g);
+  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
+  for (int32_t r = 0; r < nrounds; ++r) {
+    // B = 0, pos = -1 unless the round orders any move.
+    gkd::fill<int32_t>(dorder.p, 0, 1, st);
+    gkd::fill<int32_t>(dorder.p + 1, -1, 1, st);
+    GK_TRY(gkd::launch_select_list(g, nullptr, gain, dest, pw, dpw_off.p, dcap.p, dthr.p, cand.p, sb, hcnt.p,
+                                   active.p, maxdeg, st));
+    GK_TRY(gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, part, dest, gain, pw, dcut.p,
+                               drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p, lastcnt.p, active.p,
+                               cand.p, dthr.p, dpaths.p, dorder.p, st));
+    GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, part, upd.p, bnd, gain, dest, nullptr, n,
+                                cand.p, st));
+    if (maxdeg > lanes)
+      GK_TRY(gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, part, upd.p, bnd, gain, dest, nullptr, cand.p, st));
+    int32_t hdr[2] = {0, -1};
+    GK_TRY(fetch(ctx, st, {{hdr, dorder.p, sizeof hdr}}));
+    if (nmoves) nmoves[r] = hdr[0];
+    if (pos) pos[r] = hdr[1];
+    if (order && hdr[0] > 0)
+      GK_HIP(hipMemcpyAsync(order, dorder.p + 2, sizeof(int32_t) * size_t(hdr[0]), hipMemcpyDeviceToDevice, st));
+  }
+  long long hcut = 0, hp[3] = {0, 0, 0};
+  GK_TRY(fetch(ctx, st, {{&hcut, dcut.p, sizeof hcut}, {hp, dpaths.p, sizeof hp}}));
+  *gain_sum = -hcut;
+  if (paths)
+    for (int i = 0; i < 3; ++i) paths[i] = hp[i];
   return GK_OK;
 }
 

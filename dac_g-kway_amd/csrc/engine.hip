@@ -562,7 +562,7 @@ int Engine::refine_step(bool* worked) {
   const int32_t N = g.n;
   Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, dmax, xlist, dthr, wpos,
       sscr;
-  Buf<long long> dcap, dcut, dmoved;
+  Buf<long long> dcap, dcut, dmoved, dpaths;
   Buf<uint64_t> cand, movable, selbits, defbits, longbits;
   Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
@@ -607,6 +607,8 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(dmoved.alloc(P_, J));
   gkd::fill<int32_t>(drounds.p, 0, J, st_);
   gkd::fill<long long>(dmoved.p, 0, J, st_);
+  GK_TRY(dpaths.alloc(P_, 3LL * J));
+  gkd::fill<long long>(dpaths.p, 0, 3LL * J, st_);
   GK_STAGE(ctx_, GK_ST_PROJECT, 44LL * N,
            gkd::launch_project_b(N, rs.cmap.p, rs.cpart.p, rs.cbnd.p, rs.cgain.p, rs.cdest.p, part->p, bnd->p,
                                  gain.p, dest.p, upd.p, st_));
@@ -648,7 +650,7 @@ int Engine::refine_step(bool* worked) {
     GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
              gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
                                  dpw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                 lastcnt.p, active.p, cand.p, dthr.p, st_));
+                                 lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr, st_));
     GK_STAGE(ctx_, GK_ST_GAIN, 0,
              gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
                                  nullptr, wl_items, cand.p, st_));
@@ -685,9 +687,10 @@ int Engine::refine_step(bool* worked) {
     if (done_rounds >= 32 && per_check < 32) per_check *= 2;
   }
   prof_ms_[5] += now_ms() - tr0;
-  std::vector<long long> hcut(J), hmoved(J);
+  std::vector<long long> hcut(J), hmoved(J), hpaths(3 * size_t(J));
   std::vector<int32_t> hrounds(J), hlast(J);
-  GK_TRY(download({{hcut.data(), dcut.p, sizeof(long long) * J},
+  GK_TRY(download({{hpaths.data(), dpaths.p, sizeof(long long) * 3 * J},
+                   {hcut.data(), dcut.p, sizeof(long long) * J},
                    {hpw.data(), dpw.p, sizeof(int32_t) * hpw.size()},
                    {hrounds.data(), drounds.p, sizeof(int32_t) * J},
                    {hmoved.data(), dmoved.p, sizeof(long long) * J},
@@ -700,6 +703,7 @@ int Engine::refine_step(bool* worked) {
     gk_level_stats& ls = j.S.levels[j.cur];
     ls.rounds += hrounds[i];
     ls.moves += hmoved[i];
+    for (int q = 0; q < 3; ++q) j.S.round_paths[q] += hpaths[3 * size_t(i) + q];
     ls.cut_after = hcut[i];
     j.cut = hcut[i];
     std::copy(hpw.begin() + pw_off[i], hpw.begin() + pw_off[i + 1], j.hpw.begin());

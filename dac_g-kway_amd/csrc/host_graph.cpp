@@ -170,21 +170,28 @@ int write_metis(const std::string& path, const HostGraph& g, std::string& err) {
   else
     fprintf(f, "%d %lld\n", g.n, static_cast<long long>(g.nnz() / 2));
   char line[64];
+  bool bad = false;
   for (int32_t v = 0; v < g.n; ++v) {
     bool first = true;
-    if (g.has_vwgt) { fprintf(f, "%d", g.vwgt[v]); first = false; }
+    if (g.has_vwgt) {
+      if (fprintf(f, "%d", g.vwgt[v]) < 0) bad = true;
+      first = false;
+    }
     for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
       int len;
       if (g.has_ewgt)
         len = snprintf(line, sizeof line, first ? "%d %d" : " %d %d", g.adjncy[j] + 1, g.adjwgt[j]);
       else
         len = snprintf(line, sizeof line, first ? "%d" : " %d", g.adjncy[j] + 1);
-      fwrite(line, 1, len, f);
+      if (fwrite(line, 1, len, f) != size_t(len)) bad = true;
       first = false;
     }
-    fputc('\n', f);
+    if (fputc('\n', f) == EOF) bad = true;
+    if (bad) break;
   }
-  if (fclose(f) != 0) { err = "write failed: " + path; return GK_ERR_IO; }
+  // A failed flush of the buffer (a full disk, say) shows up in ferror or fclose.
+  if (ferror(f)) bad = true;
+  if (fclose(f) != 0 || bad) { err = "write failed: " + path; return GK_ERR_IO; }
   return GK_OK;
 }
 
@@ -243,11 +250,14 @@ int write_partition(const std::string& path, const int32_t* part, int32_t n, std
   std::vector<char> buf(1 << 20);
   setvbuf(f, buf.data(), _IOFBF, buf.size());
   char line[16];
-  for (int32_t v = 0; v < n; ++v) {
+  bool bad = false;
+  for (int32_t v = 0; v < n && !bad; ++v) {
     int len = snprintf(line, sizeof line, "%d\n", part[v]);
-    fwrite(line, 1, len, f);
+    if (fwrite(line, 1, len, f) != size_t(len)) bad = true;
   }
-  if (fclose(f) != 0) { err = "write failed: " + path; return GK_ERR_IO; }
+  // A failed flush of the buffer (a full disk, say) shows up in ferror or fclose.
+  if (ferror(f)) bad = true;
+  if (fclose(f) != 0 || bad) { err = "write failed: " + path; return GK_ERR_IO; }
   return GK_OK;
 }
 

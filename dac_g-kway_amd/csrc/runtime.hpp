@@ -33,8 +33,13 @@ class Pool {
   int get(size_t bytes, void** out) {
     bytes = (bytes + 255) & ~size_t(255);
     if (bytes == 0) bytes = 256;
+    if (limit_ > 0 && in_use_ + static_cast<int64_t>(bytes) > limit_) {
+      gkh::set_error("device pool limit of " + std::to_string(limit_) + " bytes reached (" +
+                     std::to_string(in_use_) + " in use, " + std::to_string(bytes) + " requested)");
+      return GK_ERR_NOMEM;
+    }
     auto it = free_.lower_bound(bytes);
-    if (it != free_.end() && it->first <= 2 * bytes) {
+    if (it != free_.end() && it->first <= 2 * bytes && (limit_ <= 0 || in_use_ + int64_t(it->first) <= limit_)) {
       *out = it->second;
       note_out(it->first);
       free_.erase(it);
@@ -65,6 +70,9 @@ class Pool {
   }
   int64_t in_use() const { return in_use_; }
   int64_t peak() const { return peak_; }
+  // Bytes the pool may hand out at once (0 = no limit); a request beyond it
+  // fails with GK_ERR_NOMEM.
+  void set_limit(int64_t bytes) { limit_ = bytes > 0 ? bytes : 0; }
   void reset_peak() { peak_ = in_use_; }
   void trim() {
     for (auto& kv : free_) {
@@ -88,6 +96,7 @@ class Pool {
   std::multimap<size_t, void*> free_;
   std::unordered_map<void*, size_t> size_;
   int64_t in_use_ = 0, peak_ = 0;  // bytes handed out now / at most
+  int64_t limit_ = 0;
 };
 
 template <class T>

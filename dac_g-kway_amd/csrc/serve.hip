@@ -38,6 +38,8 @@ struct Request {
   int rc = GK_OK;
   std::string err;
   Buf<int32_t> dout;  // device partition while running
+  bool claimed = false;  // a gk_serve_wait caller owns the result
+  bool alone = false;    // re-run by itself after a step of several failed
 };
 
 }  // namespace
@@ -52,12 +54,12 @@ struct gk_server {
   std::deque<Request*> queue;
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
   int64_t next_id = 0;
+  int waiters = 0;      // callers inside gk_serve_wait
+  bool alone_running = false;  // an isolated re-run is on the device
   bool stop = false;
-  int fatal = GK_OK;
-  std::string fatal_err;
   bool have_shape = false;
   int32_t max_group = 0, max_moves = 0;
-  int64_t submitted = 0, completed = 0, failed = 0, inflight = 0, edges_done = 0, steps = 0;
+  int64_t submitted = 0, completed = 0, failed = 0, inflight = 0, edges_done = 0, steps = 0, isolated = 0;
   int64_t foot_now = 0, foot_peak = 0, device_peak = 0;
   std::vector<double> lat;
   double first_submit = -1, last_done = 0;
@@ -91,7 +93,7 @@ void finish_locked(gk_server* s, Request* r, int rc, const std::string& err) {
   s->cv_out.notify_all();
 }
 
-int admit(gk_server* s, Engine& E, Request* r) {
+int admit(gk_server* s, Engine* E, Request* r) {
   Pool& P = s->ctx->pool;
   hipStream_t st = s->ctx->stream;
   auto S0 = std::make_shared<Store>();
@@ -100,7 +102,7 @@ int admit(gk_server* s, Engine& E, Request* r) {
   S0->nnz = r->nnz;
   S0->v_off = {0, r->n};
   S0->e_off = {0, r->nnz};
-  GK_TRY(E.upload(S0->dv_off, S0->v_off.data(), 2));
+  GK_TRY(E->upload(S0->dv_off, S0->v_off.data(), 2));
   GK_TRY(S0->bx.alloc(P, int64_t(r->n) + 1));
   GK_TRY(S0->ba.alloc(P, r->nnz));
   GK_TRY(S0->bw.alloc(P, r->nnz));
@@ -119,7 +121,7 @@ int admit(gk_server* s, Engine& E, Request* r) {
   S0->vw = S0->bv.p;
   auto j = std::make_unique<Job>();
   j->id = static_cast<int32_t>(r->id & 0x7fffffff);
-  GK_TRY(E.job_setup(*j, r->prm, r->T, r->n));
+  GK_TRY(E->job_setup(*j, r->prm, r->T, r->n));
   j->out = r->dout.p;
   j->user = r;
   JLevel l0;
@@ -128,7 +130,7 @@ int admit(gk_server* s, Engine& E, Request* r) {
   l0.store = std::move(S0);
   l0.slot = 0;
   j->L.push_back(std::move(l0));
-  return E.add_job(std::move(j));
+  return E->add_job(std::move(j));
 }
 
 void serve_loop(gk_server* s) {
@@ -136,29 +138,37 @@ void serve_loop(gk_server* s) {
   hipStream_t st = s->ctx->stream;
   gk_params wp{};
   wp.initpart_threads = s->cfg.workers;
-  Engine E(s->ctx, st, batch_workers(wp));
-  E.done_cb_ = [s, st](Job& j) -> int {
-    Request* r = static_cast<Request*>(j.user);
-    r->part.resize(r->n);
-    if (r->n) GK_HIP(hipMemcpyAsync(r->part.data(), r->dout.p, sizeof(int32_t) * size_t(r->n), hipMemcpyDeviceToHost, st));
-    GK_HIP(hipStreamSynchronize(st));
-    r->dout.release();
-    r->S = j.S;
-    std::lock_guard<std::mutex> lk(s->m);
-    s->foot_now -= r->foot;
-    --s->inflight;
-    finish_locked(s, r, j.rc, j.err);
-    return GK_OK;
+  auto make_engine = [s, st, &wp]() {
+    auto E = std::make_unique<Engine>(s->ctx, st, batch_workers(wp));
+    E->done_cb_ = [s, st](Job& j) -> int {
+      Request* r = static_cast<Request*>(j.user);
+      r->part.resize(r->n);
+      if (r->n)
+        GK_HIP(hipMemcpyAsync(r->part.data(), r->dout.p, sizeof(int32_t) * size_t(r->n), hipMemcpyDeviceToHost, st));
+      GK_HIP(hipStreamSynchronize(st));
+      r->dout.release();
+      r->S = j.S;
+      std::lock_guard<std::mutex> lk(s->m);
+      s->foot_now -= r->foot;
+      --s->inflight;
+      if (r->alone) s->alone_running = false;
+      finish_locked(s, r, j.rc, j.err);
+      return GK_OK;
+    };
+    return E;
   };
-  int rc = GK_OK;
+  std::unique_ptr<Engine> E = make_engine();
+  std::vector<Request*> running;  // admitted and not finished (loop thread only)
   while (true) {
     std::vector<Request*> adm;
     {
       std::unique_lock<std::mutex> lk(s->m);
-      if (E.live() == 0) s->cv_in.wait(lk, [s] { return s->stop || !s->queue.empty(); });
-      if (s->stop && s->queue.empty() && E.live() == 0) break;
-      while (!s->queue.empty()) {
+      if (E->live() == 0) s->cv_in.wait(lk, [s] { return s->stop || !s->queue.empty(); });
+      if (s->stop && s->queue.empty() && E->live() == 0) break;
+      while (!s->queue.empty() && !s->alone_running) {
         Request* r = s->queue.front();
+        // An isolated re-run waits for an empty device and runs by itself.
+        if (r->alone && s->inflight > 0) break;
         const bool fits = s->foot_now + r->foot <= s->budget || s->inflight == 0;
         const bool slot = s->cfg.max_inflight <= 0 || s->inflight < s->cfg.max_inflight;
         if (!fits || !slot) break;
@@ -168,42 +178,61 @@ void serve_loop(gk_server* s) {
         ++s->inflight;
         r->state = 1;
         adm.push_back(r);
+        if (r->alone) {
+          s->alone_running = true;
+          break;
+        }
       }
     }
+    int rc = GK_OK;
     for (Request* r : adm) {
-      rc = admit(s, E, r);
+      running.push_back(r);
+      rc = admit(s, E.get(), r);
       if (rc != GK_OK) break;
     }
-    if (rc == GK_OK && E.live() > 0) {
+    if (rc == GK_OK && E->live() > 0) {
       bool worked = false;
-      rc = E.step(&worked);
-      E.drop_done();
+      rc = E->step(&worked);
+      E->drop_done();
     }
+    running.erase(std::remove_if(running.begin(), running.end(), [](Request* r) { return r->state == 2; }),
+                  running.end());
     {
       std::lock_guard<std::mutex> lk(s->m);
       ++s->steps;
       s->device_peak = std::max(s->device_peak, s->ctx->pool.peak());
     }
-    if (rc != GK_OK) break;
+    if (rc == GK_OK) continue;
+    // A failed admission or step.  The engine and its jobs are dropped (their
+    // device buffers return to the pool) and a new engine takes over.  A
+    // request that failed by itself is finished with the error; when several
+    // were on the device, each is queued again, in order, to run alone, so
+    // that a failure only reaches the request that causes it.
+    const std::string msg = gk_last_error();
+    (void)hipStreamSynchronize(st);
+    (void)hipGetLastError();
+    E.reset();
+    E = make_engine();
+    std::lock_guard<std::mutex> lk(s->m);
+    const bool several = running.size() > 1;
+    for (auto it = running.rbegin(); it != running.rend(); ++it) {
+      Request* r = *it;
+      r->dout.release();
+      s->foot_now -= r->foot;
+      --s->inflight;
+      if (!several || r->alone) {
+        finish_locked(s, r, rc, msg);
+      } else {
+        r->alone = true;
+        r->state = 0;
+        ++s->isolated;
+        s->queue.push_front(r);
+      }
+    }
+    running.clear();
+    s->alone_running = false;
   }
   (void)hipStreamSynchronize(st);
-  if (rc != GK_OK) {
-    // A device or engine failure ends the loop: every request not finished
-    // yet fails with the same status.  Device buffers go back to the pool
-    // here, on the thread that owns the pool.
-    const std::string msg = gk_last_error();
-    std::lock_guard<std::mutex> lk(s->m);
-    s->fatal = rc;
-    s->fatal_err = msg;
-    for (auto& kv : s->reqs)
-      if (kv.second->state != 2) {
-        kv.second->dout.release();
-        finish_locked(s, kv.second.get(), rc, msg);
-      }
-    s->queue.clear();
-    s->inflight = 0;
-    s->foot_now = 0;
-  }
 }
 
 int take_result(gk_server* s, std::unordered_map<int64_t, std::unique_ptr<Request>>::iterator it,
@@ -281,7 +310,6 @@ int gk_serve_submit(gk_server* s, const gk_csr* g, const gk_params* p, int64_t* 
   if (r->T >= (int64_t(1) << 31)) { gkh::set_error("total vertex weight >= 2^31"); return GK_ERR_LIMIT; }
   r->foot = gk_serve_footprint(g->n, g->nnz, p);
   std::lock_guard<std::mutex> lk(s->m);
-  if (s->fatal != GK_OK) { gkh::set_error("server stopped: " + s->fatal_err); return s->fatal; }
   if (s->stop) { gkh::set_error("server is closing"); return GK_ERR_ARG; }
   if (!s->have_shape) {
     s->have_shape = true;
@@ -311,7 +339,8 @@ int gk_serve_poll(gk_server* s, int64_t request, int32_t* part_out, gk_stats* st
   if (!s) { gkh::set_error("null server"); return GK_ERR_ARG; }
   std::lock_guard<std::mutex> lk(s->m);
   auto it = s->reqs.find(request);
-  if (it == s->reqs.end()) { gkh::set_error("unknown request"); return GK_ERR_ARG; }
+  if (it == s->reqs.end()) { gkh::set_error("unknown request (or its result was already taken)"); return GK_ERR_ARG; }
+  if (it->second->claimed) { gkh::set_error("request is being waited on by another caller"); return GK_ERR_ARG; }
   if (it->second->state != 2) return 0;
   const int rc = take_result(s, it, part_out, stats);
   return rc == GK_OK ? 1 : rc;
@@ -321,10 +350,18 @@ int gk_serve_wait(gk_server* s, int64_t request, int32_t* part_out, gk_stats* st
   if (!s) { gkh::set_error("null server"); return GK_ERR_ARG; }
   std::unique_lock<std::mutex> lk(s->m);
   auto it = s->reqs.find(request);
-  if (it == s->reqs.end()) { gkh::set_error("unknown request"); return GK_ERR_ARG; }
+  if (it == s->reqs.end()) { gkh::set_error("unknown request (or its result was already taken)"); return GK_ERR_ARG; }
   Request* r = it->second.get();
+  if (r->claimed) { gkh::set_error("request is being waited on by another caller"); return GK_ERR_ARG; }
+  // Claimed: no other caller can take (and free) the request while this one
+  // sleeps; gk_serve_close waits for every waiter to leave.
+  r->claimed = true;
+  ++s->waiters;
   s->cv_out.wait(lk, [r] { return r->state == 2; });
-  return take_result(s, s->reqs.find(request), part_out, stats);
+  const int rc = take_result(s, s->reqs.find(request), part_out, stats);
+  --s->waiters;
+  s->cv_out.notify_all();
+  return rc;
 }
 
 int gk_serve_query(gk_server* s, gk_serve_stats* out) {
@@ -341,6 +378,7 @@ int gk_serve_query(gk_server* s, gk_serve_stats* out) {
   out->footprint_now = s->foot_now;
   out->footprint_peak = s->foot_peak;
   out->device_peak = s->device_peak;
+  out->isolated = s->isolated;
   if (!s->lat.empty()) {
     std::vector<double> l = s->lat;
     std::sort(l.begin(), l.end());
@@ -366,9 +404,14 @@ int gk_serve_close(gk_server* s) {
   }
   s->cv_in.notify_all();
   if (s->th.joinable()) s->th.join();
-  const int rc = s->fatal;
+  {
+    // Every request has finished; callers still inside gk_serve_wait take
+    // their results before the server goes away.
+    std::unique_lock<std::mutex> lk(s->m);
+    s->cv_out.wait(lk, [s] { return s->waiters == 0; });
+  }
   delete s;
-  return rc;
+  return GK_OK;
 }
 
 }  // extern "C"

@@ -23,6 +23,8 @@
  *   gk_k_balance           find_mv_wgt_sequence + scan + find_balance_sequence
  *                                                        gkway/uncoarsen.hpp:239-288,37-80
  *   gk_k_apply             apply_sequence_move           gkway/uncoarsen.hpp:290-323
+ *   gk_k_round             one pass of the refinement loop body: the above
+ *                          four plus update_vertex_info  gkway/uncoarsen.hpp:456-506
  *   gk_k_cut               calculate_cutsize + calculate_partition_wgt
  *                                                        gkway/uncoarsen.hpp:398-424,
  *                                                        gkway/cuda_check.hpp:61-73
@@ -153,6 +155,11 @@ typedef struct gk_stats {
   double stage_ms[GK_ST_COUNT];
   int32_t stage_launches[GK_ST_COUNT];
   int64_t stage_bytes[GK_ST_COUNT];
+  /* Refinement rounds by the ordering path they took (all levels): [0] more
+   * than max_moves selected (exact top-max_moves radix select), [1] of those,
+   * compressed keys wider than 32 bits (64-bit key sort), [2] rounds that
+   * tested vertices deferred by the previous round's gain threshold. */
+  int64_t round_paths[3];
 } gk_stats;
 
 /* ---- context ------------------------------------------------------------ */
@@ -163,6 +170,12 @@ int gk_ctx_destroy(gk_ctx* ctx);
    (GK_ST_*); default all.  Timing one stage keeps the event overhead out of
    the others.  No counterpart in the reference (it has no stage timers). */
 int gk_ctx_set_stage_mask(gk_ctx* ctx, uint32_t mask);
+/* Caps the device memory the context's pool may hand out at once (bytes;
+   0 = no cap, the default).  An allocation beyond it fails with
+   GK_ERR_NOMEM, as a failed hipMalloc does.  No counterpart in the reference
+   (it allocates per call, gkway/graph_partitioner.hpp:253-273); for sharing a
+   device between tenants, and for forcing allocation failures in tests. */
+int gk_ctx_set_pool_limit(gk_ctx* ctx, int64_t bytes);
 const char* gk_strerror(int status);
 const char* gk_last_error(void);
 
@@ -239,20 +252,29 @@ typedef struct gk_serve_stats {
   int64_t device_peak;     /* peak device bytes held by the context's pool   */
   double p50_ms, p99_ms, max_ms;  /* latency = completion - submission      */
   double edges_per_s;      /* edges_done / (last completion - first submit)  */
+  int64_t isolated;        /* re-runs of requests alone after a failed step  */
 } gk_serve_stats;
 
 int gk_serve_open(gk_ctx* ctx, const gk_serve_config* cfg, gk_server** out);
 /* Copies the host graph; *request receives the request id. */
 int gk_serve_submit(gk_server* s, const gk_csr* graph, const gk_params* params, int64_t* request);
 /* 1 = finished (part_out[n] and *stats written, the request is released),
- * 0 = still queued or running, < 0 = the request failed (also released). */
+ * 0 = still queued or running, < 0 = the request failed (also released).
+ * GK_ERR_ARG for an unknown id, a result already taken, or a request another
+ * caller is waiting on: each result is taken exactly once.
+ * Failures are per request: when a step shared by several requests fails
+ * (an allocation beyond the device memory, say), each of them is run again
+ * by itself, and only a request that fails alone reports the error. */
 int gk_serve_poll(gk_server* s, int64_t request, int32_t* part_out, gk_stats* stats);
-/* Blocks until the request finishes; GK_OK or its error. */
+/* Blocks until the request finishes; GK_OK or its error.  The waiting
+ * caller owns the request: a second wait or a poll on it gets GK_ERR_ARG. */
 int gk_serve_wait(gk_server* s, int64_t request, int32_t* part_out, gk_stats* stats);
 int gk_serve_query(gk_server* s, gk_serve_stats* out);
 /* Footprint estimate used for admission (bytes) of a graph at parameters p. */
 int64_t gk_serve_footprint(int32_t n, int32_t nnz, const gk_params* p);
-/* Finishes every queued and running request, then stops the loop. */
+/* Finishes every queued and running request, waits for the callers inside
+ * gk_serve_wait to take their results, then stops the loop and frees the
+ * server.  Call it once, after which no other call may use s. */
 int gk_serve_close(gk_server* s);
 
 /* ---- host I/O --------------------------------------------------------------
@@ -325,6 +347,23 @@ int gk_k_balance(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* moves,
 int gk_k_apply(gk_ctx* ctx, const gk_csr* dg, const int32_t* moves, int32_t pos,
                const int32_t* gain, const int32_t* dest, int32_t* part, int32_t* updated,
                int64_t* gain_sum, void* stream);
+/* nrounds refinement rounds exactly as the V-cycle runs them, with the
+ * product's round kernels: per round the movable set and its independent
+ * selection, the best min(selected, max_moves) moves by (gain desc, id asc)
+ * (all of them, or a top-max_moves radix select), the largest balanced
+ * prefix, the apply, then the gain pass over the updated vertices.  The state
+ * part/bnd/gain/dest (n ints) and pw (k ints) is device memory, read and
+ * advanced in place; gain[v] > 0 only where bnd[v] != 0 (as the gain pass
+ * leaves it).  The first round tests every candidate; later rounds use the
+ * gain threshold the previous round set, as in the V-cycle.  Per round r
+ * (host arrays of nrounds, may be NULL): nmoves[r] = moves ordered (0 when
+ * none was selected), pos[r] = last applied position (-1: none).
+ * *gain_sum = summed gain of every applied move.  order (device, may be
+ * NULL): max_moves ints, the ordered moves of the last round that selected
+ * any.  paths (host, may be NULL): as gk_stats.round_paths over these rounds. */
+int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t max_moves, int32_t nrounds,
+               int32_t* part, int32_t* bnd, int32_t* gain, int32_t* dest, int32_t* pw, int32_t* nmoves,
+               int32_t* pos, int64_t* gain_sum, int32_t* order, int64_t* paths, void* stream);
 /* Weighted cut over entries (v,u), v<u, part differs; part weights pw[k]. */
 int gk_k_cut(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part,
              int64_t* cut, int64_t* pw, void* stream);

@@ -360,3 +360,87 @@ def test_generators_structure(gk):
     # same seed, same graph
     g2 = gk.dag(20000, 100, 3)
     assert np.array_equal(g.adjncy, g2.adjncy) and np.array_equal(g.adjwgt, g2.adjwgt)
+
+
+# ------------------------------------------------------------------ C++ mirror of the reference API
+GRAPH_PROBE = r'''
+#include <cstdio>
+#include "gkway/graph.hpp"
+int main(int argc, char** argv) {
+  gk::Graph g(argv[1]);
+  const size_t n = g.get_num_vertex(), e = g.get_num_edge();
+  std::printf("%zu %zu %d\n", n, e, g.get_total_vertex_weight());
+  for (size_t v = 0; v <= n; ++v) std::printf("%d ", g.get_adjp()[v]);
+  std::printf("\n");
+  for (size_t j = 0; j < e; ++j) std::printf("%d ", g.get_adjncy()[j]);
+  std::printf("\n");
+  for (size_t j = 0; j < e; ++j) std::printf("%d ", g.get_adjncy_source()[j]);
+  std::printf("\n");
+  for (size_t j = 0; j < e; ++j) std::printf("%d ", g.get_adjwgt()[j]);
+  std::printf("\n");
+  for (size_t v = 0; v < n; ++v) std::printf("%d ", g.get_vwgt()[v]);
+  std::printf("\n");
+  std::vector<int> part(n);
+  for (size_t v = 0; v < n; ++v) part[v] = int(v % 3);
+  g.dump_result(part, argv[2]);
+  return 0;
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def graph_probe(tmp_path_factory):
+    d = tmp_path_factory.mktemp("probe")
+    src = d / "probe.cpp"
+    src.write_text(GRAPH_PROBE)
+    exe = d / "probe"
+    lib = os.path.join(ROOT, "dac_g-kway_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L", lib, "-lgkway", f"-Wl,-rpath,{lib}"], check=True)
+    return exe
+
+
+def test_cpp_graph_mirrors_reference_conventions(gk, graph_probe, tmp_path):
+    """gk::Graph on the sample graph: the reference's getters (graph.hpp:146-183)
+    -- xadj, 1-based neighbour ids as in the file, 1-based source ids per
+    entry, unit edge and vertex weights, 2 x m entries -- and dump_result
+    writing <cwd>/<name>.out."""
+    path = os.path.join(GOLDEN, "delaunay_n11.graph")
+    out = subprocess.run([str(graph_probe), path, "probe_out"], cwd=tmp_path, capture_output=True, text=True,
+                         check=True).stdout.splitlines()
+    n, e, tw = map(int, out[0].split())
+    g = gk.read_metis(path)
+    assert (n, e, tw) == (2048, 2 * 6127, 2048) == (g.n, g.nnz, int(g.vwgt.sum()))
+    arr = [np.array(line.split(), dtype=np.int64) for line in out[1:6]]
+    assert np.array_equal(arr[0], g.xadj)
+    assert np.array_equal(arr[1], g.adjncy + 1)
+    assert np.array_equal(arr[2], np.repeat(np.arange(g.n), g.degrees()) + 1)
+    assert (arr[3] == 1).all() and (arr[4] == 1).all()
+    # the file's own first row, 1-based as written
+    with open(path) as f:
+        rows = [ln for ln in f.read().splitlines() if not ln.startswith("%")]
+    assert list(arr[1][: g.xadj[1]]) == [int(t) for t in rows[1].split()]
+    assert (tmp_path / "probe_out.out").read_text() == "".join(f"{v % 3}\n" for v in range(n))
+
+
+def test_cpp_graph_weighted_file(gk, graph_probe, tmp_path):
+    # fmt 011: vertex and edge weights are read and returned as given
+    p = tmp_path / "w.graph"
+    p.write_text("3 2 011\n5 2 7\n1 1 7 3 9\n2 2 9\n")
+    out = subprocess.run([str(graph_probe), str(p), "w"], cwd=tmp_path, capture_output=True, text=True,
+                         check=True).stdout.splitlines()
+    assert out[0].split() == ["3", "4", "8"]
+    assert out[2].split() == ["2", "1", "3", "2"]
+    assert out[3].split() == ["1", "2", "2", "3"]
+    assert out[4].split() == ["7", "7", "9", "9"]
+    assert out[5].split() == ["5", "1", "2"]
+
+
+@pytest.mark.skipif(not os.path.exists("/dev/full"), reason="needs /dev/full")
+def test_write_failures_are_reported(gk):
+    # Writes that fail when the buffer is flushed (no space left) return GK_ERR_IO.
+    part = np.zeros(1 << 20, np.int32)
+    L = gk.lib()
+    assert L.gk_write_partition(b"/dev/full", part.ctypes.data_as(gk.I32P), len(part)) == -2
+    g = gk.delaunay(3000, 1)
+    assert L.gk_graph_write(b"/dev/full", ctypes.byref(g.csr())) == -2

@@ -128,3 +128,17 @@ def test_autotune_same_result(gk, oracle, ctx):
             assert (a.h_cand, a.h_cmap, a.h_coarse, a.h_part) == (b.h_cand, b.h_cmap, b.h_coarse, b.h_part), (rep, l)
     opart, _ = oracle.partition(g, 64)
     assert np.array_equal(base, opart)
+
+
+# A circuit-timing-shaped DAG at 2^21 vertices (200 layers, edge weights
+# 1..100, vertex weights 1..4, k = 256, 1024 moves per round): thousands of
+# rounds, most selecting more than 1024 moves, so the top-1024 radix select of
+# the round kernel runs throughout; every level's trace equals the oracle's.
+@pytest.mark.slow
+def test_dag_2m_k256(gk, oracle, ctx):
+    g = gk.dag(1 << 21, 200, 3)
+    st = compare(gk, oracle, ctx, g, 256)
+    paths = list(st.round_paths)
+    print(f"round paths (top-M select, 64-bit keys, deferred tests): {paths}", flush=True)
+    assert paths[0] > 1000
+    assert st.max_part_wgt <= st.cap

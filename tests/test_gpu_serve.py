@@ -1,7 +1,7 @@
 """Serving form (gk_serve_*): requests submitted while others are on the GPU
 are admitted by footprint, stepped together with them, and each result must
 equal, bit for bit, the same graph partitioned alone (gk_partition on another
-context)."""
+context) and the oracle's partition of it."""
 import time
 
 import numpy as np
@@ -13,6 +13,11 @@ pytestmark = pytest.mark.gpu
 
 
 def check_alone(ctx, g, k, part, st):
+    # alone on another context, and against the oracle
+    import _oracle
+    op, ost = _oracle.partition(g, k)
+    assert np.array_equal(part, op), f"oracle n={g.n} k={k}"
+    assert st.cut == ost.cut
     p1, s1 = ctx.partition(g, k)
     assert np.array_equal(part, p1), f"n={g.n} k={k}"
     assert (st.cut, st.nlevels, st.coarsest_n, st.init_cut) == (s1.cut, s1.nlevels, s1.coarsest_n, s1.init_cut)
@@ -91,3 +96,66 @@ def test_serve_rejects_other_shape(gk):
         srv.wait(srv.submit(g, 4))
         with pytest.raises(RuntimeError):
             srv.submit(g, 4, max_moves=256)
+
+
+# Failures stay with the request that causes them.  A device pool limit below
+# what the large graph needs makes its allocations fail (admission at 0.9 x
+# its CSR bytes, a later step at 1.3 x) while several requests share the
+# device; the loop then runs each of them again alone: the small ones finish
+# (equal to alone and the oracle), only the large one reports the error.
+@pytest.mark.parametrize("factor", [0.9, 1.3])
+def test_serve_failure_isolated(gk, ctx, factor):
+    smalls, ks = mixed_set(gk, 6, 14, lo=2000, hi=8000, ks=(8, 16))
+    big = gk.delaunay(400000, 15)
+    limit = int(factor * 4 * (2 * big.n + 1 + 2 * big.nnz))
+    print(f"pool limit {limit} bytes", flush=True)
+    with gk.Server(pool_limit=limit) as srv:
+        ids = [srv.submit(g, k) for g, k in zip(smalls, ks)]
+        bid = srv.submit(big, 64)
+        res = [srv.wait(r) for r in ids]
+        with pytest.raises(RuntimeError, match="pool limit"):
+            srv.wait(bid)
+        s = srv.stats()
+    assert s.completed == len(smalls) and s.failed == 1
+    assert s.isolated >= 2
+    for (part, st), g, k in zip(res, smalls, ks):
+        check_alone(ctx, g, k, part, st)
+
+
+def test_serve_over_budget_runs_alone(gk, ctx):
+    # Every request is bigger than the budget: each is admitted only onto an
+    # empty device, so the admitted footprint never exceeds the largest one.
+    graphs, ks = mixed_set(gk, 4, 16, lo=20000, hi=40000, ks=(16,))
+    feet = [gk.lib().gk_serve_footprint(g.n, g.nnz, gk.params(k)) for g, k in zip(graphs, ks)]
+    with gk.Server(hbm_budget=min(feet) // 2) as srv:
+        res = [srv.wait(r) for r in [srv.submit(g, k) for g, k in zip(graphs, ks)]]
+        s = srv.stats()
+    assert s.completed == len(graphs) and s.failed == 0
+    assert s.footprint_peak == max(feet)
+    for (part, st), g, k in zip(res, graphs, ks):
+        check_alone(ctx, g, k, part, st)
+
+
+def test_serve_one_taker_per_request(gk):
+    # Two callers wait on one request: one takes the result, the other is
+    # refused (GK_ERR_ARG), and the result cannot be taken again.
+    import ctypes
+    import threading
+    g = gk.delaunay(30000, 5)
+    L = gk.lib()
+    rcs = {}
+    with gk.Server() as srv:
+        rid = srv.submit(g, 8)
+
+        def waiter(i):
+            part = np.zeros(g.n, np.int32)
+            st = gk.Stats()
+            rcs[i] = L.gk_serve_wait(srv._h, rid, part.ctypes.data_as(gk.I32P), ctypes.byref(st))
+
+        ts = [threading.Thread(target=waiter, args=(i,)) for i in range(2)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert sorted(rcs.values()) == [-1, 0]
+        assert L.gk_serve_poll(srv._h, rid, None, None) == -1
