@@ -8,8 +8,9 @@
  * under the rule decisions that make it deterministic; each function cites
  * the reference code it restates.  Vertex ids are 0-based.  The initial
  * partition of the coarsest graph is an input callback: the reference calls
- * METIS there (gkway/metis_partition.hpp:46-47); tests pass the product's
- * gk_initpart, which plays METIS's role.                                    */
+ * METIS there (gkway/metis_partition.hpp:46-47); tests pass oracle_initpart,
+ * the restatement of the build's own partitioner for that step (initpart.cpp),
+ * so no product code takes part in an oracle run.                           */
 #ifndef GK_ORACLE_H
 #define GK_ORACLE_H
 #include <stdint.h>
@@ -46,6 +47,10 @@ int64_t oracle_apply(const gk_csr* g, const int32_t* moves, int32_t pos, const i
                      const int32_t* dest, int32_t* part, int32_t* updated);
 void oracle_cut(const gk_csr* g, int32_t k, const int32_t* part, int64_t* cut, int64_t* pw);
 int32_t oracle_coarsen_floor(int32_t n0, int32_t k, int32_t policy, int32_t override_to);
+/* Restatement of gk_initpart's rules (initpart.cpp); same signature as the
+ * callback above.  nthreads is ignored (single-threaded). */
+int oracle_initpart(const gk_csr* g, int32_t k, int64_t cap, int32_t seed, int32_t nthreads,
+                    int32_t* part, int64_t* cut);
 int64_t oracle_part_cap(int64_t total_vwgt, int32_t k, int32_t ubfactor);
 
 #ifdef __cplusplus

@@ -1,9 +1,10 @@
 """ctypes bindings of the CPU oracle (oracle/liboracle.so) -- test code only.
 
 The oracle is the checker: tests compare the HIP path with it.  The initial
-partition of the coarsest graph is passed to the oracle as a callback; tests
-hand it the product's host gk_initpart (the component that plays METIS's role
-in the reference), so both paths start refinement from the same partition.
+partition of the coarsest graph is passed to the oracle as a callback (the
+reference calls METIS there); the oracle's own restatement of the build's
+initial partitioner, oracle_initpart, is the default, so an oracle run uses
+no product code.  The product's gk_initpart is compared with it directly.
 """
 from __future__ import annotations
 
@@ -42,6 +43,7 @@ def lib():
         O.oracle_apply.restype = c_int64
         O.oracle_apply.argtypes = [POINTER(gk.Csr), I32P, c_int32, I32P, I32P, I32P, I32P]
         O.oracle_cut.argtypes = [POINTER(gk.Csr), c_int32, I32P, POINTER(c_int64), POINTER(c_int64)]
+        O.oracle_initpart.argtypes = [POINTER(gk.Csr), c_int32, c_int64, c_int32, c_int32, I32P, POINTER(c_int64)]
         _O = O
     return _O
 
@@ -50,16 +52,29 @@ def _p(a):
     return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(I32P)
 
 
-def initpart_callback():
-    """The product's host initial partitioner as an oracle callback."""
-    return ctypes.cast(gk.lib().gk_initpart, INITPART_FN)
+def initpart_callback(product=False):
+    """The oracle's restatement of the initial partitioner as an oracle
+    callback (product=True: the product's gk_initpart instead)."""
+    if product:
+        return ctypes.cast(gk.lib().gk_initpart, INITPART_FN)
+    return ctypes.cast(lib().oracle_initpart, INITPART_FN)
 
 
-def partition(g: "gk.Graph", k, **kw):
+def initpart(g, k, cap, seed=1):
+    """oracle_initpart on a host graph: (part, cut)."""
+    part = np.zeros(max(g.n, 1), np.int32)
+    cut = c_int64(0)
+    rc = lib().oracle_initpart(ctypes.byref(g.csr()), k, cap, seed, 1, part.ctypes.data_as(I32P), ctypes.byref(cut))
+    if rc != 0:
+        raise RuntimeError(f"oracle_initpart failed: {rc}")
+    return part[:g.n], cut.value
+
+
+def partition(g: "gk.Graph", k, product_initpart=False, **kw):
     p = gk.params(k, **kw)
     part = np.zeros(g.n, np.int32)
     st = gk.Stats()
-    cb = initpart_callback()
+    cb = initpart_callback(product_initpart)
     rc = lib().oracle_partition(ctypes.byref(g.csr()), ctypes.byref(p), cb, part.ctypes.data_as(I32P),
                                 ctypes.byref(st))
     if rc != 0:

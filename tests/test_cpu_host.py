@@ -303,6 +303,30 @@ def test_initpart_k1_and_tiny(gk):
     assert sorted(p) == [0, 1] and c == 1
 
 
+def _initpart_corpus(gk):
+    """Graphs of every family and size class the V-cycle hands to the initial
+    partitioner: tiny, mesh-like, weighted DAGs, road chains, the sample graph."""
+    out = [gk.read_metis(os.path.join(GOLDEN, "delaunay_n11.graph"))]
+    for n in (3, 40, 700, 5000):
+        out += [gk.delaunay(n, n + 1), gk.dag(n, max(2, int(n ** 0.5)), n + 2), gk.road(n, n + 3)]
+    return out
+
+
+@pytest.mark.parametrize("threads", [1, 8])
+def test_initpart_equals_oracle_restatement(gk, oracle, threads):
+    # The product's initial partitioner (indexed heaps, fork-join pool,
+    # parallel contraction) against the oracle's single-threaded restatement
+    # of its rules: identical partitions and cuts for every graph, k and seed.
+    for g in _initpart_corpus(gk):
+        for k in (2, 3, 8, 13, 64):
+            for seed in (1, 7):
+                cap = gk.lib().gk_part_cap(int(g.vwgt.sum()), k, 30)
+                p, c = gk.initpart(g, k, cap, seed=seed, threads=threads)
+                op, oc = oracle.initpart(g, k, cap, seed)
+                assert np.array_equal(p, op), (g.n, k, seed)
+                assert c == oc
+
+
 # Recorded METIS 5.2.1 cuts on the sample graph (see test_cpu_oracle.py).
 @pytest.mark.parametrize("k,metis", [(2, 91), (4, 205), (8, 349), (64, 2083)])
 def test_initpart_quality_on_sample_graph(gk, k, metis):

@@ -310,3 +310,110 @@ def test_quality_vs_metis_on_sample_graph(gk, oracle, k):
     part, st = oracle.partition(g, k)
     assert st.cut <= 1.05 * METIS_N11[k]
     assert st.imbalance <= 1.03 + 1e-12
+
+
+# ------------------------------------------------------------------ initial partitioner
+def graph_of_edges(gk, n, edges):
+    """Unit-weight symmetric graph from an undirected edge list."""
+    rows = [[] for _ in range(n)]
+    for u, v in edges:
+        rows[u].append(v)
+        rows[v].append(u)
+    xadj = np.concatenate([[0], np.cumsum([len(r) for r in rows])])
+    adj = np.array([u for r in rows for u in sorted(r)], np.int32)
+    return gk.Graph(xadj, adj)
+
+
+def path(n):
+    return [(i, i + 1) for i in range(n - 1)]
+
+
+def ring(n):
+    return path(n) + [(n - 1, 0)]
+
+
+def grid(r, c):
+    e = [(i * c + j, i * c + j + 1) for i in range(r) for j in range(c - 1)]
+    return e + [(i * c + j, (i + 1) * c + j) for i in range(r - 1) for j in range(c)]
+
+
+def cliques(count, size, linked):
+    """`count` cliques of `size` vertices; linked: consecutive cliques joined
+    by one edge in a ring (or a single edge when count == 2)."""
+    e = [(b * size + i, b * size + j) for b in range(count) for i in range(size) for j in range(i + 1, size)]
+    if linked:
+        e += [(b * size, ((b + 1) % count) * size + 1) for b in range(count if count > 2 else 1)]
+    return e
+
+
+# Minimum k-way cuts fixed by the structure (unit weights, parts of n/k
+# vertices): a path splits into k runs (k - 1 edges), a ring into k arcs
+# (k edges), an r x c grid with r <= c across its short side (r edges per
+# cut line; 2 x 2 blocks of a square grid: 2r), linked cliques at their
+# links, disjoint cliques at nothing.
+CLOSED_FORMS = [
+    ("path64", 64, path(64), 2, 1), ("path64", 64, path(64), 4, 3), ("path64", 64, path(64), 8, 7),
+    ("ring64", 64, ring(64), 2, 2), ("ring64", 64, ring(64), 4, 4),
+    ("grid4x16", 64, grid(4, 16), 2, 4), ("grid4x16", 64, grid(4, 16), 4, 12),
+    ("grid8x8", 64, grid(8, 8), 2, 8), ("grid8x8", 64, grid(8, 8), 4, 16),
+    ("grid16x32", 512, grid(16, 32), 2, 16),
+    ("two_k10", 20, cliques(2, 10, True), 2, 1),
+    ("ring_of_four_k8", 32, cliques(4, 8, True), 4, 4),
+    ("four_k6", 24, cliques(4, 6, False), 4, 0),
+]
+
+
+@pytest.mark.parametrize("name,n,edges,k,best", CLOSED_FORMS, ids=[f"{c[0]}-k{c[3]}" for c in CLOSED_FORMS])
+def test_initpart_closed_forms(gk, oracle, name, n, edges, k, best):
+    g = graph_of_edges(gk, n, edges)
+    cap = oracle.lib().oracle_part_cap(n, k, 30)
+    part, cut = oracle.initpart(g, k, cap)
+    assert cut == best == cut_of(dense(g), part)
+    assert np.bincount(part, minlength=k).max() <= cap
+
+
+def exact_bisection_optimum(g):
+    """Minimum cut over all bisections into two halves of n/2 vertices."""
+    n = g.n
+    masks = np.arange(1 << n)
+    bits = (masks[:, None] >> np.arange(n)) & 1
+    cut = np.zeros(len(masks), np.int64)
+    for v in range(n):
+        for j in range(g.xadj[v], g.xadj[v + 1]):
+            u = g.adjncy[j]
+            if v < u:
+                cut += g.adjwgt[j] * (bits[:, v] != bits[:, u])
+    ok = bits.sum(1) == n // 2
+    return int(cut[ok].min()), float(cut[ok].mean())
+
+
+def test_initpart_bisection_near_brute_force_optimum(gk, oracle):
+    # Unit vertex weights and even n: the cap allows exactly n/2 per side, so
+    # every feasible bisection is enumerated.  The partitioner is a heuristic;
+    # on these 60 random graphs it finds the optimum in most and stays close
+    # in all, far below an average balanced split.
+    hits, tot_cut, tot_opt = 0, 0, 0
+    for n in (12, 14):
+        for seed in range(30):
+            g, _, _ = rand_graph(gk, n, 0.3, seed * 7 + n, vwmax=1)
+            cap = oracle.lib().oracle_part_cap(n, 2, 30)
+            part, cut = oracle.initpart(g, 2, cap)
+            opt, mean = exact_bisection_optimum(g)
+            assert np.bincount(part, minlength=2).tolist() == [n // 2, n // 2]
+            assert cut == cut_of(dense(g), part)
+            assert opt <= cut <= (opt + mean) / 2, (n, seed, cut, opt, mean)
+            hits += cut == opt
+            tot_cut += cut
+            tot_opt += opt
+    assert hits >= 45
+    assert tot_cut <= 1.06 * tot_opt
+
+
+@pytest.mark.parametrize("k", sorted(METIS_N11))
+def test_initpart_quality_vs_metis_on_sample_graph(gk, oracle, k):
+    # The whole sample graph partitioned by the initial partitioner alone.
+    g = gk.read_metis(os.path.join(GOLDEN, "delaunay_n11.graph"))
+    cap = oracle.lib().oracle_part_cap(g.n, k, 30)
+    part, cut = oracle.initpart(g, k, cap)
+    assert cut <= 1.05 * METIS_N11[k]
+    assert np.bincount(part, minlength=k).max() <= cap
