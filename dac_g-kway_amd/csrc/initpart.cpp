@@ -70,66 +70,107 @@ struct G {
 };
 
 // ------------------------------------------------------------------ heap
-// Indexed binary max-heap over vertices keyed by (gain, -vertex).
+// Indexed binary max-heap of vertices ordered by (gain desc, vertex asc).
+// Both go into one 64-bit key: the gain in the high 32 bits (a gain is
+// bounded by a vertex's weighted degree, below 2^31) and the complemented
+// vertex id in the low 32, so one integer compare orders two entries.  Keys
+// are distinct, so the order in which vertices leave the heap depends on the
+// keys alone, never on the heap's layout: building it in O(n) by sift-down
+// gives the same sequence as n pushes.
 class Heap {
  public:
+  static int64_t pack(int64_t g, int32_t v) {
+    return static_cast<int64_t>(static_cast<uint64_t>(g) << 32) | int64_t(0xFFFFFFFFu - static_cast<uint32_t>(v));
+  }
+  static int32_t vertex(int64_t key) { return static_cast<int32_t>(0xFFFFFFFFu - static_cast<uint32_t>(key)); }
+  // Empties the heap over vertices [0, n); only the positions of vertices
+  // still queued are cleared when the size is unchanged.
   void reset(int32_t n) {
-    pos_.assign(n, -1);
+    if (static_cast<int32_t>(pos_.size()) != n) {
+      pos_.assign(n, -1);
+    } else {
+      for (int64_t k : h_) pos_[vertex(k)] = -1;
+    }
     h_.clear();
-    h_.reserve(n);
+  }
+  // Replaces the contents (heap empty before) by `keys`, heapified in O(n).
+  void build(const std::vector<int64_t>& keys) {
+    h_.assign(keys.begin(), keys.end());
+    const int32_t n = static_cast<int32_t>(h_.size());
+    for (int32_t i = n / 2 - 1; i >= 0; --i) sift_down(i);
+    for (int32_t i = 0; i < n; ++i) pos_[vertex(h_[i])] = i;
   }
   bool empty() const { return h_.empty(); }
   bool contains(int32_t v) const { return pos_[v] >= 0; }
-  int32_t top() const { return h_[0].v; }
-  int64_t top_key() const { return h_[0].g; }
+  int32_t top() const { return vertex(h_[0]); }
+  int64_t top_key() const { return h_[0] >> 32; }  // the gain of the top vertex
   void push(int32_t v, int64_t g) {
-    pos_[v] = static_cast<int32_t>(h_.size());
-    h_.push_back({g, v});
-    up(pos_[v]);
+    const int32_t i = static_cast<int32_t>(h_.size());
+    h_.push_back(pack(g, v));
+    pos_[v] = i;
+    up(i);
   }
   void update(int32_t v, int64_t g) {
-    int32_t i = pos_[v];
-    int64_t old = h_[i].g;
-    h_[i].g = g;
-    if (g > old) up(i); else down(i);
+    const int32_t i = pos_[v];
+    const int64_t old = h_[i];
+    h_[i] = pack(g, v);
+    if (h_[i] > old) up(i); else down(i);
   }
   void remove(int32_t v) {
-    int32_t i = pos_[v];
+    const int32_t i = pos_[v];
     pos_[v] = -1;
-    if (i == static_cast<int32_t>(h_.size()) - 1) { h_.pop_back(); return; }
-    h_[i] = h_.back();
+    const int64_t last = h_.back();
     h_.pop_back();
-    pos_[h_[i].v] = i;
+    if (i == static_cast<int32_t>(h_.size())) return;
+    h_[i] = last;
+    pos_[vertex(last)] = i;
     up(i);
-    down(pos_[h_[i].v]);
+    down(pos_[vertex(last)]);
   }
+
  private:
-  struct E { int64_t g; int32_t v; };
-  static bool better(const E& a, const E& b) { return a.g != b.g ? a.g > b.g : a.v < b.v; }
   void up(int32_t i) {
+    const int64_t k = h_[i];
     while (i > 0) {
-      int32_t p = (i - 1) / 2;
-      if (!better(h_[i], h_[p])) break;
-      std::swap(h_[i], h_[p]);
-      pos_[h_[i].v] = i;
-      pos_[h_[p].v] = p;
+      const int32_t p = (i - 1) / 2;
+      if (h_[p] >= k) break;
+      h_[i] = h_[p];
+      pos_[vertex(h_[i])] = i;
       i = p;
     }
+    h_[i] = k;
+    pos_[vertex(k)] = i;
   }
   void down(int32_t i) {
-    int32_t n = static_cast<int32_t>(h_.size());
+    const int32_t n = static_cast<int32_t>(h_.size());
+    const int64_t k = h_[i];
     while (true) {
-      int32_t l = 2 * i + 1, r = l + 1, b = i;
-      if (l < n && better(h_[l], h_[b])) b = l;
-      if (r < n && better(h_[r], h_[b])) b = r;
-      if (b == i) break;
-      std::swap(h_[i], h_[b]);
-      pos_[h_[i].v] = i;
-      pos_[h_[b].v] = b;
-      i = b;
+      int32_t c = 2 * i + 1;
+      if (c >= n) break;
+      if (c + 1 < n && h_[c + 1] > h_[c]) ++c;
+      if (h_[c] <= k) break;
+      h_[i] = h_[c];
+      pos_[vertex(h_[i])] = i;
+      i = c;
     }
+    h_[i] = k;
+    pos_[vertex(k)] = i;
   }
-  std::vector<E> h_;
+  // down() without position updates, for build().
+  void sift_down(int32_t i) {
+    const int32_t n = static_cast<int32_t>(h_.size());
+    const int64_t k = h_[i];
+    while (true) {
+      int32_t c = 2 * i + 1;
+      if (c >= n) break;
+      if (c + 1 < n && h_[c + 1] > h_[c]) ++c;
+      if (h_[c] <= k) break;
+      h_[i] = h_[c];
+      i = c;
+    }
+    h_[i] = k;
+  }
+  std::vector<int64_t> h_;
   std::vector<int32_t> pos_;
 };
 
@@ -385,27 +426,48 @@ int64_t cut2(const G& g, const std::vector<uint8_t>& side) {
 // Boundary FM refinement.  Moves that keep both sides within their maxima
 // (or reduce the overload) are taken in max-gain order; the best prefix of a
 // pass (lowest (overload, cut)) is kept.
+// Buffers of fm2, one set per thread (fm2 never waits on the pool, so no
+// other fm2 call can run on its thread while it holds them).
+struct FmWork {
+  std::vector<int64_t> id, ed, keys[2];
+  std::vector<uint8_t> locked;
+  std::vector<int32_t> moved;
+  Heap q[2];
+};
+
 void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int64_t& cut) {
   const int32_t n = g.n;
-  std::vector<int64_t> id(n), ed(n);
+  thread_local FmWork W;
+  std::vector<int64_t>& id = W.id;
+  std::vector<int64_t>& ed = W.ed;
+  id.assign(n, 0);
+  ed.assign(n, 0);
   int64_t pw[2] = {0, 0};
   for (int32_t v = 0; v < n; ++v) pw[side[v]] += g.vw[v];
-  Heap q[2];
-  std::vector<uint8_t> locked(n);
-  std::vector<int32_t> moved;
+  Heap* q = W.q;
+  std::vector<uint8_t>& locked = W.locked;
+  std::vector<int32_t>& moved = W.moved;
+  locked.resize(n);
   const int32_t limit = std::max<int32_t>(25, std::min<int32_t>(n / 20, 300));
   // Internal / external degrees: computed once, then kept exact through the
   // moves of a pass and their rollback.
   for (int32_t v = 0; v < n; ++v) {
-    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
-      (side[g.adj[j]] == side[v] ? id[v] : ed[v]) += g.w[j];
+    const uint8_t sv = side[v];
+    int64_t a = 0, b = 0;
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) (side[g.adj[j]] == sv ? a : b) += g.w[j];
+    id[v] = a;
+    ed[v] = b;
   }
   for (int pass = 0; pass < passes; ++pass) {
     q[0].reset(n);
     q[1].reset(n);
     std::fill(locked.begin(), locked.end(), 0);
+    W.keys[0].clear();
+    W.keys[1].clear();
     for (int32_t v = 0; v < n; ++v)
-      if (ed[v] > 0) q[side[v]].push(v, ed[v] - id[v]);
+      if (ed[v] > 0) W.keys[side[v]].push_back(Heap::pack(ed[v] - id[v], v));
+    q[0].build(W.keys[0]);
+    q[1].build(W.keys[1]);
     moved.clear();
     int64_t cur_cut = cut, best_cut = cut;
     int64_t best_ol = overload(pw, bal);
