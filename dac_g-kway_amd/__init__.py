@@ -188,6 +188,7 @@ def lib():
         L.gk_k_apply.argtypes = [VP, POINTER(Csr), I32P, c_int32, I32P, I32P, I32P, I32P,
                                  POINTER(c_int64), VP]
         L.gk_k_cut.argtypes = [VP, POINTER(Csr), c_int32, I32P, POINTER(c_int64), POINTER(c_int64), VP]
+        L.gk_k_cut_noalloc.argtypes = [VP, POINTER(Csr), c_int32, I32P, VP, VP, VP]
         L.gk_k_round.argtypes = [VP, POINTER(Csr), c_int32, c_int64, c_int32, c_int32, I32P, I32P, I32P, I32P,
                                  I32P, I32P, I32P, POINTER(c_int64), I32P, POINTER(c_int64), VP]
         L.gk_graph_read.argtypes = [c_char_p, POINTER(Csr)]

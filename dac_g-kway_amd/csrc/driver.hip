@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -647,6 +648,44 @@ struct DeviceGuard {
   explicit DeviceGuard(int d) { (void)hipSetDevice(d); }
 };
 
+// ---------------------------------------------------------------- no-alloc cut
+// gk_k_cut_noalloc keeps the cut kernel's per-block slots (one 64-bit cut
+// share per block, then k 64-bit part weights per block) in `scratch` instead
+// of pool memory: 2^18 slots = 2 MiB on every device the library runs on,
+// held until the process ends.  Every slot a launch reads was written by the
+// same launch, so nothing needs clearing between calls.
+constexpr int64_t kCutScratchSlots = int64_t(1) << 18;
+static_assert(kCutScratchSlots >= 2 * (int64_t(kMaxK) + 1), "two blocks of slots at the largest k");
+static __device__ __align__(256) unsigned long long scratch[kCutScratchSlots];
+
+// On the host, `scratch` names a host-side shadow, not device memory.  Its
+// device address on each device is looked up once, with that device current
+// (gk_ctx_create), and only that address is ever handed to kernels.
+constexpr int kScratchDevices = 256;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_addr[kScratchDevices] = {};
+
+int scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices) return GK_OK;  // gk_k_cut_noalloc rejects such devices
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_addr[device]) return GK_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    gkh::set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return GK_ERR_NODEVICE;
+  }
+  g_scratch_addr[device] = static_cast<unsigned long long*>(p);
+  return GK_OK;
+}
+
+unsigned long long* scratch_addr(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_addr[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -662,6 +701,7 @@ int gk_ctx_create(int device, gk_ctx** out) {
   }
   if (device < 0 || device >= ndev) { gkh::set_error("device out of range"); return GK_ERR_ARG; }
   GK_HIP(hipSetDevice(device));
+  GK_TRY(scratch_setup(device));
   gk_ctx* c = new gk_ctx;
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1128,6 +1168,28 @@ int gk_k_cut(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, int6
     for (int32_t q = 0; q < k; ++q) pw[q] = static_cast<int64_t>(hp[q]);
   }
   return GK_OK;
+}
+
+int gk_k_cut_noalloc(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, int64_t* dev_cut,
+                     int64_t* dev_pw, void* stream) {
+  if (!ctx || !dg || !dg->adjwgt || !dg->vwgt || !dev_cut || !dev_pw || k < 1 || dg->n < 0 ||
+      (dg->n > 0 && (!part || !dg->xadj || !dg->adjncy))) {
+    gkh::set_error("bad argument");
+    return GK_ERR_ARG;
+  }
+  if (k > kMaxK) {
+    gkh::set_error("k = " + std::to_string(k) + " exceeds " + std::to_string(kMaxK) +
+                   ", the largest k of the no-allocation cut (its scratch holds 2^18 slots)");
+    return GK_ERR_LIMIT;
+  }
+  unsigned long long* scr = scratch_addr(ctx->device);
+  if (!scr) { gkh::set_error("no scratch address for this device"); return GK_ERR_INTERNAL; }
+  DeviceGuard guard(ctx->device);
+  hipStream_t st = pick(ctx, stream);
+  const int rc = gkd::launch_cut(dview(dg), k, part, reinterpret_cast<unsigned long long*>(dev_cut),
+                                 reinterpret_cast<unsigned long long*>(dev_pw), scr, st, kCutScratchSlots);
+  if (rc != GK_OK) gkh::set_error("cut launch failed");
+  return rc;
 }
 
 }  // extern "C"

@@ -112,10 +112,12 @@ int launch_apply(const DGraph& g, int32_t M, const int32_t* moves, const int32_t
                  hipStream_t st);
 int launch_boundary(const DGraph& g, const int32_t* part, int32_t* bnd, hipStream_t st);
 // cut and part weights pw64[k] (both overwritten); scratch holds
-// cut_scratch(n, k) values.
+// cut_scratch(n, k) values, or fewer when max_slots caps it (fewer blocks;
+// the sums do not depend on the block count).
 int64_t cut_scratch(int64_t n, int32_t k);
 int launch_cut(const DGraph& g, int32_t k, const int32_t* part, unsigned long long* cut,
-               unsigned long long* pw64, unsigned long long* scratch, hipStream_t st);
+               unsigned long long* pw64, unsigned long long* scratch, hipStream_t st,
+               int64_t max_slots = INT64_MAX);
 
 int launch_sum(const int32_t* a, int64_t n, unsigned long long* out, hipStream_t st);
 int launch_cvwgt(int32_t nc, const int32_t* cptr, const int32_t* order, const int32_t* vw,

@@ -520,13 +520,17 @@ __global__ void __launch_bounds__(256) k_cut_sum(int32_t nb, int32_t k, const un
 }
 
 int launch_cut(const DGraph& g, int32_t k, const int32_t* part, unsigned long long* cut,
-               unsigned long long* pw64, unsigned long long* scratch, hipStream_t st) {
+               unsigned long long* pw64, unsigned long long* scratch, hipStream_t st,
+               int64_t max_slots) {
   if (g.n <= 0) {
     fill<unsigned long long>(cut, 0ull, 1, st);
     fill<unsigned long long>(pw64, 0ull, k, st);
     return hip_status();
   }
-  const unsigned grid = cut_grid(g.n, k);
+  unsigned grid = cut_grid(g.n, k);
+  const int64_t fit = max_slots / (int64_t(k) + 1);  // blocks whose slots fit
+  if (fit < 1) return GK_ERR_LIMIT;
+  if (int64_t(grid) > fit) grid = static_cast<unsigned>(fit);
   unsigned long long* bcut = scratch;
   unsigned long long* bpw = scratch + grid;
   k_cut<<<grid, 256, sizeof(unsigned int) * k, st>>>(g.n, k, g.xadj, g.adj, g.w, g.vw, part, bcut, bpw);

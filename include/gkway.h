@@ -367,6 +367,19 @@ int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t ma
 /* Weighted cut over entries (v,u), v<u, part differs; part weights pw[k]. */
 int gk_k_cut(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part,
              int64_t* cut, int64_t* pw, void* stream);
+/* gk_k_cut without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same kernels and the
+ * same sums as gk_k_cut, but the outputs are DEVICE memory the caller owns
+ * (dev_cut: 1 int64, dev_pw: k int64, written in stream order; no host
+ * synchronisation), and the per-block slots live in a static 2 MiB device
+ * array (2^18 64-bit slots per device, held until the process ends) instead
+ * of the context's pool.  k above 8192 is rejected with GK_ERR_LIMIT.  Calls
+ * no hipMalloc / hipFree.  All calls on one device share that array: calls
+ * that overlap on one device (other streams or threads) race, so serialise
+ * them.  dg->adjwgt and dg->vwgt are required. */
+int gk_k_cut_noalloc(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part,
+                     int64_t* dev_cut, int64_t* dev_pw, void* stream);
 
 #ifdef __cplusplus
 }
