@@ -1,17 +1,25 @@
 """Benchmark: partitioned edges/s of the MI355X multilevel k-way partitioner.
 
-Default workload (the headline configuration): 2-D Delaunay triangulation of
-2^22 uniform random points (seed 1, ~12.6M undirected edges), unit weights,
-k = 64, imbalance 3%.  One step = one full V-cycle (coarsening, initial
-partition of the coarsest graph on the host, refinement of every level, final
-cut) from a device-resident CSR to a device-resident partition.
+Default run (the headline): the batched form on a packed batch of 4096 graphs
+(seed 4; vertex counts log-uniform in [500, 50000]; even graphs 2-D Delaunay
+meshes, odd graphs layered circuit-like DAGs with edge weights 1..100 and
+vertex weights 1..4; k drawn from {2,4,8,16,32}; imbalance 3%).  One step =
+one gk_partition_packed call over the whole pack, from a device-resident
+packed CSR to device-resident partitions.  Beside it, on the same graphs:
+  * the plain path, one gk_partition_device call per graph, the median of 3
+    timed calls per graph over all 4096 graphs (speedup_vs_plain);
+  * per-graph p50/p99 latency of both;
+  * the host-to-host window of the batched form (gk_partition_batch);
+  * bit-exactness: batched == plain on every graph, == the CPU oracle on a
+    bounded sample (which is also the CPU baseline, with the host named);
+  * nested, the plain path on the 2-D Delaunay triangulation of 2^22 points
+    (k = 64), with its host-to-host window and an oracle check.
+`value` is 0 unless every result is bit-exact, deterministic and within the
+3% cap.
 
-The result of every timed step must be bit-identical to the CPU oracle's
-partition of the same graph (checked once, the oracle also being the CPU
-baseline); otherwise `value` is reported as 0.
-
-  python bench.py [--steps K] [--warmup W] [--workload delaunay|dag|road]
-                  [--n N] [--k K] [--no-cpu] [--calibrate-copy]
+  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-c2]
+  python bench.py --form plain [--workload delaunay|dag|road] [--n N] [--k K]
+  python bench.py --form serve [--rate R] [--duration S] [--pool P]
 """
 import argparse
 import ctypes
@@ -118,9 +126,72 @@ def batch_set(gk, count, seed, lo=500, hi=50000, ks=(2, 4, 8, 16, 32)):
     return graphs, [int(k) for k in kk]
 
 
-def run_batch(args, gk, torch):
-    """Batched form: all graphs in one packed CSR, one gk_partition_packed
-    call per step (device CSR -> device partitions)."""
+def host_info():
+    """CPU model and usable cores of the host this runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu": model, "nproc": usable, "cpu_count": os.cpu_count()}
+
+
+def upload_graphs(gk, torch, graphs):
+    """Device copies of each graph (a standalone CSR per graph) and a
+    partition buffer per graph, for plain-path calls."""
+    I32P = gk.I32P
+    ptr = lambda t_: ctypes.cast(t_.data_ptr(), I32P)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).cuda()
+    out = []
+    for g in graphs:
+        keep = (dev(g.xadj), dev(g.adjncy if g.nnz else np.zeros(1, np.int32)),
+                dev(g.adjwgt if g.nnz else np.zeros(1, np.int32)), dev(g.vwgt),
+                torch.zeros(max(g.n, 1), dtype=torch.int32, device="cuda"))
+        c = gk.Csr()
+        c.n, c.nnz = g.n, g.nnz
+        c.xadj, c.adjncy, c.adjwgt, c.vwgt = (ptr(t_) for t_ in keep[:4])
+        out.append((c, keep))
+    torch.cuda.synchronize()
+    return out
+
+
+def plain_over_graphs(gk, torch, ctx, graphs, ks, reps):
+    """The plain path (one gk_partition_device call per graph, device CSR in,
+    device partition out) over every graph: one untimed pass, then `reps`
+    timed passes, graph by graph.  Returns the per-pass times [reps, G] and
+    each graph's partition."""
+    L = gk.lib()
+    dgs = upload_graphs(gk, torch, graphs)
+    I32P = gk.I32P
+    times = np.zeros((reps, len(graphs)))
+    for r in range(reps + 1):
+        for i, (c, keep) in enumerate(dgs):
+            p1 = gk.params(ks[i])
+            a = time.perf_counter()
+            rc = L.gk_partition_device(ctx.handle, ctypes.byref(c), ctypes.byref(p1),
+                                       ctypes.cast(keep[4].data_ptr(), I32P), None, None)
+            dt = time.perf_counter() - a
+            if rc != 0:
+                raise RuntimeError(f"plain path failed on graph {i}: {L.gk_last_error().decode()}")
+            if r > 0:
+                times[r - 1, i] = dt
+        log(f"plain pass {r}: {times[r - 1].sum() if r else 0.0:.2f} s")
+    parts = [keep[4].cpu().numpy()[:g.n] for (c, keep), g in zip(dgs, graphs)]
+    return times, parts
+
+
+def run_batch(args, gk, torch, nested_c2=True):
+    """Batched form (the headline): all graphs in one packed CSR, one
+    gk_partition_packed call per step (device CSR -> device partitions),
+    against the plain path on the same graphs."""
     t = time.time()
     graphs, ks = batch_set(gk, args.graphs, args.seed)
     m_tot = sum(g.m for g in graphs)
@@ -181,59 +252,69 @@ def run_batch(args, gk, torch):
         L.gk_ctx_set_stage_mask(ctx.handle, 0xFFFFFFFF)
         sruns = [step(prm_stage) for _ in range(args.stage_steps)]
         stage = stage_table(gk, [r[0] for r in sruns], args.stage_steps, log_it=True)
+    value = m_tot / (total / args.steps)
 
-    # The plain path on a sample of the same graphs: speed and bit-exactness.
-    idx = list(range(0, G, max(1, G // args.plain_sample)))[:args.plain_sample]
-    plain_t, plain_m, exact = 0.0, 0, True
-    for i in idx:
-        g = graphs[i]
-        gx, ga, gw, gv = dev(g.xadj), dev(g.adjncy), dev(g.adjwgt), dev(g.vwgt)
-        gp = torch.zeros(max(g.n, 1), dtype=torch.int32, device="cuda")
-        torch.cuda.synchronize()
-        c = gk.Csr()
-        c.n, c.nnz = g.n, g.nnz
-        c.xadj, c.adjncy, c.adjwgt, c.vwgt = ptr(gx), ptr(ga), ptr(gw), ptr(gv)
-        p1 = gk.params(ks[i])
-        best = None
-        for r in range(3):
-            a = time.perf_counter()
-            rc = L.gk_partition_device(ctx.handle, ctypes.byref(c), ctypes.byref(p1), ptr(gp), None, None)
-            dt = time.perf_counter() - a
-            if rc != 0:
-                raise RuntimeError("plain path failed")
-            if r > 0:
-                best = dt if best is None else min(best, dt)
-        plain_t += best
-        plain_m += g.m
-        exact &= bool(np.array_equal(gp.cpu().numpy()[:g.n], part[v_off[i]:v_off[i + 1]]))
-    plain_value = plain_m / plain_t
-    log(f"plain path on {len(idx)} sampled graphs: {plain_value / 1e6:.2f} M edges/s, bit-exact={exact}")
+    # Host-to-host window of the batched form (host graphs packed and copied
+    # in, partitions copied out): gk_partition_batch.
+    h2h = []
+    for _ in range(2):
+        a = time.perf_counter()
+        hparts, _ = ctx.partition_batch(graphs, ks)
+        h2h.append(time.perf_counter() - a)
+    h2h_exact = all(np.array_equal(hp, part[v_off[i]:v_off[i + 1]]) for i, hp in enumerate(hparts))
+    del hparts
 
-    cpu = None
+    # The plain path on every graph: median of `plain_reps` timed calls per
+    # graph; its partitions must equal the batched ones.  (plain_reps 0 skips
+    # it, for profiling runs of the batched form alone.)
+    plain = None
+    exact_plain = True
+    if args.plain_reps > 0:
+        ptimes, pparts = plain_over_graphs(gk, torch, ctx, graphs, ks, args.plain_reps)
+        med = np.median(ptimes, axis=0)
+        plain_value = m_tot / med.sum()
+        rep_values = [m_tot / ptimes[r].sum() for r in range(args.plain_reps)]
+        exact_plain = all(np.array_equal(pp, part[v_off[i]:v_off[i + 1]]) for i, pp in enumerate(pparts))
+        del pparts
+        log(f"plain path on all {G} graphs: {plain_value / 1e6:.2f} M edges/s (passes "
+            f"{min(rep_values) / 1e6:.2f}..{max(rep_values) / 1e6:.2f}), bit-exact vs batched={exact_plain}")
+        plain = {"edges_per_s": round(plain_value, 1),
+                 "per_graph": f"median of {args.plain_reps} timed gk_partition_device calls per graph, all {G} "
+                              f"graphs, after one untimed pass",
+                 "pass_edges_per_s_min": round(min(rep_values), 1),
+                 "pass_edges_per_s_max": round(max(rep_values), 1),
+                 "p50_ms": round(float(np.percentile(med * 1e3, 50)), 3),
+                 "p99_ms": round(float(np.percentile(med * 1e3, 99)), 3)}
+
+    # The oracle on a bounded sample: bit-exactness and the CPU baseline.
+    cpu, exact_oracle, n_oracle = None, None, 0
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle
-        tc, mc, ng = 0.0, 0, 0
-        for i in idx:
+        exact_oracle = True
+        tc, mc = 0.0, 0
+        sample = sorted(np.random.default_rng(args.seed + 100).permutation(G)[:args.plain_sample].tolist())
+        for i in sample:
             a = time.perf_counter()
             opart, ost = _oracle.partition(graphs[i], ks[i], initpart_threads=1)
             tc += time.perf_counter() - a
             mc += graphs[i].m
-            ng += 1
-            exact &= bool(np.array_equal(opart, part[v_off[i]:v_off[i + 1]]))
+            n_oracle += 1
+            exact_oracle &= bool(np.array_equal(opart, part[v_off[i]:v_off[i + 1]]))
             if tc > args.cpu_budget:
                 break
         cpu = {"value": round(mc / tc, 1), "unit": "edges/s", "cores": 1, "kind": "port",
-               "sample": f"{ng} of the {G} graphs (every {max(1, G // args.plain_sample)}th) partitioned one "
-                         f"by one by the single-threaded oracle ({tc:.1f} s)"}
-        log(f"oracle: {ng} graphs in {tc:.1f}s, bit_exact={exact}")
+               "sample": f"{n_oracle} of the {G} graphs (seeded random sample) partitioned one by one "
+                         f"by the single-threaded oracle ({tc:.1f} s, {mc} edges)",
+               "host": host_info()}
+        log(f"oracle: {n_oracle} graphs in {tc:.1f}s, bit_exact={exact_oracle}")
 
     if args.calibrate_copy:
         calibrate_copy()
-    value = m_tot / (total / args.steps)
+    exact = exact_plain and h2h_exact and exact_oracle is not False
     ok = deterministic and exact and within_cap
     out = {
-        "metric": "partitioned edges/s at bit-exact cut, eps=3% (batched form, all graphs in one packed call)",
+        "metric": "partitioned edges/s at bit-exact cut, eps=3% (batched form vs plain GPU path; p50/p99)",
         "value": round(value, 1) if ok else 0.0,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -242,29 +323,36 @@ def run_batch(args, gk, torch):
         "ms_per_step": round(total / args.steps * 1e3, 3),
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "latency": "per-graph completion time inside a batch step",
+        "latency": "per-graph completion time inside a batch step (all graphs of all timed steps)",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (seeded generators)",
         "config": {"workload": f"packed batch of {G} graphs (seed {args.seed}): n log-uniform in [500, 50000], "
-                               f"even = Delaunay, odd = layered DAG; k in {{2,4,8,16,32}}",
+                               f"even = 2-D Delaunay, odd = layered DAG (w 1..100, vw 1..4); k in {{2,4,8,16,32}}",
                    "n_total": int(pg.n), "m_total": int(m_tot), "eps": 0.03,
                    "window": "device packed CSR -> device partitions",
                    "rounds": "hipGraph replay" if args.graph_rounds else "stream launches",
                    "autotune": bool(args.autotune)},
-        "plain_path_edges_per_s": round(plain_value, 1),
-        "speedup_vs_plain": round(value / plain_value, 2),
+        "plain": plain,
+        "speedup_vs_plain": round(value / plain["edges_per_s"], 2) if plain else None,
+        "host_to_host_edges_per_s": round(m_tot / min(h2h), 1),
         "worst_imbalance": round(worst_imb, 5),
         "all_parts_within_cap": within_cap,
-        "bit_exact_vs_plain_and_oracle": exact,
+        "bit_exact": {"batched_vs_plain_all_graphs": exact_plain, "host_to_host_vs_device": h2h_exact,
+                      "vs_oracle_sample": exact_oracle, "oracle_sample_graphs": n_oracle},
         "deterministic": deterministic,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if nested_c2 and not args.no_c2:
+        c2 = argparse.Namespace(**vars(args))
+        c2.workload, c2.n, c2.k, c2.seed = "delaunay", 1 << 22, 64, 1
+        c2.steps, c2.warmup, c2.stage_steps = args.c2_steps, 2, 1
+        out["plain_config2"] = run_plain(c2, gk, torch, emit=False)
+    print(json.dumps(out), flush=True)
 
 
 def calibrate_copy():
@@ -396,56 +484,8 @@ def run_serve(args, gk, torch):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", default="plain", choices=["plain", "batch", "serve"])
-    ap.add_argument("--workload", default="delaunay")
-    ap.add_argument("--n", type=int, default=1 << 22)
-    ap.add_argument("--k", type=int, default=64)
-    ap.add_argument("--seed", type=int, default=None)
-    ap.add_argument("--graphs", type=int, default=4096, help="batch form: graphs in the pack")
-    ap.add_argument("--plain-sample", type=int, default=64, help="batch form: graphs timed on the plain path")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="batch/serve forms: seconds of oracle work")
-    ap.add_argument("--rate", type=float, default=200.0, help="serve form: Poisson arrival rate (requests/s)")
-    ap.add_argument("--duration", type=float, default=20.0, help="serve form: arrival window (s)")
-    ap.add_argument("--pool", type=int, default=48, help="serve form: distinct graphs requests draw from")
-    ap.add_argument("--check", type=int, default=24, help="serve form: requests re-run alone for bit-exactness")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline + bit-exact check)")
-    ap.add_argument("--no-stage-time", action="store_true")
-    ap.add_argument("--graph-rounds", action="store_true",
-                    help="replay each level's refinement rounds from a captured hipGraph")
-    ap.add_argument("--autotune", action="store_true",
-                    help="time candidate launch shapes of matching and contraction on first use")
-    ap.add_argument("--stage-steps", type=int, default=3,
-                    help="extra steps with per-stage HIP events for the stage table and roofline")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a plain device copy of 1 GiB (PMC calibration)")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(graph-level sharding across GPUs is described in DESIGN.md)"}))
-        sys.exit(2)
-
-    gk = load_pkg()
-    import torch
-
-    if args.form == "batch":
-        if args.seed is None:
-            args.seed = 4
-        run_batch(args, gk, torch)
-        return
-    if args.form == "serve":
-        if args.seed is None:
-            args.seed = 5
-        run_serve(args, gk, torch)
-        return
-    if args.seed is None:
-        args.seed = 1
-
+def run_plain(args, gk, torch, emit=True):
+    """Plain path, one graph per call (device CSR -> device partition)."""
     t = time.time()
     g = make_graph(gk, args.workload, args.n, args.seed)
     log(f"generated {args.workload} n={g.n} m={g.m} in {time.time() - t:.1f}s")
@@ -509,6 +549,15 @@ def main():
         sstats = [step(p_stage) for _ in range(args.stage_steps)]
         stage = stage_table(gk, sstats, args.stage_steps, log_it=True)
 
+    # Host-to-host window (host CSR in, host partition out, PCIe included):
+    # gk_partition, the window to set against a host partitioner.
+    h2h = []
+    for _ in range(3):
+        a = time.perf_counter()
+        hpart, _ = ctx.partition(g, args.k)
+        h2h.append(time.perf_counter() - a)
+    h2h_exact = bool(np.array_equal(hpart, part))
+
     m = g.m
     ms_step = total / args.steps * 1e3
     value = m / (total / args.steps)
@@ -527,13 +576,13 @@ def main():
         cpu = {"value": round(m / tcpu, 1), "unit": "edges/s", "cores": 1, "kind": "port",
                "sample": f"one full partition of the same graph by the single-threaded oracle "
                          f"({tcpu:.1f} s, initial partitioner on 1 thread)",
-               "cut": int(ost.cut)}
+               "cut": int(ost.cut), "host": host_info()}
         log(f"oracle: {tcpu:.1f}s cut={ost.cut} bit_exact={bit_exact}")
 
     if args.calibrate_copy:
         calibrate_copy()
 
-    ok = deterministic and (bit_exact is not False) and st.max_part_wgt <= st.cap
+    ok = deterministic and (bit_exact is not False) and h2h_exact and st.max_part_wgt <= st.cap
     out = {
         "metric": "partitioned edges/s at bit-exact cut, eps=3% (plain path, one graph per call)",
         "value": round(value, 1) if ok else 0.0,
@@ -553,6 +602,8 @@ def main():
                    "eps": 0.03, "window": "device CSR -> device partition",
                    "rounds": "hipGraph replay" if args.graph_rounds else "stream launches",
                    "autotune": bool(args.autotune)},
+        "host_to_host_ms": round(min(h2h) * 1e3, 3),
+        "host_to_host_edges_per_s": round(m / min(h2h), 1),
         "cut": int(st.cut),
         "imbalance": round(st.imbalance, 5),
         "levels": int(st.nlevels),
@@ -564,8 +615,66 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (batch: 8, plain: 20)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (batch: 2, plain: 3)")
+    ap.add_argument("--form", default="batch", choices=["plain", "batch", "serve"])
+    ap.add_argument("--workload", default="delaunay")
+    ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--graphs", type=int, default=4096, help="batch form: graphs in the pack")
+    ap.add_argument("--plain-reps", type=int, default=3, help="batch form: timed plain-path calls per graph")
+    ap.add_argument("--plain-sample", type=int, default=256, help="batch form: graphs offered to the oracle")
+    ap.add_argument("--no-c2", action="store_true", help="batch form: skip the nested 2^22-point Delaunay run")
+    ap.add_argument("--c2-steps", type=int, default=10, help="batch form: timed steps of the nested plain run")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="batch/serve forms: seconds of oracle work")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve form: Poisson arrival rate (requests/s)")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve form: arrival window (s)")
+    ap.add_argument("--pool", type=int, default=48, help="serve form: distinct graphs requests draw from")
+    ap.add_argument("--check", type=int, default=24, help="serve form: requests re-run alone for bit-exactness")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (CPU baseline + bit-exact check)")
+    ap.add_argument("--no-stage-time", action="store_true")
+    ap.add_argument("--graph-rounds", action="store_true",
+                    help="replay each level's refinement rounds from a captured hipGraph")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time candidate launch shapes of matching and contraction on first use")
+    ap.add_argument("--stage-steps", type=int, default=3,
+                    help="extra steps with per-stage HIP events for the stage table and roofline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain device copy of 1 GiB (PMC calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(graph-level sharding across GPUs is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    gk = load_pkg()
+    import torch
+
+    if args.form == "batch":
+        args.seed = 4 if args.seed is None else args.seed
+        args.steps = 8 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        run_batch(args, gk, torch)
+        return
+    if args.form == "serve":
+        args.seed = 5 if args.seed is None else args.seed
+        run_serve(args, gk, torch)
+        return
+    args.seed = 1 if args.seed is None else args.seed
+    args.steps = 20 if args.steps is None else args.steps
+    args.warmup = 3 if args.warmup is None else args.warmup
+    run_plain(args, gk, torch)
 
 
 if __name__ == "__main__":

@@ -609,6 +609,13 @@ int Engine::refine_step(bool* worked) {
   gkd::fill<long long>(dmoved.p, 0, J, st_);
   GK_TRY(dpaths.alloc(P_, 3LL * J));
   gkd::fill<long long>(dpaths.p, 0, 3LL * J, st_);
+  // Gain-pass statistics: a (vertices, entries) pair per block of the gain
+  // launches (grids of at most kGainSlots blocks), summed after the rounds
+  // into the stage's algorithmic bytes.
+  constexpr int kGainSlots = 2048;
+  Buf<unsigned long long> gwork;
+  GK_TRY(gwork.alloc(P_, 2 * kGainSlots));
+  gkd::fill<unsigned long long>(gwork.p, 0ull, 2 * kGainSlots, st_);
   GK_STAGE(ctx_, GK_ST_PROJECT, 44LL * N,
            gkd::launch_project_b(N, rs.cmap.p, rs.cpart.p, rs.cbnd.p, rs.cgain.p, rs.cdest.p, part->p, bnd->p,
                                  gain.p, dest.p, upd.p, st_));
@@ -624,10 +631,10 @@ int Engine::refine_step(bool* worked) {
   GK_STAGE(ctx_, GK_ST_PROJECT, 4LL * N, gkd::launch_compact(N, upd.p, wl.p, hcnt.p, st_));
   GK_STAGE(ctx_, GK_ST_GAIN, 0,
            gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                               nullptr, N, cand.p, st_));
+                               gwork.p, N, cand.p, st_));
   if (heavy_rows)
     GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
-             gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
+             gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, gwork.p,
                                     cand.p, st_));
   const int64_t avg_deg = N > 0 ? (int64_t(g.nnz) + N - 1) / N : 0;
   const int64_t wl_items = std::min<int64_t>(N, int64_t(J) * M * (1 + 2 * avg_deg));
@@ -653,10 +660,10 @@ int Engine::refine_step(bool* worked) {
                                  lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr, st_));
     GK_STAGE(ctx_, GK_ST_GAIN, 0,
              gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                                 nullptr, wl_items, cand.p, st_));
+                                 gwork.p, wl_items, cand.p, st_));
     if (heavy_rows)
       GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
-               gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, nullptr,
+               gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, gwork.p,
                                       cand.p, st_));
     return GK_OK;
   };
@@ -689,12 +696,22 @@ int Engine::refine_step(bool* worked) {
   prof_ms_[5] += now_ms() - tr0;
   std::vector<long long> hcut(J), hmoved(J), hpaths(3 * size_t(J));
   std::vector<int32_t> hrounds(J), hlast(J);
-  GK_TRY(download({{hpaths.data(), dpaths.p, sizeof(long long) * 3 * J},
+  std::vector<unsigned long long> hwork(2 * kGainSlots);
+  GK_TRY(download({{hwork.data(), gwork.p, sizeof(unsigned long long) * hwork.size()},
+                   {hpaths.data(), dpaths.p, sizeof(long long) * 3 * J},
                    {hcut.data(), dcut.p, sizeof(long long) * J},
                    {hpw.data(), dpw.p, sizeof(int32_t) * hpw.size()},
                    {hrounds.data(), drounds.p, sizeof(int32_t) * J},
                    {hmoved.data(), dmoved.p, sizeof(long long) * J},
                    {hlast.data(), lastcnt.p, sizeof(int32_t) * J}}));
+  if (ctx_->cur) {
+    unsigned long long wv = 0, we = 0;
+    for (int i = 0; i < kGainSlots; ++i) {
+      wv += hwork[2 * i];
+      we += hwork[2 * i + 1];
+    }
+    ctx_->cur->stage_bytes[GK_ST_GAIN] += int64_t(28 * wv + 12 * we);
+  }
   std::vector<UnpackDst> dst(J);
   std::vector<int32_t> done(J, 0);
   bool any_done = false;
