@@ -365,15 +365,21 @@ G coarsen_hem(const G& g, Rng& rng, int64_t maxvw, std::vector<int32_t>& cmap, b
     }
     return c;
   }
-  c.adj.resize(g.adj.size());
-  c.w.resize(g.adj.size());
-  std::vector<int32_t> slot(nc, -1);
+  // Rows are built in per-thread buffers that only ever grow (this path
+  // never waits on the pool, so no other coarsening runs on the thread
+  // meanwhile), then copied out at their exact size.
+  thread_local std::vector<int32_t> tadj, tw, slot;
+  if (tadj.size() < g.adj.size()) {
+    tadj.resize(g.adj.size());
+    tw.resize(g.adj.size());
+  }
+  slot.assign(nc, -1);
   const int32_t* xa = g.xadj.data();
   const int32_t* ad = g.adj.data();
   const int32_t* wt = g.w.data();
   const int32_t* cm = cmap.data();
-  int32_t* ca = c.adj.data();
-  int32_t* cw = c.w.data();
+  int32_t* ca = tadj.data();
+  int32_t* cw = tw.data();
   int32_t* sl = slot.data();
   int32_t top = 0;
   for (int32_t v = 0; v < n; ++v) {
@@ -400,8 +406,8 @@ G coarsen_hem(const G& g, Rng& rng, int64_t maxvw, std::vector<int32_t>& cmap, b
     }
     c.xadj[cv + 1] = top;
   }
-  c.adj.resize(top);
-  c.w.resize(top);
+  c.adj.assign(ca, ca + top);
+  c.w.assign(cw, cw + top);
   return c;
 }
 
@@ -453,10 +459,14 @@ void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int
   // moves of a pass and their rollback.
   for (int32_t v = 0; v < n; ++v) {
     const uint8_t sv = side[v];
-    int64_t a = 0, b = 0;
-    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) (side[g.adj[j]] == sv ? a : b) += g.w[j];
+    int64_t a = 0, t = 0;
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+      const int32_t wj = g.w[j];
+      t += wj;
+      a += side[g.adj[j]] == sv ? wj : 0;
+    }
     id[v] = a;
-    ed[v] = b;
+    ed[v] = t - a;
   }
   for (int pass = 0; pass < passes; ++pass) {
     q[0].reset(n);
