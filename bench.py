@@ -51,6 +51,8 @@ def log(*a):
 def make_graph(gk, workload, n, seed):
     if workload == "delaunay":
         return gk.delaunay(n, seed)
+    if workload == "delaunay_hilbert":
+        return gk.delaunay_hilbert(n, seed)
     if workload == "dag":
         return gk.dag(n, 200, seed)
     if workload == "road":
@@ -372,20 +374,23 @@ def serve_pool(gk, count, seed, lo=1000, hi=2_000_000):
     """Distinct request graphs: vertex counts log-uniform in [lo, hi]; 40%
     Delaunay meshes, 40% layered DAGs (edge weights 1..100, vertex weights
     1..4), 20% road-like (average degree 2.1)."""
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
     ns = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
     kinds = rng.permutation(np.array(["delaunay"] * (count * 2 // 5) + ["dag"] * (count * 2 // 5) +
                                      ["road"] * (count - 2 * (count * 2 // 5))))
-    out = []
-    for i, (n, kind) in enumerate(zip(ns, kinds)):
-        s = seed * 1_000_003 + i
+
+    def make(i):
+        n, kind, s = int(ns[i]), kinds[i], seed * 1_000_003 + i
         if kind == "delaunay":
-            out.append(gk.delaunay(int(n), s))
-        elif kind == "dag":
-            out.append(gk.dag(int(n), int(min(200, max(2, round(float(np.sqrt(n)))))), s))
-        else:
-            out.append(gk.road(int(n), s))
-    return out
+            return gk.delaunay(n, s)
+        if kind == "dag":
+            return gk.dag(n, int(min(200, max(2, round(float(np.sqrt(n)))))), s)
+        return gk.road(n, s)
+
+    # The generators are C calls that release the interpreter lock.
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(make, range(count)))
 
 
 def run_serve(args, gk, torch):
@@ -412,7 +417,15 @@ def run_serve(args, gk, torch):
                 time.sleep(d)
             ids[i] = srv.submit(pool[picks[i]], int(ks[i]))
 
-    th = threading.Thread(target=submitter)
+    backlog = {}
+
+    def submit_all():
+        submitter()
+        q = srv.stats()
+        backlog.update(at_s=time.perf_counter() - t0, queued=int(q.queued), inflight=int(q.inflight),
+                       completed=int(q.completed))
+
+    th = threading.Thread(target=submit_all)
     th.start()
     results = {}
     check_every = max(1, len(arrivals) // args.check)
@@ -445,18 +458,22 @@ def run_serve(args, gk, torch):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle
         tc, mc, ng = 0.0, 0, 0
+        exact_oracle = True
         for i in sorted(results):
             g = pool[picks[i]]
             a = time.perf_counter()
-            _oracle.partition(g, int(ks[i]), initpart_threads=1)
+            opart, _ = _oracle.partition(g, int(ks[i]), initpart_threads=1)
             tc += time.perf_counter() - a
+            exact_oracle &= bool(np.array_equal(opart, results[i]))
             mc += g.m
             ng += 1
             if tc > args.cpu_budget:
                 break
         cpu = {"value": round(mc / tc, 1), "unit": "edges/s", "cores": 1, "kind": "port",
                "sample": f"{ng} of the checked requests partitioned one by one by the single-threaded oracle "
-                         f"({tc:.1f} s)"}
+                         f"({tc:.1f} s)", "host": host_info()}
+        log(f"oracle on {ng} served requests: bit_exact={exact_oracle}")
+        exact &= exact_oracle
     ok = exact and s.failed == 0 and s.completed == len(arrivals)
     out = {
         "metric": "sustained partitioned edges/s at bit-exact cut, eps=3% (serving form, continuous batching)",
@@ -475,6 +492,8 @@ def run_serve(args, gk, torch):
         "config": {"workload": f"Poisson arrivals at {args.rate}/s for {args.duration}s; n log-uniform in [1e3, 2e6]; "
                                f"40% Delaunay, 40% DAG (w 1..100), 20% road (deg 2.1); k in {{8,16,64,256}}",
                    "eps": 0.03, "window": "host CSR submitted -> host partition returned"},
+        "backlog_when_arrivals_ended": backlog,
+        "drain_s": round(wall - float(arrivals[-1]), 2),
         "engine_steps": int(s.steps),
         "footprint_peak_bytes": int(s.footprint_peak),
         "device_peak_bytes": int(s.device_peak),

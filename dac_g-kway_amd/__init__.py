@@ -227,6 +227,7 @@ def gen():
         G.gkgen_dag.argtypes = [c_int32, c_int32, c_uint64, c_int32, c_double, c_int32, c_int32,
                                 c_int32, POINTER(GenGraph)]
         G.gkgen_points.argtypes = [c_int32, c_uint64, I32P, I32P]
+        G.gkgen_delaunay_points.argtypes = [c_int32, I32P, I32P, POINTER(GenGraph)]
         G.gkgen_fnv1a_i32.restype = c_uint64
         G.gkgen_fnv1a_i32.argtypes = [I32P, c_int64, c_uint64]
         _GEN = G
@@ -248,6 +249,45 @@ def delaunay(n, seed) -> Graph:
     rc = gen().gkgen_delaunay(n, seed, ctypes.byref(gg))
     if rc != 0:
         raise RuntimeError(f"gkgen_delaunay failed: {rc}")
+    return _take(gg)
+
+
+def hilbert_index(x, y, bits=24):
+    """Position of each point (x, y in [0, 2^bits)) along the Hilbert curve
+    of the grid (the classic iterative xy -> d mapping)."""
+    x = np.asarray(x, np.int64).copy()
+    y = np.asarray(y, np.int64).copy()
+    d = np.zeros(len(x), np.int64)
+    side = 1 << bits
+    s = side >> 1
+    while s > 0:
+        rx = (x & s) > 0
+        ry = (y & s) > 0
+        d += s * s * ((3 * rx.astype(np.int64)) ^ ry.astype(np.int64))
+        flip = ~ry & rx
+        x[flip] = side - 1 - x[flip]
+        y[flip] = side - 1 - y[flip]
+        swap = ~ry
+        x[swap], y[swap] = y[swap], x[swap].copy()
+        s >>= 1
+    return d
+
+
+def delaunay_hilbert(n, seed) -> Graph:
+    """The triangulation of the same points as delaunay(n, seed), with the
+    vertices numbered in Hilbert-curve order of their points instead of in
+    drawing order (an isomorphic graph whose neighbours have nearby ids)."""
+    x = np.zeros(n, np.int32)
+    y = np.zeros(n, np.int32)
+    rc = gen().gkgen_points(n, seed, x.ctypes.data_as(I32P), y.ctypes.data_as(I32P))
+    if rc != 0:
+        raise RuntimeError(f"gkgen_points failed: {rc}")
+    order = np.argsort(hilbert_index(x, y), kind="stable")
+    xs, ys = np.ascontiguousarray(x[order]), np.ascontiguousarray(y[order])
+    gg = GenGraph()
+    rc = gen().gkgen_delaunay_points(n, xs.ctypes.data_as(I32P), ys.ctypes.data_as(I32P), ctypes.byref(gg))
+    if rc != 0:
+        raise RuntimeError(f"gkgen_delaunay_points failed: {rc}")
     return _take(gg)
 
 

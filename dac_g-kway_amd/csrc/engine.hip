@@ -33,6 +33,8 @@
 #include "../../include/gkway.h"
 #include "batch.hpp"
 #include "engine.hpp"
+
+#include <ctime>
 #include "prim.hpp"
 #include "runtime.hpp"
 
@@ -124,9 +126,10 @@ Engine::~Engine() {
   if (e && e[0] == '1' && steps_ > 0)
     std::fprintf(stderr,
                  "[gk engine] steps %lld rounds %lld | coarsen %.1f ms, init launch %.1f ms, init collect %.1f ms, "
-                 "refine %.1f ms (rounds %.1f ms), idle wait %.1f ms, total %.1f ms\n",
+                 "refine %.1f ms (rounds %.1f ms), idle wait %.1f ms, total %.1f ms; initial partitions %.1f ms "
+                 "of worker CPU\n",
                  (long long)steps_, (long long)rounds_, prof_ms_[0], prof_ms_[1], prof_ms_[2], prof_ms_[3],
-                 prof_ms_[5], prof_ms_[4], now_ms());
+                 prof_ms_[5], prof_ms_[4], now_ms(), init_cpu_us_.load() / 1e3);
 }
 
 int Engine::sync() {
@@ -385,8 +388,10 @@ int Engine::init_step(bool* worked) {
     j.S.nlevels = j.nl;
     j.S.coarsest_n = nt;
     Job* jp = &j;
-    j.fut = pool_.submit([jp]() -> int {
+    j.fut = pool_.submit([this, jp]() -> int {
       Job& J2 = *jp;
+      timespec c0{}, c1{};
+      clock_gettime(CLOCK_THREAD_CPUTIME_ID, &c0);
       const int32_t nt2 = static_cast<int32_t>(J2.hv.size());
       gk_csr hc;
       hc.n = nt2;
@@ -413,6 +418,8 @@ int Engine::init_step(bool* worked) {
           }
       }
       J2.hcut = cut0;
+      clock_gettime(CLOCK_THREAD_CPUTIME_ID, &c1);
+      init_cpu_us_ += (c1.tv_sec - c0.tv_sec) * 1000000LL + (c1.tv_nsec - c0.tv_nsec) / 1000;
       return GK_OK;
     });
     mark(j, INIT_RUNNING);

@@ -10,6 +10,7 @@
 #include <functional>
 #include <future>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <queue>
 #include <string>
@@ -170,6 +171,7 @@ class Engine {
   bool graph_rounds_ = false;
   int64_t steps_ = 0, rounds_ = 0;
   double prof_ms_[6] = {0, 0, 0, 0, 0, 0};  // host wall time per step phase
+  std::atomic<long long> init_cpu_us_{0};   // worker CPU time in initial partitions
   std::vector<std::unique_ptr<Job>> jobs_;
   WorkerPool pool_;  // declared last: joined before the jobs it works on go away
 };

@@ -70,3 +70,35 @@ def test_partition_file(gk, oracle, ctx, tmp_path):
     assert np.array_equal(read_out(out + ".out", g.n), opart)
     assert st.cut == ost.cut
     assert L.gk_partition_file(ctx.handle, b"/nonexistent/x.graph", out.encode(), 8, None) == -2
+
+
+def test_mapped_binary_csr(gk, oracle, ctx, tmp_path):
+    # A graph stored as a binary CSR file, mapped without a copy
+    # (gk_graph_map_bin) and handed to gk_partition straight from the
+    # mapping; the same file through gk_partition_file and the CLI, which
+    # read binary files as they read METIS text.  All equal the oracle.
+    g = gk.dag(30000, 60, 9)
+    path = str(tmp_path / "dag.gkb")
+    gk.write_bin(g, path)
+    c = gk.Csr()
+    h = ctypes.c_void_p()
+    L = gk.lib()
+    assert L.gk_graph_map_bin(path.encode(), 1, ctypes.byref(c), ctypes.byref(h)) == 0
+    try:
+        assert c.n == g.n and c.nnz == g.nnz
+        part = np.zeros(g.n, np.int32)
+        st = gk.Stats()
+        p = gk.params(16)
+        print("mapped binary csr: dag 30000 k=16", flush=True)
+        assert L.gk_partition(ctx.handle, ctypes.byref(c), ctypes.byref(p), part.ctypes.data_as(gk.I32P),
+                              ctypes.byref(st)) == 0
+    finally:
+        L.gk_graph_unmap(h)
+    opart, ost = oracle.partition(g, 16)
+    assert np.array_equal(part, opart) and st.cut == ost.cut
+    out = str(tmp_path / "pf")
+    assert L.gk_partition_file(ctx.handle, path.encode(), out.encode(), 16, None) == 0
+    assert np.array_equal(read_out(out + ".out", g.n), opart)
+    r = subprocess.run([gk.CLI, path, "16", "cli"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert np.array_equal(read_out(tmp_path / "cli.out", g.n), opart)
