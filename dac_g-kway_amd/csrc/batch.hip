@@ -273,8 +273,8 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // job, over the candidate bitmask kept by the gain kernels (boundary vertices
 // with positive gain), so vertices outside the mask cost one bit of a
 // coalesced load.  k_select_fit turns the candidates into the movable set
-// (move fits the cap); k_select_list and k_select_long keep the movable
-// vertices with no smaller-id movable neighbour, as a bitmask; a scan turns
+// (move fits the cap); k_select_list keeps the movable vertices with no
+// smaller-id movable neighbour, as a bitmask; a scan turns
 // the bitmask into a list ordered by vertex id, in which every job's selected
 // vertices form one slice.  No kernel of the selection shares a counter
 // between waves (same-address atomics serialise in L2).  vjob == null: one
@@ -283,7 +283,7 @@ int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, i
 // per 256-thread block.
 constexpr int kSelectWords = 8;
 constexpr int kSelectBlockWords = 4 * kSelectWords;
-// Rows longer than this are checked by k_select_long, one wave per vertex.
+// Rows longer than this are tested by the whole wave, 64 neighbours a step.
 constexpr int32_t kSelectShortRow = 32;
 
 // Position of the k-th (0-based) set bit of w (k < popcount(w)).
@@ -366,9 +366,11 @@ __global__ void __launch_bounds__(256) k_select_fit(int32_t n, const int32_t* __
 // mask's bits to lanes as k_select_fit.  Only vertices with gain >= thr[job]
 // are tested; the others get a bit in defbits, and the round kernel tests
 // them only when fewer than M were selected (every key above the threshold
-// beats every key below it).  Rows longer than kSelectShortRow get a bit in
-// longbits for k_select_long.  Results are bitmask words (selbits, defbits,
-// longbits), all written, so no counter is shared between waves.
+// beats every key below it).  A lane tests a row of up to kSelectShortRow
+// entries itself; longer rows of a dealing pass are then tested one after
+// another by the whole wave, 64 neighbours per step.  Results are bitmask
+// words (selbits, defbits), all written, so no counter is shared between
+// waves.
 __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* __restrict__ xadj,
                                                      const int32_t* __restrict__ adj,
                                                      const int32_t* __restrict__ vjob,
@@ -376,15 +378,14 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
                                                      const int32_t* __restrict__ thr,
                                                      const uint64_t* __restrict__ movable,
                                                      uint64_t* __restrict__ selbits, uint64_t* __restrict__ defbits,
-                                                     uint64_t* __restrict__ longbits, int32_t* __restrict__ bsum) {
-  __shared__ unsigned long long sw[4][3][kSelectWords];
+                                                     int32_t* __restrict__ bsum) {
+  __shared__ unsigned long long sw[4][2][kSelectWords];
   __shared__ int32_t wsum[4];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
   if (lane < kSelectWords) {
     sw[wv][0][lane] = 0;
     sw[wv][1][lane] = 0;
-    sw[wv][2][lane] = 0;
   }
   __syncthreads();
   const int64_t nwords = (int64_t(n) + 63) >> 6;
@@ -397,15 +398,18 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
     const int r = base + lane;
     int wi, bit;
     deal_bit(incl, word, r, &wi, &bit);
+    int32_t v = 0, s = 0, e = 0;
+    int which = -1;  // 0 selected, 1 deferred
+    bool long_row = false;
     if (r < total) {
-      const int32_t v = int32_t(((w0 + wi) << 6) + bit);
+      v = int32_t(((w0 + wi) << 6) + bit);
       const int32_t j = vjob ? vjob[v] : 0;
-      const int32_t s = xadj[v], e = xadj[v + 1];
-      int which = -1;  // 0 selected, 1 deferred, 2 long row
+      s = xadj[v];
+      e = xadj[v + 1];
       if (gain[v] < thr[j]) {
         which = 1;
       } else if (e - s > kSelectShortRow) {
-        which = 2;
+        long_row = true;
       } else {
         // eight neighbours per step: their ids, then their movable bits
         bool sel = true;
@@ -421,15 +425,32 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
         }
         if (sel) which = 0;
       }
-      if (which >= 0) atomicOr(&sw[wv][which][wi], 1ull << bit);
     }
+    // The pass's long rows, one at a time, by the whole wave.
+    uint64_t lm = ballot(long_row);
+    while (lm) {
+      const int src = __ffsll(static_cast<long long>(lm)) - 1;
+      lm &= lm - 1;
+      const int32_t lv = __shfl(v, src, 64), ls = __shfl(s, src, 64), le = __shfl(e, src, 64);
+      bool blocked = false;
+      for (int32_t x0 = ls; x0 < le && !blocked; x0 += 64) {
+        const int32_t x = x0 + lane;
+        bool b = false;
+        if (x < le) {
+          const int32_t u = adj[x];
+          b = u < lv && ((movable[u >> 6] >> (u & 63)) & 1ull);
+        }
+        blocked = ballot(b) != 0;
+      }
+      if (lane == src && !blocked) which = 0;
+    }
+    if (which >= 0) atomicOr(&sw[wv][which][wi], 1ull << bit);
   }
   __syncthreads();
   int c = 0;
   if (lane < kSelectWords && w0 + lane < nwords) {
     selbits[w0 + lane] = sw[wv][0][lane];
     defbits[w0 + lane] = sw[wv][1][lane];
-    longbits[w0 + lane] = sw[wv][2][lane];
     c = popc64(sw[wv][0][lane]);
   }
   // selected vertices of the block's kSelectBlockWords words
@@ -437,53 +458,6 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
   if (lane == 0) wsum[wv] = c;
   __syncthreads();
   if (threadIdx.x == 0) bsum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-}
-
-// Vertices with rows longer than kSelectShortRow: a wave per vertex, 64
-// neighbours at a time, over the words of longbits (grid-stride); selected
-// ones are OR-ed into selbits.
-__global__ void __launch_bounds__(256) k_select_long(int32_t n, const uint64_t* __restrict__ longbits,
-                                                     const int32_t* __restrict__ xadj,
-                                                     const int32_t* __restrict__ adj,
-                                                     const uint64_t* __restrict__ movable,
-                                                     uint64_t* __restrict__ selbits, int32_t* __restrict__ bsum) {
-  const int lane = lane_id();
-  const int64_t nwords = (int64_t(n) + 63) >> 6;
-  const int64_t nwaves = int64_t(gridDim.x) * 4;
-  const int64_t wave = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  // 64 words per wave and step, one per lane, interleaved over the waves (so
-  // dense runs of long rows spread over many waves); then the non-empty ones
-  for (int64_t base = wave; base < nwords; base += nwaves * 64) {
-    const int64_t wl = base + int64_t(lane) * nwaves;
-    const uint64_t mine = wl < nwords ? longbits[wl] : 0;
-    uint64_t nz = ballot(mine != 0);
-    while (nz) {
-      const int src = __ffsll(static_cast<long long>(nz)) - 1;
-      nz &= nz - 1;
-      const int64_t w = base + int64_t(src) * nwaves;
-      uint64_t lb = __shfl(mine, src, 64);
-      while (lb) {
-        const int bit = __ffsll(static_cast<long long>(lb)) - 1;
-        lb &= lb - 1;
-        const int32_t v = int32_t((w << 6) + bit);
-        const int32_t s = xadj[v], e = xadj[v + 1];
-        bool blocked = false;
-        for (int32_t x0 = s; x0 < e && !blocked; x0 += 64) {
-          const int32_t x = x0 + lane;
-          bool b = false;
-          if (x < e) {
-            const int32_t u = adj[x];
-            b = u < v && ((movable[u >> 6] >> (u & 63)) & 1ull);
-          }
-          blocked = ballot(b) != 0;
-        }
-        if (!blocked && lane == 0) {
-          atomicOr(reinterpret_cast<unsigned long long*>(&selbits[w]), 1ull << bit);
-          atomicAdd(&bsum[w / kSelectBlockWords], 1);
-        }
-      }
-    }
-  }
 }
 
 // Ordered list of the selected vertices: every word's first list slot
@@ -544,19 +518,14 @@ int select_scratch_ints(int64_t n) {
 int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
                        const int32_t* pw, const int32_t* pw_off, const long long* cap, const int32_t* thr,
                        const uint64_t* cand, const SelectBufs& sb, int32_t* reset, int32_t* active,
-                       int32_t max_deg, hipStream_t st) {
+                       hipStream_t st) {
   if (g.n <= 0) return 0;
   const int64_t nwords = (int64_t(g.n) + 63) / 64;
   const int64_t waves = (nwords + kSelectWords - 1) / kSelectWords;
   const unsigned grid = grid_for(waves * 64, 256);
   k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable, reset, active);
   k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, gain, thr, sb.movable, sb.selbits, sb.defbits,
-                                      sb.longbits, sb.scratch);
-  if (max_deg > kSelectShortRow) {  // no long rows at all: nothing to sweep
-    unsigned lgrid = grid_for(nwords, 256);  // up to a wave per 64 words' worth of lanes
-    if (lgrid > 1024) lgrid = 1024;
-    k_select_long<<<lgrid, 256, 0, st>>>(g.n, sb.longbits, g.xadj, g.adj, sb.movable, sb.selbits, sb.scratch);
-  }
+                                      sb.scratch);
   k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list);
   return hip_status();
 }

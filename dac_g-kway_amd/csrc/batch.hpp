@@ -60,14 +60,13 @@ int launch_job_eoffsets(int32_t nj, const int32_t* c_off, const int32_t* cxadj, 
 int launch_unpack_part(int32_t nj, const UnpackDst* dst, const int32_t* v_off, int32_t n, const int32_t* part,
                        const int32_t* bnd, hipStream_t st);
 // Buffers of the round selection (n vertices, nwords = (n + 63) / 64):
-// movable, selbits, defbits, longbits and wpos hold nwords + 1 entries, the
+// movable, selbits, defbits and wpos hold nwords + 1 entries, the
 // last word of selbits zero (filled once); list holds n ints; scratch holds
 // select_scratch_ints(n) ints (per-block selection counts).
 struct SelectBufs {
   uint64_t* movable;
   uint64_t* selbits;
   uint64_t* defbits;
-  uint64_t* longbits;
   int32_t* wpos;
   int32_t* list;
   int* scratch;
@@ -78,8 +77,7 @@ int select_scratch_ints(int64_t n);
 // movable set (candidates whose move fits the job's cap), the selection
 // (no smaller-id movable neighbour) as a bitmask, and its ordered list; a
 // job's movable vertices with gain below thr[job] (1 before a level's first
-// round) are only marked in defbits (max_deg: the graph's largest row, to
-// skip the long-row pass when there is none).  launch_round_b: one workgroup per job
+// round) are only marked in defbits.  launch_round_b: one workgroup per job
 // (best min(count, M) moves, LDS sort, balanced prefix, apply), which tests
 // the deferred vertices when fewer than M were selected (into the job's
 // segment of xlist, n ints) and sets the job's next threshold.  *active
@@ -95,7 +93,7 @@ int select_scratch_ints(int64_t n);
 int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
                        const int32_t* pw, const int32_t* pw_off, const long long* cap, const int32_t* thr,
                        const uint64_t* cand, const SelectBufs& sb, int32_t* reset, int32_t* active,
-                       int32_t max_deg, hipStream_t st);
+                       hipStream_t st);
 int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, const int32_t* kk, const long long* cap,
                    const int32_t* pw_off, const DGraph& g, int32_t* part, const int32_t* dest, const int32_t* gain,
                    int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const SelectBufs& sb,

@@ -430,7 +430,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     Buf<int32_t> xlist, dthr, wpos, sscr;
     Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, active, drounds, lastcnt, dvoff, dpw_off, dkk;
     Buf<long long> dcap, dmoved, dpaths;
-    Buf<uint64_t> cand, movable, selbits, defbits, longbits;
+    Buf<uint64_t> cand, movable, selbits, defbits;
     Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
     GK_TRY(dpaths.alloc(P, 3));
@@ -452,7 +452,6 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(movable.alloc(P, nwords0 + 1));
     GK_TRY(selbits.alloc(P, nwords0 + 1));
     GK_TRY(defbits.alloc(P, nwords0 + 1));
-    GK_TRY(longbits.alloc(P, nwords0 + 1));
     GK_TRY(wpos.alloc(P, nwords0 + 1));
     GK_TRY(sscr.alloc(P, gkd::select_scratch_ints(n0)));
     GK_TRY(xlist.alloc(P, n0));
@@ -470,7 +469,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(dkk.alloc(P, 1));
     GK_TRY(dcap.alloc(P, 1));
     gkd::fill<uint64_t>(selbits.p, 0ull, nwords0 + 1, st);
-    const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
+    const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p};
     gkd::fill<int32_t>(dvoff.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p + 1, k, 1, st);
@@ -514,7 +513,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       auto one_round = [&]() -> int {
         GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
                  gkd::launch_select_list(g, nullptr, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
-                                         hcnt.p, active.p, maxdeg, st));
+                                         hcnt.p, active.p, st));
         GK_STAGE(ctx, GK_ST_BALANCE, 0,
                  gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p, pw.p,
                                      dcut.p, drounds.p + l, dmoved.p + l, sb, xlist.p, hkey.p, upd.p, wl.p,
@@ -1070,7 +1069,7 @@ int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t ma
   Buf<int32_t> upd, wl, heavy, hcnt, list, xlist, wpos, sscr, dthr, active, drounds, lastcnt, dvoff, dpw_off, dkk,
       dorder, dmax;
   Buf<long long> dcap, dcut, dmoved, dpaths;
-  Buf<uint64_t> cand, movable, selbits, defbits, longbits;
+  Buf<uint64_t> cand, movable, selbits, defbits;
   Buf<uint32_t> hkey;
   GK_TRY(upd.alloc(P, n));
   GK_TRY(wl.alloc(P, n));
@@ -1085,7 +1084,6 @@ int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t ma
   GK_TRY(movable.alloc(P, nwords + 1));
   GK_TRY(selbits.alloc(P, nwords + 1));
   GK_TRY(defbits.alloc(P, nwords + 1));
-  GK_TRY(longbits.alloc(P, nwords + 1));
   GK_TRY(dthr.alloc(P, 1));
   GK_TRY(active.alloc(P, 1));
   GK_TRY(drounds.alloc(P, 1));
@@ -1119,13 +1117,13 @@ int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t ma
   int32_t maxdeg = 0;
   GK_TRY(fetch(ctx, st, {{&maxdeg, dmax.p, sizeof maxdeg}}));
   const int lanes = gain_lanes(g);
-  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
+  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p};
   for (int32_t r = 0; r < nrounds; ++r) {
     // B = 0, pos = -1 unless the round orders any move.
     gkd::fill<int32_t>(dorder.p, 0, 1, st);
     gkd::fill<int32_t>(dorder.p + 1, -1, 1, st);
     GK_TRY(gkd::launch_select_list(g, nullptr, gain, dest, pw, dpw_off.p, dcap.p, dthr.p, cand.p, sb, hcnt.p,
-                                   active.p, maxdeg, st));
+                                   active.p, st));
     GK_TRY(gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, part, dest, gain, pw, dcut.p,
                                drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p, lastcnt.p, active.p,
                                cand.p, dthr.p, dpaths.p, dorder.p, st));

@@ -570,7 +570,7 @@ int Engine::refine_step(bool* worked) {
   Buf<int32_t> dpw_off, dkk, dpw, upd, wl, heavy, list, hcnt, active, drounds, lastcnt, dmax, xlist, dthr, wpos,
       sscr;
   Buf<long long> dcap, dcut, dmoved, dpaths;
-  Buf<uint64_t> cand, movable, selbits, defbits, longbits;
+  Buf<uint64_t> cand, movable, selbits, defbits;
   Buf<uint32_t> hkey;
   auto part = std::make_shared<Buf<int32_t>>();
   auto bnd = std::make_shared<Buf<int32_t>>();
@@ -596,14 +596,13 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(movable.alloc(P_, nwords + 1));
   GK_TRY(selbits.alloc(P_, nwords + 1));
   GK_TRY(defbits.alloc(P_, nwords + 1));
-  GK_TRY(longbits.alloc(P_, nwords + 1));
   GK_TRY(wpos.alloc(P_, nwords + 1));
   GK_TRY(sscr.alloc(P_, gkd::select_scratch_ints(N)));
   GK_TRY(xlist.alloc(P_, N));
   GK_TRY(dthr.alloc(P_, J));
   gkd::fill<int32_t>(dthr.p, 1, J, st_);
   gkd::fill<uint64_t>(selbits.p, 0ull, nwords + 1, st_);
-  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, longbits.p, wpos.p, list.p, sscr.p};
+  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p};
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
@@ -660,7 +659,7 @@ int Engine::refine_step(bool* worked) {
   auto one_round = [&]() -> int {
     GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
              gkd::launch_select_list(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
-                                     hcnt.p, active.p, maxdeg, st_));
+                                     hcnt.p, active.p, st_));
     GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
              gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
                                  dpw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
