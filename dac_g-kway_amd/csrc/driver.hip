@@ -486,10 +486,11 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       std::swap(cur_part, oth_part);
       std::swap(cur_bnd, oth_bnd);
       const int lanes = gain_lanes(g);
-      // Rows longer than the lane count go to the heavy-row kernel; without
-      // any, its launches are skipped.
+      // Rows longer than the lane count: the gain pass takes them itself for
+      // k <= kGainInlineK, else they go to the heavy-row kernel, whose
+      // launches are skipped when there are none.
       const int32_t maxdeg = hmaxdeg[l];
-      const bool heavy_rows = maxdeg > lanes;
+      const bool heavy_rows = maxdeg > lanes && gkd::gain_heavy_separate(k);
       // After projection every gain is 0: no candidate yet.
       gkd::fill<uint64_t>(cand.p, 0ull, (int64_t(g.n) + 63) / 64 + 1, st);
       gkd::fill<int32_t>(dvoff.p + 1, g.n, 1, st);
@@ -940,7 +941,8 @@ int gk_k_gain(gk_ctx* ctx, const gk_csr* dg, int32_t k, const int32_t* part, con
   GK_TRY(gkd::launch_compact(g.n, flags.p, wl.p, cnt.p, st));
   GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, cnt.p, heavy.p, part, flags.p, bnd, gain, dest, nullptr, g.n, nullptr,
                               st));
-  GK_TRY(gkd::launch_gain_heavy(g, k, heavy.p, cnt.p, part, flags.p, bnd, gain, dest, nullptr, nullptr, st));
+  if (gkd::gain_heavy_separate(k))
+    GK_TRY(gkd::launch_gain_heavy(g, k, heavy.p, cnt.p, part, flags.p, bnd, gain, dest, nullptr, nullptr, st));
   GK_HIP(hipStreamSynchronize(st));
   return GK_OK;
 }
@@ -1129,7 +1131,7 @@ int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t ma
                                cand.p, dthr.p, dpaths.p, dorder.p, st));
     GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, part, upd.p, bnd, gain, dest, nullptr, n,
                                 cand.p, st));
-    if (maxdeg > lanes)
+    if (maxdeg > lanes && gkd::gain_heavy_separate(k))
       GK_TRY(gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, part, upd.p, bnd, gain, dest, nullptr, cand.p, st));
     int32_t hdr[2] = {0, -1};
     GK_TRY(fetch(ctx, st, {{hdr, dorder.p, sizeof hdr}}));

@@ -627,12 +627,17 @@ int Engine::refine_step(bool* worked) {
                                  gain.p, dest.p, upd.p, st_));
   GK_STAGE(ctx_, GK_ST_PROJECT, 8LL * N + N / 8, gkd::launch_cand_init(N, bnd->p, gain.p, cand.p, st_));
   const int lanes = gain_lanes(g);
-  // Rows longer than the gain pass's lane count go to the heavy-row kernel;
-  // when there are none in the pack, that launch is skipped.
-  int32_t maxdeg = 0;
-  GK_TRY(gkd::launch_max_deg(g, dmax.p, st_));
-  GK_TRY(fetch(ctx_, st_, {{&maxdeg, dmax.p, sizeof maxdeg}}));
-  const bool heavy_rows = maxdeg > lanes;
+  // Rows longer than the gain pass's lane count: the pass takes them itself
+  // for k <= kGainInlineK; above it they go to the heavy-row kernel, whose
+  // launch is skipped when the pack has none (which costs a read-back of the
+  // largest row length).
+  bool heavy_rows = false;
+  if (gkd::gain_heavy_separate(kmax)) {
+    int32_t maxdeg = 0;
+    GK_TRY(gkd::launch_max_deg(g, dmax.p, st_));
+    GK_TRY(fetch(ctx_, st_, {{&maxdeg, dmax.p, sizeof maxdeg}}));
+    heavy_rows = maxdeg > lanes;
+  }
   gkd::fill<int32_t>(hcnt.p, 0, 2, st_);
   GK_STAGE(ctx_, GK_ST_PROJECT, 4LL * N, gkd::launch_compact(N, upd.p, wl.p, hcnt.p, st_));
   GK_STAGE(ctx_, GK_ST_GAIN, 0,

@@ -82,6 +82,11 @@ int launch_compact(int32_t n, const int32_t* flags, int32_t* list, int32_t* cnt,
 // through heavy[] (count wlc[1]) to the LDS-histogram kernel.  Clears upd of
 // every processed vertex.  work (may be null) accumulates [0] vertices
 // recomputed, [1] their entries.
+// Part counts up to which the gain pass takes rows longer than its lane
+// count itself (per-wave LDS histograms); above it they go to the heavy list
+// and launch_gain_heavy.
+constexpr int32_t kGainInlineK = 256;
+inline bool gain_heavy_separate(int32_t k) { return k > kGainInlineK; }
 int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
                    const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
                    unsigned long long* work, int64_t max_items, uint64_t* cand, hipStream_t st);

@@ -101,11 +101,14 @@ __device__ __forceinline__ void set_cand(uint64_t* cand, int32_t v, bool on) {
 // the weights of the lanes with the same part (an O(S) shuffle loop), which
 // gives the connectivity to every adjacent part without a histogram.  Gain is
 // the signed max over the adjacent parts of conn_q - conn_p, floored at 0,
-// lowest q on ties.  Rows longer than S are appended to the heavy list
-// (wlc[1] entries) for k_gain_heavy.  Processed vertices get upd[v] = 0.
+// lowest q on ties.  Rows longer than S: with k <= kGainInlineK the whole
+// wave takes them one at a time after each pass, with a per-wave LDS
+// histogram of the k connectivities (as k_gain_heavy); with larger k they are
+// appended to the heavy list (wlc[1] entries) for k_gain_heavy.  Processed
+// vertices get upd[v] = 0.
 template <int S>
 __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
-                                                 const int32_t* __restrict__ xadj,
+                                                 int32_t k, const int32_t* __restrict__ xadj,
                                                  const int32_t* __restrict__ adj,
                                                  const int32_t* __restrict__ w,
                                                  const int32_t* __restrict__ part,
@@ -114,65 +117,130 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
                                                  int32_t* __restrict__ heavy,
                                                  unsigned long long* __restrict__ work,
                                                  uint64_t* __restrict__ cand) {
+  __shared__ int32_t hh[4][kGainInlineK];
   const int32_t cnt = wlc[0];
-  const int64_t sg0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) / S;
-  const int64_t nsg = int64_t(gridDim.x) * (256 / S);
+  const int lane = lane_id();
+  const int wv = threadIdx.x >> 6;
   const int sub = threadIdx.x & (S - 1);
+  const bool inline_heavy = k <= kGainInlineK;
+  // Group g of the grid (S lanes) takes worklist entries g, g + nsg, ...;
+  // the loop runs while any group of the wave has an entry, so the wave can
+  // take its long rows together.
+  const int64_t nsg = int64_t(gridDim.x) * (256 / S);
+  const int64_t wave_g0 = (int64_t(blockIdx.x) * 4 + wv) * (64 / S);
+  const int grp = lane / S;
   int done_v = 0, done_e = 0;
-  for (int64_t e = sg0; e < cnt; e += nsg) {
-    const int32_t v = wl[e];
-    const int32_t s = xadj[v], d = xadj[v + 1] - s;
-    if (d > S) {
-      if (sub == 0) heavy[atomicAdd(&wlc[1], 1)] = v;
-      continue;
+  for (int64_t eb = wave_g0; eb < cnt; eb += nsg) {
+    const int64_t e = eb + grp;
+    int32_t v = 0, s = 0, d = 0;
+    if (e < cnt) {
+      v = wl[e];
+      s = xadj[v];
+      d = xadj[v + 1] - s;
     }
-    const int32_t p = part[v];
-    int32_t q = -1, wj = 0;
-    if (sub < d) {
-      q = part[adj[s + sub]];
-      wj = w[s + sub];
-    }
-    int32_t conn = 0, id = 0;
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const int32_t qi = __shfl(q, i, S);
-      const int32_t wi = __shfl(wj, i, S);
-      conn += (qi == q) ? wi : 0;
-      id += (qi == p) ? wi : 0;
-    }
-    const bool ext = q >= 0 && q != p;
-    int32_t g = ext ? conn - id : INT32_MIN;
-    int32_t bq = ext ? q : INT32_MAX;
-    int32_t anyext = ext ? 1 : 0;
-#pragma unroll
-    for (int o = S / 2; o > 0; o >>= 1) {
-      const int32_t og = __shfl_xor(g, o, S), oq = __shfl_xor(bq, o, S);
-      if (og > g || (og == g && oq < bq)) { g = og; bq = oq; }
-      anyext |= __shfl_xor(anyext, o, S);
-    }
-    if (sub == 0) {
-      bnd[v] = anyext;
-      if (anyext && g > 0) {
-        gain[v] = g;
-        dest[v] = bq;
-      } else {
-        gain[v] = 0;
-        dest[v] = p;
+    const bool long_row = e < cnt && d > S;
+    if (long_row && !inline_heavy && sub == 0) heavy[atomicAdd(&wlc[1], 1)] = v;
+    if (e < cnt && d <= S) {
+      const int32_t p = part[v];
+      int32_t q = -1, wj = 0;
+      if (sub < d) {
+        q = part[adj[s + sub]];
+        wj = w[s + sub];
       }
-      if (cand) set_cand(cand, v, anyext && g > 0);
-      upd[v] = 0;
-      ++done_v;
-      done_e += d;
+      int32_t conn = 0, id = 0;
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const int32_t qi = __shfl(q, i, S);
+        const int32_t wi = __shfl(wj, i, S);
+        conn += (qi == q) ? wi : 0;
+        id += (qi == p) ? wi : 0;
+      }
+      const bool ext = q >= 0 && q != p;
+      int32_t g = ext ? conn - id : INT32_MIN;
+      int32_t bq = ext ? q : INT32_MAX;
+      int32_t anyext = ext ? 1 : 0;
+#pragma unroll
+      for (int o = S / 2; o > 0; o >>= 1) {
+        const int32_t og = __shfl_xor(g, o, S), oq = __shfl_xor(bq, o, S);
+        if (og > g || (og == g && oq < bq)) { g = og; bq = oq; }
+        anyext |= __shfl_xor(anyext, o, S);
+      }
+      if (sub == 0) {
+        bnd[v] = anyext;
+        if (anyext && g > 0) {
+          gain[v] = g;
+          dest[v] = bq;
+        } else {
+          gain[v] = 0;
+          dest[v] = p;
+        }
+        if (cand) set_cand(cand, v, anyext && g > 0);
+        upd[v] = 0;
+        ++done_v;
+        done_e += d;
+      }
+    }
+    if (!inline_heavy) continue;
+    // The pass's long rows, one at a time, by the whole wave.
+    int32_t* hist = hh[wv];
+    uint64_t lm = ballot(long_row && sub == 0);
+    while (lm) {
+      const int src = __ffsll(static_cast<long long>(lm)) - 1;
+      lm &= lm - 1;
+      const int32_t hv = __shfl(v, src, 64), hs = __shfl(s, src, 64), he = hs + __shfl(d, src, 64);
+      for (int32_t q = lane; q < k; q += 64) hist[q] = 0;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const int32_t p = part[hv];
+      bool ext = false;
+      for (int32_t j = hs + lane; j < he; j += 64) {
+        const int32_t q = part[adj[j]];
+        atomicAdd(&hist[q], w[j]);
+        ext |= (q != p);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const bool any = ballot(ext) != 0;
+      const int32_t id = hist[p];
+      int32_t g = INT32_MIN, bq = INT32_MAX;
+      for (int32_t q = lane; q < k; q += 64) {
+        const int32_t h = hist[q];
+        if (q != p && h > 0) {
+          const int32_t gq = h - id;
+          if (gq > g || (gq == g && q < bq)) { g = gq; bq = q; }
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const int32_t og = __shfl_xor(g, o, 64), oq = __shfl_xor(bq, o, 64);
+        if (og > g || (og == g && oq < bq)) { g = og; bq = oq; }
+      }
+      if (lane == 0) {
+        bnd[hv] = any ? 1 : 0;
+        if (any && g > 0) {
+          gain[hv] = g;
+          dest[hv] = bq;
+        } else {
+          gain[hv] = 0;
+          dest[hv] = p;
+        }
+        if (cand) set_cand(cand, hv, any && g > 0);
+        upd[hv] = 0;
+        ++done_v;
+        done_e += he - hs;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
   }
   if (work) {
     // Statistics: one slot pair per block (no two blocks of a launch share
     // one, launches are stream-ordered), summed on the host.
     __shared__ int ws[2][4];
-    const int a = wave_sum(done_v), b = wave_sum(done_e);
+    const int a2 = wave_sum(done_v), b2 = wave_sum(done_e);
     if (lane_id() == 0) {
-      ws[0][threadIdx.x >> 6] = a;
-      ws[1][threadIdx.x >> 6] = b;
+      ws[0][threadIdx.x >> 6] = a2;
+      ws[1][threadIdx.x >> 6] = b2;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -255,7 +323,7 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
   if (grid > 2048) grid = 2048;
 #define GK_GAIN_CASE(S)                                                                            \
   case S:                                                                                           \
-    k_gain_wl<S><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work, cand); \
+    k_gain_wl<S><<<grid, 256, 0, st>>>(wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work, cand); \
     break;
   switch (lanes) {
     GK_GAIN_CASE(4)
@@ -263,8 +331,8 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
     GK_GAIN_CASE(16)
     GK_GAIN_CASE(32)
     default:
-      k_gain_wl<64><<<grid, 256, 0, st>>>(wl, wlc, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work,
-                                          cand);
+      k_gain_wl<64><<<grid, 256, 0, st>>>(wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy,
+                                          work, cand);
       break;
   }
 #undef GK_GAIN_CASE
