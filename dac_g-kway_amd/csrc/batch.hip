@@ -330,8 +330,10 @@ __global__ void __launch_bounds__(256) k_select_fit(int32_t n, const int32_t* __
                                                     const long long* __restrict__ cap,
                                                     const uint64_t* __restrict__ cand,
                                                     uint64_t* __restrict__ movable, int32_t* __restrict__ reset,
-                                                    int32_t* __restrict__ active) {
+                                                    int32_t* __restrict__ active,
+                                                    unsigned long long* __restrict__ abytes) {
   __shared__ unsigned long long mw[4][kSelectWords];
+  __shared__ int32_t wtot[4];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     reset[0] = 0;
     reset[1] = 0;
@@ -357,8 +359,14 @@ __global__ void __launch_bounds__(256) k_select_fit(int32_t n, const int32_t* __
       if ((long long)pw[pw_off[j] + dest[v]] + vw[v] <= cap[j]) atomicOr(&mw[wv][wi], 1ull << bit);
     }
   }
+  if (lane == 0) wtot[wv] = total;
   __syncthreads();
   if (lane < kSelectWords && w0 + lane < nwords) movable[w0 + lane] = mw[wv][lane];
+  // Algorithmic bytes: the block's candidate and movable words, and per
+  // candidate its job, destination, part weight and vertex weight.
+  if (abytes && threadIdx.x == 0)
+    atomicAdd(&abytes[blockIdx.x % kSelectByteSlots],
+              (unsigned long long)(16 * kSelectBlockWords + 16 * (wtot[0] + wtot[1] + wtot[2] + wtot[3])));
 }
 
 // Selection proper: a movable vertex is selected when no smaller-id
@@ -378,9 +386,11 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
                                                      const int32_t* __restrict__ thr,
                                                      const uint64_t* __restrict__ movable,
                                                      uint64_t* __restrict__ selbits, uint64_t* __restrict__ defbits,
-                                                     int32_t* __restrict__ bsum) {
+                                                     int32_t* __restrict__ bsum,
+                                                     unsigned long long* __restrict__ abytes) {
   __shared__ unsigned long long sw[4][2][kSelectWords];
   __shared__ int32_t wsum[4];
+  __shared__ long long wbytes[4];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
   if (lane < kSelectWords) {
@@ -394,6 +404,7 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
   if (lane < kSelectWords && w0 + lane < nwords) word = movable[w0 + lane];
   const int incl = wave_incl_scan(popc64(word));
   const int total = __shfl(incl, kSelectWords - 1, 64);
+  long long tested = 0;  // row entries of the tested vertices
   for (int base = 0; base < total; base += 64) {
     const int r = base + lane;
     int wi, bit;
@@ -409,8 +420,10 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
       if (gain[v] < thr[j]) {
         which = 1;
       } else if (e - s > kSelectShortRow) {
+        tested += e - s;
         long_row = true;
       } else {
+        tested += e - s;
         // eight neighbours per step: their ids, then their movable bits
         bool sel = true;
         for (int32_t x = s; x < e && sel; x += 8) {
@@ -455,9 +468,21 @@ __global__ void __launch_bounds__(256) k_select_list(int32_t n, const int32_t* _
   }
   // selected vertices of the block's kSelectBlockWords words
   c = wave_sum(c);
-  if (lane == 0) wsum[wv] = c;
+  tested = wave_sum64(tested);
+  if (lane == 0) {
+    wsum[wv] = c;
+    // movable words read, selected and deferred words written; per movable
+    // vertex its row bounds, job and gain, per tested entry the neighbour id
+    // and one movable bit
+    wbytes[wv] = 8LL * total + 4LL * tested + tested / 8;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) bsum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (abytes)
+      atomicAdd(&abytes[blockIdx.x % kSelectByteSlots],
+                (unsigned long long)(24LL * kSelectBlockWords + wbytes[0] + wbytes[1] + wbytes[2] + wbytes[3]));
+  }
 }
 
 // Ordered list of the selected vertices: every word's first list slot
@@ -469,7 +494,8 @@ constexpr int kScatterWords = 2048;
 
 __global__ void __launch_bounds__(256) k_sel_scatter(int64_t nwords, const uint64_t* __restrict__ selbits,
                                                      const int32_t* __restrict__ bsum, int32_t* __restrict__ wpos,
-                                                     int32_t* __restrict__ list) {
+                                                     int32_t* __restrict__ list,
+                                                     unsigned long long* __restrict__ abytes) {
   __shared__ int lds[8];
   const int64_t wb = int64_t(blockIdx.x) * kScatterWords;
   // offset: selection blocks [0, wb / kSelectBlockWords)
@@ -500,6 +526,10 @@ __global__ void __launch_bounds__(256) k_sel_scatter(int64_t nwords, const uint6
     }
   }
   if (wb + kScatterWords >= nwords && threadIdx.x == 255) wpos[nwords] = r;
+  // Algorithmic bytes: the block's selected words read and word positions
+  // written, and one list slot per selected vertex (tot: the block's count).
+  if (abytes && threadIdx.x == 0)
+    atomicAdd(&abytes[blockIdx.x % kSelectByteSlots], (unsigned long long)(12LL * kScatterWords + 4LL * tot));
 }
 
 // List slot of vertex x (the number of selected vertices before it).
@@ -523,10 +553,12 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain
   const int64_t nwords = (int64_t(g.n) + 63) / 64;
   const int64_t waves = (nwords + kSelectWords - 1) / kSelectWords;
   const unsigned grid = grid_for(waves * 64, 256);
-  k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable, reset, active);
+  k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable, reset, active,
+                                     sb.abytes);
   k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, gain, thr, sb.movable, sb.selbits, sb.defbits,
-                                      sb.scratch);
-  k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list);
+                                      sb.scratch, sb.abytes);
+  k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list,
+                                                                  sb.abytes);
   return hip_status();
 }
 
@@ -589,7 +621,8 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     int32_t* __restrict__ rounds, long long* __restrict__ moved, const SelectBufs sb, int32_t* __restrict__ xlist,
     uint32_t* __restrict__ hkey, int32_t* __restrict__ upd, int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
     int32_t* __restrict__ lastcnt, int32_t* __restrict__ active, uint64_t* __restrict__ cand,
-    int32_t* __restrict__ thr, long long* __restrict__ paths, int32_t* __restrict__ order) {
+    int32_t* __restrict__ thr, long long* __restrict__ paths, int32_t* __restrict__ order,
+    long long* __restrict__ abytes) {
   __shared__ RoundLds R;
   extern __shared__ int32_t dpw[];
   const int t = threadIdx.x;
@@ -875,6 +908,11 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     thr[j] = R.lz[2];
     rounds[j] += 1;
     moved[j] += pos + 1;
+    // Algorithmic bytes of the round: each selected vertex's id and gain,
+    // each ordered move's weight, gain, part and destination, the job's part
+    // weights, and per applied item (a mover or a neighbour) its id, update
+    // flag and worklist slot.
+    if (abytes) abytes[j] += 8LL * count + 16LL * B + 4LL * kk[j] + 12LL * tot;
   }
 }
 
@@ -883,13 +921,13 @@ int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, co
                    int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const SelectBufs& sb,
                    int32_t* xlist, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
                    int32_t* active, uint64_t* cand, int32_t* thr, long long* paths, int32_t* order,
-                   hipStream_t st) {
+                   long long* abytes, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
   k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, M, kk, cap, pw_off, g.xadj, g.adj, part, dest,
                                                              gain, g.vw, pw, cut, rounds, moved, sb, xlist, hkey,
                                                              upd, wl, wlc, lastcnt, active, cand, thr, paths,
-                                                             order);
+                                                             order, abytes);
   return hip_status();
 }
 

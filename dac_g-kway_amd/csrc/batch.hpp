@@ -70,7 +70,11 @@ struct SelectBufs {
   int32_t* wpos;
   int32_t* list;
   int* scratch;
+  // May be null: kSelectByteSlots slots to which the selection kernels add
+  // their algorithmic bytes (block b into slot b % kSelectByteSlots).
+  unsigned long long* abytes;
 };
+constexpr int kSelectByteSlots = 2048;
 int select_scratch_ints(int64_t n);
 
 // Refinement round without host round trips.  launch_select_list: the
@@ -89,7 +93,8 @@ int select_scratch_ints(int64_t n);
 // that ran the top-M radix select, rounds whose compressed keys exceeded 32
 // bits, rounds that tested deferred vertices.  order (may be null): per job
 // M + 2 ints, [0] = B, [1] = pos and [2, 2 + B) the ordered moves of the
-// job's latest round that selected any.
+// job's latest round that selected any.  abytes (may be null): per job, the
+// algorithmic bytes of its rounds are added.
 int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain, const int32_t* dest,
                        const int32_t* pw, const int32_t* pw_off, const long long* cap, const int32_t* thr,
                        const uint64_t* cand, const SelectBufs& sb, int32_t* reset, int32_t* active,
@@ -99,7 +104,7 @@ int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, co
                    int32_t* pw, long long* cut, int32_t* rounds, long long* moved, const SelectBufs& sb,
                    int32_t* xlist, uint32_t* hkey, int32_t* upd, int32_t* wl, int32_t* wlc, int32_t* lastcnt,
                    int32_t* active, uint64_t* cand, int32_t* thr, long long* paths, int32_t* order,
-                   hipStream_t st);
+                   long long* abytes, hipStream_t st);
 // *out = largest row length of g.
 int launch_max_deg(const DGraph& g, int32_t* out, hipStream_t st);
 // Final cut and part weights of the jobs covered by the chunk table: chunk c

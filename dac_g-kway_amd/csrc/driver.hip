@@ -429,7 +429,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     const int32_t M = prm.max_moves;
     Buf<int32_t> xlist, dthr, wpos, sscr;
     Buf<int32_t> gain, dest, upd, heavy, hcnt, wl, list, active, drounds, lastcnt, dvoff, dpw_off, dkk;
-    Buf<long long> dcap, dmoved, dpaths;
+    Buf<long long> dcap, dmoved, dpaths, dabytes;
     Buf<uint64_t> cand, movable, selbits, defbits;
     Buf<uint32_t> hkey;
     Buf<unsigned long long> work;
@@ -440,6 +440,10 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     constexpr int kGainSlots = 2048;
     GK_TRY(work.alloc(P, 2 * kGainSlots));
     gkd::fill<unsigned long long>(work.p, 0ull, 2 * kGainSlots, st);
+    // the selection kernels' algorithmic bytes, likewise per block slot
+    Buf<unsigned long long> selbytes;
+    GK_TRY(selbytes.alloc(P, gkd::kSelectByteSlots));
+    gkd::fill<unsigned long long>(selbytes.p, 0ull, gkd::kSelectByteSlots, st);
     GK_TRY(gain.alloc(P, n0));
     GK_TRY(dest.alloc(P, n0));
     GK_TRY(upd.alloc(P, n0));
@@ -463,13 +467,14 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     GK_TRY(drounds.alloc(P, nl));
     GK_TRY(lastcnt.alloc(P, 1));
     GK_TRY(dmoved.alloc(P, nl));
+    GK_TRY(dabytes.alloc(P, nl));
     GK_TRY(dcutlv.alloc(P, nl));
     GK_TRY(dvoff.alloc(P, 2));
     GK_TRY(dpw_off.alloc(P, 2));
     GK_TRY(dkk.alloc(P, 1));
     GK_TRY(dcap.alloc(P, 1));
     gkd::fill<uint64_t>(selbits.p, 0ull, nwords0 + 1, st);
-    const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p};
+    const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p, selbytes.p};
     gkd::fill<int32_t>(dvoff.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p, 0, 1, st);
     gkd::fill<int32_t>(dpw_off.p + 1, k, 1, st);
@@ -498,6 +503,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       gkd::fill<int32_t>(lastcnt.p, 1, 1, st);
       gkd::fill<int32_t>(dthr.p, 1, 1, st);
       gkd::fill<long long>(dmoved.p + l, 0, 1, st);
+      gkd::fill<long long>(dabytes.p + l, 0, 1, st);
       // The projected boundary becomes the first gain worklist.
       gkd::fill<int32_t>(hcnt.p, 0, 2, st);
       GK_STAGE(ctx, GK_ST_PROJECT, 4LL * g.n, gkd::launch_compact(g.n, upd.p, wl.p, hcnt.p, st));
@@ -512,13 +518,14 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
       const int64_t wl_items = std::min<int64_t>(g.n, int64_t(M) * (1 + 2 * avg_deg));
       auto one_round = [&]() -> int {
-        GK_STAGE(ctx, GK_ST_SELECT, g.n / 8,
+        GK_STAGE(ctx, GK_ST_SELECT, 0,
                  gkd::launch_select_list(g, nullptr, gain.p, dest.p, pw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
                                          hcnt.p, active.p, st));
         GK_STAGE(ctx, GK_ST_BALANCE, 0,
                  gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, cur_part, dest.p, gain.p, pw.p,
                                      dcut.p, drounds.p + l, dmoved.p + l, sb, xlist.p, hkey.p, upd.p, wl.p,
-                                     hcnt.p, lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr, st));
+                                     hcnt.p, lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr,
+                                     dabytes.p + l, st));
         GK_STAGE(ctx, GK_ST_GAIN, 0,
                  gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
                                      work.p, wl_items, cand.p, st));
@@ -557,20 +564,19 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
         ls.h_part = fnv_host(h);
       }
     }
-    std::vector<long long> hcutlv(nl), hmovedlv(nl);
+    std::vector<long long> hcutlv(nl), hmovedlv(nl), habytes(nl);
     std::vector<int32_t> hroundslv(nl);
     long long hpaths[3] = {0, 0, 0};
     GK_TRY(fetch(ctx, st, {{hcutlv.data(), dcutlv.p, sizeof(long long) * size_t(nl - 1)},
                            {hmovedlv.data(), dmoved.p, sizeof(long long) * size_t(nl - 1)},
+                           {habytes.data(), dabytes.p, sizeof(long long) * size_t(nl - 1)},
                            {hroundslv.data(), drounds.p, sizeof(int32_t) * size_t(nl - 1)},
                            {hpaths, dpaths.p, sizeof hpaths}}));
     for (int i = 0; i < 3; ++i) S.round_paths[i] = hpaths[i];
     for (int32_t l = 0; l + 1 < nl; ++l) {
       gk_level_stats& ls = S.levels[l];
-      const DGraph& g = L[l].g;
-      const int64_t avg_deg = g.n > 0 ? (int64_t(g.nnz) + g.n - 1) / g.n : 0;
-      // Moves: 24 bytes each in the round kernel and its apply.
-      S.stage_bytes[GK_ST_BALANCE] += 24LL * hmovedlv[l] + 8LL * hmovedlv[l] * avg_deg;
+      // The round kernel's algorithmic bytes, counted on the device.
+      S.stage_bytes[GK_ST_BALANCE] += habytes[l];
       ls.rounds = hroundslv[l];
       ls.moves = hmovedlv[l];
       ls.cut_after = hcutlv[l];
@@ -584,6 +590,9 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       we += wk[2 * i + 1];
     }
     S.stage_bytes[GK_ST_GAIN] += int64_t(28 * wv + 12 * we);
+    std::vector<unsigned long long> sbk;
+    GK_TRY(to_host(st, sbk, selbytes.p, gkd::kSelectByteSlots));
+    for (unsigned long long b : sbk) S.stage_bytes[GK_ST_SELECT] += int64_t(b);
   }
   GK_HIP(hipStreamSynchronize(st));
   const auto t3 = clk::now();
@@ -1119,7 +1128,7 @@ int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t ma
   int32_t maxdeg = 0;
   GK_TRY(fetch(ctx, st, {{&maxdeg, dmax.p, sizeof maxdeg}}));
   const int lanes = gain_lanes(g);
-  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p};
+  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p, nullptr};
   for (int32_t r = 0; r < nrounds; ++r) {
     // B = 0, pos = -1 unless the round orders any move.
     gkd::fill<int32_t>(dorder.p, 0, 1, st);
@@ -1128,7 +1137,7 @@ int gk_k_round(gk_ctx* ctx, const gk_csr* dg, int32_t k, int64_t cap, int32_t ma
                                    active.p, st));
     GK_TRY(gkd::launch_round_b(1, k, dvoff.p, M, dkk.p, dcap.p, dpw_off.p, g, part, dest, gain, pw, dcut.p,
                                drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p, lastcnt.p, active.p,
-                               cand.p, dthr.p, dpaths.p, dorder.p, st));
+                               cand.p, dthr.p, dpaths.p, dorder.p, nullptr, st));
     GK_TRY(gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, part, upd.p, bnd, gain, dest, nullptr, n,
                                 cand.p, st));
     if (maxdeg > lanes && gkd::gain_heavy_separate(k))

@@ -602,7 +602,10 @@ int Engine::refine_step(bool* worked) {
   GK_TRY(dthr.alloc(P_, J));
   gkd::fill<int32_t>(dthr.p, 1, J, st_);
   gkd::fill<uint64_t>(selbits.p, 0ull, nwords + 1, st_);
-  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p};
+  Buf<unsigned long long> selbytes;  // the selection kernels' algorithmic bytes, per block slot
+  GK_TRY(selbytes.alloc(P_, gkd::kSelectByteSlots));
+  gkd::fill<unsigned long long>(selbytes.p, 0ull, gkd::kSelectByteSlots, st_);
+  const gkd::SelectBufs sb{movable.p, selbits.p, defbits.p, wpos.p, list.p, sscr.p, selbytes.p};
   GK_TRY(hkey.alloc(P_, N));
   GK_TRY(hcnt.alloc(P_, 2));
   GK_TRY(active.alloc(P_, 1));
@@ -615,6 +618,9 @@ int Engine::refine_step(bool* worked) {
   gkd::fill<long long>(dmoved.p, 0, J, st_);
   GK_TRY(dpaths.alloc(P_, 3LL * J));
   gkd::fill<long long>(dpaths.p, 0, 3LL * J, st_);
+  Buf<long long> dab;  // the round kernel's algorithmic bytes per job
+  GK_TRY(dab.alloc(P_, J));
+  gkd::fill<long long>(dab.p, 0, J, st_);
   // Gain-pass statistics: a (vertices, entries) pair per block of the gain
   // launches (grids of at most kGainSlots blocks), summed after the rounds
   // into the stage's algorithmic bytes.
@@ -662,13 +668,13 @@ int Engine::refine_step(bool* worked) {
   const int64_t pack_items = int64_t(N) + int64_t(g.nnz);
   const int kMaxRounds = static_cast<int>(std::min<int64_t>(256, 16 * std::max<int64_t>(1, pack_items >> 24)));
   auto one_round = [&]() -> int {
-    GK_STAGE(ctx_, GK_ST_SELECT, N / 8,
+    GK_STAGE(ctx_, GK_ST_SELECT, 0,
              gkd::launch_select_list(g, rs.vjob.p, gain.p, dest.p, dpw.p, dpw_off.p, dcap.p, dthr.p, cand.p, sb,
                                      hcnt.p, active.p, st_));
-    GK_STAGE(ctx_, GK_ST_BALANCE, 8LL * J,
+    GK_STAGE(ctx_, GK_ST_BALANCE, 0,
              gkd::launch_round_b(J, kmax, F.dv_off.p, M, dkk.p, dcap.p, dpw_off.p, g, part->p, dest.p, gain.p,
                                  dpw.p, dcut.p, drounds.p, dmoved.p, sb, xlist.p, hkey.p, upd.p, wl.p, hcnt.p,
-                                 lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr, st_));
+                                 lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr, dab.p, st_));
     GK_STAGE(ctx_, GK_ST_GAIN, 0,
              gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
                                  gwork.p, wl_items, cand.p, st_));
@@ -708,7 +714,11 @@ int Engine::refine_step(bool* worked) {
   std::vector<long long> hcut(J), hmoved(J), hpaths(3 * size_t(J));
   std::vector<int32_t> hrounds(J), hlast(J);
   std::vector<unsigned long long> hwork(2 * kGainSlots);
+  std::vector<long long> hab(J);
+  std::vector<unsigned long long> hsel(gkd::kSelectByteSlots);
   GK_TRY(download({{hwork.data(), gwork.p, sizeof(unsigned long long) * hwork.size()},
+                   {hsel.data(), selbytes.p, sizeof(unsigned long long) * hsel.size()},
+                   {hab.data(), dab.p, sizeof(long long) * J},
                    {hpaths.data(), dpaths.p, sizeof(long long) * 3 * J},
                    {hcut.data(), dcut.p, sizeof(long long) * J},
                    {hpw.data(), dpw.p, sizeof(int32_t) * hpw.size()},
@@ -722,6 +732,8 @@ int Engine::refine_step(bool* worked) {
       we += hwork[2 * i + 1];
     }
     ctx_->cur->stage_bytes[GK_ST_GAIN] += int64_t(28 * wv + 12 * we);
+    for (int32_t i = 0; i < J; ++i) ctx_->cur->stage_bytes[GK_ST_BALANCE] += hab[i];
+    for (unsigned long long b : hsel) ctx_->cur->stage_bytes[GK_ST_SELECT] += int64_t(b);
   }
   std::vector<UnpackDst> dst(J);
   std::vector<int32_t> done(J, 0);
