@@ -79,15 +79,20 @@ def launch_flags(gk, args):
     return (gk.GK_FLAG_GRAPH_ROUNDS if args.graph_rounds else 0) | (gk.GK_FLAG_AUTOTUNE if args.autotune else 0)
 
 
-def dominant_stage(gk, L, ctx, stage_step, off):
-    """Runs one step with every stage timed, returns the stage with the most
-    device time and restricts the context's stage timing to it."""
+def dominant_stage(gk, L, ctx, stage_step, off, steps=2):
+    """Runs `steps` steps with every stage timed, returns the stage with the
+    most device time over them and restricts the context's stage timing to
+    it."""
     if off:
         return None
-    st = stage_step()
-    dom = max(range(len(gk.STAGE_NAMES)), key=lambda i: st.stage_ms[i])
+    tot = [0.0] * len(gk.STAGE_NAMES)
+    for _ in range(steps):
+        st = stage_step()
+        for i in range(len(gk.STAGE_NAMES)):
+            tot[i] += st.stage_ms[i]
+    dom = max(range(len(gk.STAGE_NAMES)), key=lambda i: tot[i])
     L.gk_ctx_set_stage_mask(ctx.handle, 1 << dom)
-    log(f"dominant stage: {gk.STAGE_NAMES[dom]} ({st.stage_ms[dom]:.3f} ms in a timed-stage step)")
+    log(f"dominant stage: {gk.STAGE_NAMES[dom]} ({tot[dom] / steps:.3f} ms per timed-stage step)")
     return gk.STAGE_NAMES[dom]
 
 
@@ -106,7 +111,14 @@ def roofline_of(stage, workload_tag, dom=None):
     tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_{workload_tag}.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            r["traffic"] = json.load(f).get("hbm_bytes_per_call")
+            prof = json.load(f)
+        r["traffic"] = prof.get("hbm_bytes_per_call")
+        # The stage's kernel time per call in the committed kernel trace of
+        # the workload: short launches read longer under events, which carry
+        # their dispatch.
+        if prof.get("kernel_us_per_call"):
+            r["avg_call_us_trace"] = prof["kernel_us_per_call"]
+            r["achieved_trace"] = round(r["alg_bytes_per_call"] / (prof["kernel_us_per_call"] * 1e-6) / 1e9, 1)
     return r
 
 

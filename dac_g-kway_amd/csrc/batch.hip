@@ -16,6 +16,7 @@
 
 #include "balance.hpp"
 #include "batch.hpp"
+#include "kernels.hpp"
 #include "prim.hpp"
 #include "../../include/gkway.h"
 
@@ -553,12 +554,12 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain
   const int64_t nwords = (int64_t(g.n) + 63) / 64;
   const int64_t waves = (nwords + kSelectWords - 1) / kSelectWords;
   const unsigned grid = grid_for(waves * 64, 256);
-  k_select_fit<<<grid, 256, 0, st>>>(g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable, reset, active,
-                                     sb.abytes);
-  k_select_list<<<grid, 256, 0, st>>>(g.n, g.xadj, g.adj, vjob, gain, thr, sb.movable, sb.selbits, sb.defbits,
-                                      sb.scratch, sb.abytes);
-  k_sel_scatter<<<grid_for(nwords, kScatterWords), 256, 0, st>>>(nwords, sb.selbits, sb.scratch, sb.wpos, sb.list,
-                                                                  sb.abytes);
+  gk_launch(k_select_fit, dim3(grid), dim3(256), 0, st, g.n, g.vw, vjob, dest, pw, pw_off, cap, cand, sb.movable,
+            reset, active, sb.abytes);
+  gk_launch(k_select_list, dim3(grid), dim3(256), 0, st, g.n, g.xadj, g.adj, vjob, gain, thr, sb.movable, sb.selbits,
+            sb.defbits, sb.scratch, sb.abytes);
+  gk_launch(k_sel_scatter, dim3(grid_for(nwords, kScatterWords)), dim3(256), 0, st, nwords, sb.selbits, sb.scratch,
+            sb.wpos, sb.list, sb.abytes);
   return hip_status();
 }
 
@@ -924,10 +925,9 @@ int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, co
                    long long* abytes, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
-  k_round_b<<<nj, kBalThreads, sizeof(int32_t) * kmax, st>>>(v_off, M, kk, cap, pw_off, g.xadj, g.adj, part, dest,
-                                                             gain, g.vw, pw, cut, rounds, moved, sb, xlist, hkey,
-                                                             upd, wl, wlc, lastcnt, active, cand, thr, paths,
-                                                             order, abytes);
+  gk_launch(k_round_b, dim3(nj), dim3(kBalThreads), uint32_t(sizeof(int32_t) * kmax), st, v_off, M, kk, cap, pw_off,
+            g.xadj, g.adj, part, dest, gain, g.vw, pw, cut, rounds, moved, sb, xlist, hkey, upd, wl, wlc, lastcnt,
+            active, cand, thr, paths, order, abytes);
   return hip_status();
 }
 

@@ -56,13 +56,13 @@ template <int B>
 static void match_block(const DGraph& g, int lanes, int32_t* cand, hipStream_t st) {
   const unsigned grid = grid_for(int64_t(g.n) * lanes, B);
   switch (lanes) {
-    case 1: k_match<1, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 2: k_match<2, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 4: k_match<4, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 8: k_match<8, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 16: k_match<16, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    case 32: k_match<32, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
-    default: k_match<64, B><<<grid, B, 0, st>>>(g.n, g.xadj, g.adj, g.w, cand); break;
+    case 1: gk_launch(k_match<1, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
+    case 2: gk_launch(k_match<2, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
+    case 4: gk_launch(k_match<4, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
+    case 8: gk_launch(k_match<8, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
+    case 16: gk_launch(k_match<16, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
+    case 32: gk_launch(k_match<32, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
+    default: gk_launch(k_match<64, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
   }
 }
 
@@ -404,9 +404,9 @@ static void contract_rows_block(const DGraph& g, int lanes, const int32_t* rows,
                                 const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len, hipStream_t st) {
   const unsigned grid = grid_for(int64_t(nrows) * lanes, B);
   switch (lanes) {
-    case 16: k_contract_rows<16, B><<<grid, B, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
-    case 32: k_contract_rows<32, B><<<grid, B, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
-    default: k_contract_rows<64, B><<<grid, B, 0, st>>>(rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    case 16: gk_launch(k_contract_rows<16, B>, dim3(grid), dim3(B), 0, st, rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    case 32: gk_launch(k_contract_rows<32, B>, dim3(grid), dim3(B), 0, st, rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    default: gk_launch(k_contract_rows<64, B>, dim3(grid), dim3(B), 0, st, rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
   }
 }
 

@@ -33,6 +33,10 @@ using gkd::DGraph;
 
 using namespace gkrt;
 
+namespace gkd {
+bool kernel_events(hipEvent_t* start, hipEvent_t* stop) { return gkrt::StageClock::kernel_pair(start, stop); }
+}  // namespace gkd
+
 namespace gkrt {
 
 // Smallest launches worth timing candidates for: matching over at least
@@ -141,28 +145,33 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
   GK_TRY(tw.alloc(P, g.nnz));
   GK_TRY(len.alloc(P, nc));
   const int lanes[3] = {16, 32, 64};
-  ctx->clock.begin(GK_ST_CONTRACT);
-  for (int cls = 0; cls < 3; ++cls) {
-    const int32_t* rows = lists.p + int64_t(cls) * nc;
-    auto run = [&](const Tuner::Shape& x) {
-      return gkd::launch_contract_rows(g, x.lanes, x.block, rows, hc[cls], cptr, order, cmap, toff.p, tadj.p,
-                                       tw.p, len.p, st);
-    };
-    Tuner::Shape shape{lanes[cls], 256};
-    if (ctx->autotune && hc[cls] >= kTuneMinRows) {
-      // Block size per row class; the lanes of a class are fixed by its rows.
-      const uint64_t key = Tuner::key(1 + cls, g.n, g.nnz);
-      if (const Tuner::Shape* s = ctx->tuner.find(key)) {
-        shape = *s;
-      } else {
-        const int L = lanes[cls];
-        GK_TRY(ctx->tuner.tune(key, {{L, 256}, {L, 64}, {L, 128}, {L, 512}}, st, run, &shape));
-        continue;  // every candidate wrote the rows
+  auto row_classes = [&]() -> int {
+    for (int cls = 0; cls < 3; ++cls) {
+      const int32_t* rows = lists.p + int64_t(cls) * nc;
+      auto run = [&](const Tuner::Shape& x) {
+        return gkd::launch_contract_rows(g, x.lanes, x.block, rows, hc[cls], cptr, order, cmap, toff.p, tadj.p,
+                                         tw.p, len.p, st);
+      };
+      Tuner::Shape shape{lanes[cls], 256};
+      if (ctx->autotune && hc[cls] >= kTuneMinRows) {
+        // Block size per row class; the lanes of a class are fixed by its rows.
+        const uint64_t key = Tuner::key(1 + cls, g.n, g.nnz);
+        if (const Tuner::Shape* s = ctx->tuner.find(key)) {
+          shape = *s;
+        } else {
+          const int L = lanes[cls];
+          GK_TRY(ctx->tuner.tune(key, {{L, 256}, {L, 64}, {L, 128}, {L, 512}}, st, run, &shape));
+          continue;  // every candidate wrote the rows
+        }
       }
+      GK_TRY(run(shape));
     }
-    GK_TRY(run(shape));
-  }
+    return GK_OK;
+  };
+  ctx->clock.begin(GK_ST_CONTRACT);
+  const int rc_rows = row_classes();
   ctx->clock.end(GK_ST_CONTRACT);
+  if (rc_rows != GK_OK) return rc_rows;
   count_stage(ctx, GK_ST_CONTRACT, 0);  // bytes added once the coarse entry count is known
   if (hc[3] > 0) {
     const int32_t nbig = hc[3];

@@ -2,10 +2,30 @@
 // the host driver.  All take device pointers and a stream; all return a gk
 // status code (0 = launched).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
 namespace gkd {
+
+// Kernel-level stage timing.  While the host times a stage kernel by kernel
+// (the refinement round and the coarsening stages that can dominate a run),
+// kernel_events hands out a start/stop event pair for each launch made
+// through gk_launch, which records them at the kernel's own start and end
+// (hipExtLaunchKernelGGL), so the stage's time is the sum of its kernels'
+// durations as a kernel trace reports them, without event packets between
+// the launches.  Otherwise gk_launch is a plain launch.  Defined in
+// driver.hip.
+bool kernel_events(hipEvent_t* start, hipEvent_t* stop);
+
+template <typename... KArgs, typename... Args>
+inline void gk_launch(void (*k)(KArgs...), dim3 grid, dim3 block, uint32_t shm, hipStream_t st, Args... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (kernel_events(&e0, &e1))
+    hipExtLaunchKernelGGL(k, grid, block, shm, st, e0, e1, 0u, args...);
+  else
+    hipLaunchKernelGGL(k, grid, block, shm, st, args...);
+}
 
 struct DGraph {
   int32_t n = 0;

@@ -323,7 +323,8 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
   if (grid > 2048) grid = 2048;
 #define GK_GAIN_CASE(S)                                                                            \
   case S:                                                                                           \
-    k_gain_wl<S><<<grid, 256, 0, st>>>(wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy, work, cand); \
+    gk_launch(k_gain_wl<S>, dim3(grid), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, \
+              heavy, work, cand);                                                                 \
     break;
   switch (lanes) {
     GK_GAIN_CASE(4)
@@ -331,8 +332,8 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
     GK_GAIN_CASE(16)
     GK_GAIN_CASE(32)
     default:
-      k_gain_wl<64><<<grid, 256, 0, st>>>(wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, heavy,
-                                          work, cand);
+      gk_launch(k_gain_wl<64>, dim3(grid), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain,
+                dest, heavy, work, cand);
       break;
   }
 #undef GK_GAIN_CASE
@@ -346,8 +347,8 @@ int launch_gain_heavy(const DGraph& g, int32_t k, const int32_t* heavy, const in
   // Grid-stride over the heavy list; more waves when k (the LDS histogram)
   // is small.
   const unsigned grid = k <= 1024 ? 2048 : 256;
-  k_gain_heavy<<<grid, 64, sizeof(int32_t) * k, st>>>(heavy, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain,
-                                                      dest, work, cand);
+  gk_launch(k_gain_heavy, dim3(grid), dim3(64), uint32_t(sizeof(int32_t) * k), st, heavy, wlc, k, g.xadj, g.adj, g.w,
+            part, upd, bnd, gain, dest, work, cand);
   return hip_status();
 }
 

@@ -151,21 +151,49 @@ class StageClock {
   void start(bool on, hipStream_t st) {
     on_ = on;
     st_ = st;
+    kclock() = nullptr;
     for (auto& v : recs_) v.clear();
     used_ = 0;
   }
   void set_mask(uint32_t m) { mask_ = m; }
   bool timed(int id) const { return on_ && ((mask_ >> id) & 1u); }
+  // Stages whose launches go through gkd::gk_launch are timed kernel by
+  // kernel (their time = the sum of their kernels' durations); the others
+  // are bracketed by a pair of events on the stream.
+  static bool per_kernel(int id) {
+    return id == GK_ST_MATCH || id == GK_ST_CONTRACT || id == GK_ST_GAIN || id == GK_ST_GAIN_HEAVY ||
+           id == GK_ST_SELECT || id == GK_ST_BALANCE;
+  }
   void begin(int id) {
     if (!on_ || !((mask_ >> id) & 1u)) return;
+    if (per_kernel(id)) {
+      kclock() = this;
+      kstage() = id;
+      return;
+    }
     cur_[id] = take();
     (void)hipEventRecord(cur_[id], st_);
   }
   void end(int id) {
     if (!on_ || !((mask_ >> id) & 1u)) return;
+    if (per_kernel(id)) {
+      kclock() = nullptr;
+      return;
+    }
     hipEvent_t e = take();
     (void)hipEventRecord(e, st_);
     recs_[id].emplace_back(cur_[id], e);
+  }
+  // An event pair for one launch of the stage being timed kernel by kernel
+  // on this thread; false when there is none.
+  static bool kernel_pair(hipEvent_t* a, hipEvent_t* b) {
+    StageClock* c = kclock();
+    if (!c) return false;
+    *a = c->take();
+    *b = c->take();
+    if (!*a || !*b) return false;
+    c->recs_[kstage()].emplace_back(*a, *b);
+    return true;
   }
   int collect(gk_stats& S) {
     if (!on_) return GK_OK;
@@ -186,10 +214,21 @@ class StageClock {
   hipEvent_t take() {
     if (used_ == pool_.size()) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      // Timing-only events: no system-scope fence (cache writeback and
+      // invalidation) at each record, which would stretch the short
+      // launches they bracket; the times are read after a stream sync.
+      if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
       pool_.push_back(e);
     }
     return pool_[used_++];
+  }
+  static StageClock*& kclock() {
+    thread_local StageClock* c = nullptr;
+    return c;
+  }
+  static int& kstage() {
+    thread_local int id = 0;
+    return id;
   }
   bool on_ = false;
   uint32_t mask_ = ~0u;
@@ -263,7 +302,7 @@ class Tuner {
     }
     while (ev_.size() < 4 * m) {
       hipEvent_t e;
-      GK_HIP(hipEventCreate(&e));
+      GK_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       ev_.push_back(e);
     }
     // Warm the device first (clock ramp after idle, caches, address
