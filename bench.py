@@ -13,7 +13,8 @@ packed CSR to device-resident partitions.  Beside it, on the same graphs:
   * bit-exactness: batched == plain on every graph, == the CPU oracle on a
     bounded sample (which is also the CPU baseline, with the host named);
   * nested, the plain path on the 2-D Delaunay triangulation of 2^22 points
-    (k = 64), with its host-to-host window and an oracle check.
+    (k = 64), with its host-to-host window and an oracle check, once with
+    the points numbered in random order and once along a Hilbert curve.
 `value` is 0 unless every result is bit-exact, deterministic and within the
 3% cap.
 
@@ -366,6 +367,12 @@ def run_batch(args, gk, torch, nested_c2=True):
         c2.workload, c2.n, c2.k, c2.seed = "delaunay", 1 << 22, 64, 1
         c2.steps, c2.warmup, c2.stage_steps = args.c2_steps, 2, 1
         out["plain_config2"] = run_plain(c2, gk, torch, emit=False)
+        # The same mesh with its vertices numbered along a Hilbert curve:
+        # neighbours get nearby ids, as in the DIMACS10 Delaunay inputs
+        # (delaunay_n11: median |u - v| over its edges is 20 of 2048, against
+        # about n / 3 for random numbering).
+        c2.workload = "delaunay_hilbert"
+        out["plain_config2_hilbert"] = run_plain(c2, gk, torch, emit=False)
     print(json.dumps(out), flush=True)
 
 
@@ -665,7 +672,7 @@ def main():
     ap.add_argument("--graphs", type=int, default=4096, help="batch form: graphs in the pack")
     ap.add_argument("--plain-reps", type=int, default=3, help="batch form: timed plain-path calls per graph")
     ap.add_argument("--plain-sample", type=int, default=256, help="batch form: graphs offered to the oracle")
-    ap.add_argument("--no-c2", action="store_true", help="batch form: skip the nested 2^22-point Delaunay run")
+    ap.add_argument("--no-c2", action="store_true", help="batch form: skip the nested 2^22-point Delaunay runs")
     ap.add_argument("--c2-steps", type=int, default=10, help="batch form: timed steps of the nested plain run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="batch/serve forms: seconds of oracle work")
     ap.add_argument("--rate", type=float, default=200.0, help="serve form: Poisson arrival rate (requests/s)")

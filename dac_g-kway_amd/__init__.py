@@ -206,6 +206,7 @@ def lib():
                                          POINTER(I32P), POINTER(Stats)]
         L.gk_serve_open.argtypes = [VP, POINTER(ServeConfig), POINTER(c_void_p)]
         L.gk_serve_submit.argtypes = [VP, POINTER(Csr), POINTER(Params), POINTER(c_int64)]
+        L.gk_serve_submit_many.argtypes = [VP, c_int32, POINTER(Csr), POINTER(Params), POINTER(c_int64)]
         L.gk_serve_poll.argtypes = [VP, c_int64, I32P, POINTER(Stats)]
         L.gk_serve_wait.argtypes = [VP, c_int64, I32P, POINTER(Stats)]
         L.gk_serve_query.argtypes = [VP, POINTER(ServeStats)]
@@ -444,6 +445,20 @@ class Server:
                                f"{lib().gk_last_error().decode()}")
         self._n[rid.value] = g.n
         return rid.value
+
+    def submit_many(self, graphs, ks, **kw) -> list:
+        """Queues the graphs (graph i at k = ks[i]) together; their ids."""
+        cnt = len(graphs)
+        csrs = (Csr * cnt)(*[g.csr() for g in graphs])
+        ps = (Params * cnt)(*[params(k, **kw) for k in ks])
+        ids = (c_int64 * cnt)()
+        rc = lib().gk_serve_submit_many(self._h, cnt, csrs, ps, ids)
+        if rc != GK_OK:
+            raise RuntimeError(f"gk_serve_submit_many: {lib().gk_strerror(rc).decode()}: "
+                               f"{lib().gk_last_error().decode()}")
+        for g, rid in zip(graphs, ids):
+            self._n[rid] = g.n
+        return list(ids)
 
     def poll(self, rid):
         """None while pending, else (part, Stats)."""

@@ -250,19 +250,28 @@ int launch_cvwgt(int32_t nc, const int32_t* cptr, const int32_t* order, const in
 
 // --------------------------------------------------------------- contraction
 // Per coarse row: upper bound of entries (sum of member degrees) and its
-// lane class; row ids appended to one list per class.
+// lane class; row ids appended to one list per class, each with its member
+// range in order[] (desc: first member, member count).  Member m's row start
+// and length are copied to mrow[m] on the way, so that the row kernels reach
+// a member's entries with one load instead of order[] then xadj[].
 __global__ void __launch_bounds__(256) k_row_bounds(int32_t nc, const int32_t* __restrict__ cptr,
                                                     const int32_t* __restrict__ order,
                                                     const int32_t* __restrict__ xadj,
                                                     int32_t* __restrict__ ub, int32_t* __restrict__ lists,
+                                                    int2* __restrict__ desc, int2* __restrict__ mrow,
                                                     int32_t* __restrict__ cnt) {
   const int64_t c = int64_t(blockIdx.x) * 256 + threadIdx.x;
   int cls = -1;
+  int32_t m0 = 0, m1 = 0;
   if (c < nc) {
+    m0 = cptr[c];
+    m1 = cptr[c + 1];
     int32_t s = 0;
-    for (int32_t j = cptr[c]; j < cptr[c + 1]; ++j) {
+    for (int32_t j = m0; j < m1; ++j) {
       const int32_t v = order[j];
-      s += xadj[v + 1] - xadj[v];
+      const int32_t a = xadj[v], d = xadj[v + 1] - a;
+      mrow[j] = make_int2(a, d);
+      s += d;
     }
     ub[c] = s;
     cls = s <= 16 ? 0 : (s <= 32 ? 1 : (s <= 64 ? 2 : 3));
@@ -290,7 +299,9 @@ __global__ void __launch_bounds__(256) k_row_bounds(int32_t nc, const int32_t* _
     int off = bbase[cls];
     for (int w = 0; w < wave; ++w) off += wcnt[cls][w];
     const uint64_t mine = cls == 0 ? m[0] : (cls == 1 ? m[1] : (cls == 2 ? m[2] : m[3]));
-    lists[int64_t(cls) * nc + off + popc64(mine & lt)] = int32_t(c);
+    const int64_t slot = int64_t(cls) * nc + off + popc64(mine & lt);
+    lists[slot] = int32_t(c);
+    desc[slot] = make_int2(m0, m1 - m0);
   }
 }
 
@@ -301,23 +312,24 @@ struct UbLoad {
 };
 
 int launch_contract_bounds(const DGraph& g, int32_t nc, const int32_t* cptr, const int32_t* order,
-                           int32_t* ub, int32_t* toff, int32_t* lists, int32_t* list_cnt,
-                           int32_t* scratch, hipStream_t st) {
+                           int32_t* ub, int32_t* toff, int32_t* lists, int2* desc, int2* mrow,
+                           int32_t* list_cnt, int32_t* scratch, hipStream_t st) {
   fill<int32_t>(list_cnt, 0, 4, st);
   if (nc <= 0) return 0;
-  k_row_bounds<<<grid_for(nc, 256), 256, 0, st>>>(nc, cptr, order, g.xadj, ub, lists, list_cnt);
+  k_row_bounds<<<grid_for(nc, 256), 256, 0, st>>>(nc, cptr, order, g.xadj, ub, lists, desc, mrow, list_cnt);
   int rc = scan(int64_t(nc) + 1, UbLoad{ub, nc}, StoreExcl{toff}, nullptr, scratch, st);
   return rc ? GK_ERR_HIP : hip_status();
 }
 
-// Rows whose upper bound fits S lanes: one entry per lane, bitonic sort by
-// coarse neighbour id across the S lanes, segmented sum of equal neighbours,
-// self entries dropped, run tails compacted with a ballot.
+// Rows whose upper bound fits S lanes, any coarse id width: one entry per
+// lane, bitonic sort of (coarse neighbour, weight) pairs across the S lanes
+// by shuffles, segmented sum of equal neighbours, self entries dropped, run
+// tails compacted with a ballot.  Used when the coarse ids do not leave room
+// for the lane number beside them in 32 bits (k_contract_rows).
 template <int S, int B>
-__global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__ rows, int32_t nrows,
-                                                       const int32_t* __restrict__ cptr,
-                                                       const int32_t* __restrict__ order,
-                                                       const int32_t* __restrict__ xadj,
+__global__ void __launch_bounds__(B) k_contract_rows_wide(const int32_t* __restrict__ rows,
+                                                       const int2* __restrict__ desc, int32_t nrows,
+                                                       const int2* __restrict__ mrow,
                                                        const int32_t* __restrict__ adj,
                                                        const int32_t* __restrict__ w,
                                                        const int32_t* __restrict__ cmap,
@@ -329,18 +341,19 @@ __global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__
   const int sub = static_cast<int>(gt % S);
   if (ri >= nrows) return;
   const int32_t c = rows[ri];
+  const int2 d = desc[ri];
+  const int32_t m0 = d.x, nm = d.y;
   // Locate this lane's entry in the concatenation of the member rows: lane m
   // loads member m's row start and length, a shuffle scan gives the offsets,
-  // and each lane finds its member among them (two dependent loads instead of
-  // two per member).  More members than lanes (max_group > S): member by member.
+  // and each lane finds its member among them.  More members than lanes
+  // (max_group > S): member by member.
   int32_t j = -1;
-  const int32_t m0 = cptr[c], nm = cptr[c + 1] - m0;
   if (nm <= S) {
     int32_t ms = 0, md = 0;
     if (sub < nm) {
-      const int32_t v = order[m0 + sub];
-      ms = xadj[v];
-      md = xadj[v + 1] - ms;
+      const int2 r = mrow[m0 + sub];
+      ms = r.x;
+      md = r.y;
     }
     int32_t inc = md;
 #pragma unroll
@@ -355,11 +368,9 @@ __global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__
   } else {
     int32_t pre = 0;
     for (int32_t m = m0; m < m0 + nm; ++m) {
-      const int32_t v = order[m];
-      const int32_t s = xadj[v];
-      const int32_t d = xadj[v + 1] - s;
-      if (sub >= pre && sub < pre + d) j = s + (sub - pre);
-      pre += d;
+      const int2 r = mrow[m];
+      if (sub >= pre && sub < pre + r.y) j = r.x + (sub - pre);
+      pre += r.y;
     }
   }
   int32_t key = INT32_MAX, val = 0;
@@ -398,29 +409,149 @@ __global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__
   if (sub == 0) len[c] = popc64(seg);
 }
 
-template <int B>
-static void contract_rows_block(const DGraph& g, int lanes, const int32_t* rows, int32_t nrows,
-                                const int32_t* cptr, const int32_t* order, const int32_t* cmap,
-                                const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len, hipStream_t st) {
+// Rows whose upper bound fits S lanes (S = 16, 32, 64; B / S rows per
+// block), coarse ids below 2^(32 - log2 S) - 1.  Per row:
+//   1. lane m < nm loads member m's row bounds (mrow); an add-scan of the
+//      lengths gives each member's first lane, which the member marks in a
+//      per-wave LDS slot array; lane i takes the entry of the last member
+//      marked at or below it (a ballot of the marks);
+//   2. its entry (coarse neighbour, weight); self entries become the
+//      sentinel 0xffffffff, others the key neighbour << log2 S | lane;
+//   3. an in-register bitonic sort of the keys (DPP partners), the weight
+//      follows its key by one ds_bpermute;
+//   4. runs of equal neighbours: a segmented add-scan minus the exclusive
+//      sum at the run's head gives the run total at its tail; tails are
+//      compacted with a ballot.
+// The dependent loads per entry: the row's list slot (id and member range),
+// the members' row bounds, the entry, its coarse id.
+template <int S, int B>
+__global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__ rows,
+                                                       const int2* __restrict__ desc, int32_t nrows,
+                                                       const int2* __restrict__ mrow,
+                                                       const int32_t* __restrict__ adj,
+                                                       const int32_t* __restrict__ w,
+                                                       const int32_t* __restrict__ cmap,
+                                                       const int32_t* __restrict__ toff,
+                                                       int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
+                                                       int32_t* __restrict__ len) {
+  static_assert(S == 16 || S == 32 || S == 64, "segment of 16, 32 or 64 lanes");
+  static_assert(B % 64 == 0, "whole waves");
+  constexpr int LB = S == 16 ? 4 : (S == 32 ? 5 : 6);
+  constexpr uint32_t kNone = 0xffffffffu;
+  __shared__ int32_t mark[B];
+  const int64_t ri = (int64_t(blockIdx.x) * B + threadIdx.x) >> LB;
+  const int lane = lane_id();
+  const int sub = lane & (S - 1);
+  const int base = lane - sub;
+  // Segments past the last row stay active (masked stores) so that every
+  // cross-lane step runs on whole segments.
+  const bool live = ri < nrows;
+  int32_t c = -1, m0 = 0, nm = 0, to = 0;
+  if (live) {
+    c = rows[ri];
+    const int2 d = desc[ri];
+    m0 = d.x;
+    nm = d.y;
+    to = toff[c];
+  }
+  int32_t j = -1;
+  if (nm <= S) {
+    int32_t ms = 0, md = 0;
+    if (sub < nm) {
+      const int2 r = mrow[m0 + sub];
+      ms = r.x;
+      md = r.y;
+    }
+    const int32_t inc = seg_incl_scan<S>(md);
+    const int32_t e = inc - md;
+    const int32_t tot = __shfl(inc, base + S - 1, 64);
+    int32_t* wm = mark + (threadIdx.x - lane);  // this wave's 64 slots
+    wm[lane] = INT32_MIN;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (md > 0) wm[base + e] = ms - e;  // distinct lanes: members with entries start apart
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t starts = ballot(wm[lane] != INT32_MIN);
+    if (sub < tot) j = wm[last_set_at_or_below(starts, lane)] + sub;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    // More members than lanes (members without entries): member by member.
+    int32_t pre = 0;
+    for (int32_t m = m0; m < m0 + nm; ++m) {
+      const int2 r = mrow[m];
+      if (sub >= pre && sub < pre + r.y) j = r.x + (sub - pre);
+      pre += r.y;
+    }
+  }
+  uint32_t key = kNone;
+  int32_t val = 0;
+  if (j >= 0) {
+    const int32_t q = cmap[adj[j]];
+    val = w[j];
+    if (q != c) key = (uint32_t(q) << LB) | uint32_t(sub);
+  }
+  key = seg_sort_u32<S>(key);
+  const bool valid = key != kNone;
+  const int32_t sv = __shfl(val, base + int(key & (S - 1)), 64);
+  const uint32_t nb = key >> LB;
+  const uint32_t prev = dpp_mov<dpp::kWaveShr1>(key), next = dpp_mov<dpp::kWaveShl1>(key);
+  const bool head = valid && (sub == 0 || (prev >> LB) != nb);
+  const bool tail = valid && (sub == S - 1 || (next >> LB) != nb);
+  const int32_t x = valid ? sv : 0;
+  const int32_t p = seg_incl_scan<S>(x);
+  const int h = last_set_at_or_below(ballot(head), lane);
+  const int32_t eh = __shfl(p - x, h < 0 ? lane : h, 64);
+  const uint64_t tails = ballot(tail);
+  const uint64_t segm = S == 64 ? ~0ull : (((1ull << S) - 1) << base);
+  if (live && tail) {
+    const int32_t o = to + popc64(tails & segm & lanemask_lt());
+    tadj[o] = int32_t(nb);
+    tw[o] = p - eh;
+  }
+  if (live && sub == 0) len[c] = popc64(tails & segm);
+}
+
+// Largest coarse id count for k_contract_rows<64, B> (ids below 2^26 - 1);
+// the 16- and 32-lane kernels allow more, one limit keeps the choice simple.
+constexpr int32_t kNarrowIdsMax = (1 << 26) - 1;
+
+template <int B, bool Wide>
+static void contract_rows_block(const DGraph& g, int lanes, const int32_t* rows, const int2* desc, int32_t nrows,
+                                const int2* mrow, const int32_t* cmap, const int32_t* toff, int32_t* tadj,
+                                int32_t* tw, int32_t* len, hipStream_t st) {
   const unsigned grid = grid_for(int64_t(nrows) * lanes, B);
+  auto go = [&](auto kern) {
+    gk_launch(kern, dim3(grid), dim3(B), 0, st, rows, desc, nrows, mrow, g.adj, g.w, cmap, toff, tadj, tw, len);
+  };
   switch (lanes) {
-    case 16: gk_launch(k_contract_rows<16, B>, dim3(grid), dim3(B), 0, st, rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
-    case 32: gk_launch(k_contract_rows<32, B>, dim3(grid), dim3(B), 0, st, rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
-    default: gk_launch(k_contract_rows<64, B>, dim3(grid), dim3(B), 0, st, rows, nrows, cptr, order, g.xadj, g.adj, g.w, cmap, toff, tadj, tw, len); break;
+    case 16: Wide ? go(k_contract_rows_wide<16, B>) : go(k_contract_rows<16, B>); break;
+    case 32: Wide ? go(k_contract_rows_wide<32, B>) : go(k_contract_rows<32, B>); break;
+    default: Wide ? go(k_contract_rows_wide<64, B>) : go(k_contract_rows<64, B>); break;
   }
 }
 
-int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, int32_t nrows,
-                         const int32_t* cptr, const int32_t* order, const int32_t* cmap,
-                         const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len,
-                         hipStream_t st) {
-  if (nrows <= 0) return 0;
+template <bool Wide>
+static void contract_rows_any(const DGraph& g, int lanes, int block, const int32_t* rows, const int2* desc,
+                              int32_t nrows, const int2* mrow, const int32_t* cmap, const int32_t* toff,
+                              int32_t* tadj, int32_t* tw, int32_t* len, hipStream_t st) {
   switch (block) {
-    case 64: contract_rows_block<64>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
-    case 128: contract_rows_block<128>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
-    case 512: contract_rows_block<512>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
-    default: contract_rows_block<256>(g, lanes, rows, nrows, cptr, order, cmap, toff, tadj, tw, len, st); break;
+    case 64: contract_rows_block<64, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
+    case 128: contract_rows_block<128, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
+    case 512: contract_rows_block<512, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
+    default: contract_rows_block<256, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
   }
+}
+
+int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, const int2* desc,
+                         int32_t nrows, int32_t nc, const int2* mrow, const int32_t* cmap, const int32_t* toff,
+                         int32_t* tadj, int32_t* tw, int32_t* len, bool force_wide, hipStream_t st) {
+  if (nrows <= 0) return 0;
+  if (force_wide || nc > kNarrowIdsMax)
+    contract_rows_any<true>(g, lanes, block, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st);
+  else
+    contract_rows_any<false>(g, lanes, block, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st);
   return hip_status();
 }
 

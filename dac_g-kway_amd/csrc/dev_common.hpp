@@ -50,6 +50,96 @@ __device__ __forceinline__ int wave_max(int x) {
   return x;
 }
 
+// ---------------------------------------------------------------- DPP moves
+// Lane permutations and shifts as DPP source modifiers (gfx9 encoding): a
+// VALU operation instead of an LDS-pipe ds_bpermute.  A "row" is 16 lanes.
+namespace dpp {
+constexpr int kQuadXor1 = 0xB1;       // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kQuadXor2 = 0x4E;       // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int kQuadXor3 = 0x1B;       // quad_perm [3,2,1,0]: lane ^ 3
+constexpr int kRowShr = 0x110;        // + n (1..15): lane - n within the row, zero below
+constexpr int kRowRor8 = 0x128;       // rotate by 8 within the row: lane ^ 8
+constexpr int kWaveShl1 = 0x130;      // lane + 1 (lane 63: zero)
+constexpr int kWaveShr1 = 0x138;      // lane - 1 (lane 0: zero)
+constexpr int kRowMirror = 0x140;     // lane ^ 15
+constexpr int kRowHalfMirror = 0x141; // lane ^ 7
+constexpr int kRowBcast15 = 0x142;    // lane 15 of each row to the next row
+constexpr int kRowBcast31 = 0x143;    // lane 31 to rows 2 and 3
+}  // namespace dpp
+
+// x from the lane that DPP control Ctrl names; lanes of rows outside
+// RowMask, and lanes whose source is outside the row or wave, get 0.
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t x) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), Ctrl, RowMask, 0xF, true));
+}
+
+// x from lane ^ M inside aligned groups of 32 lanes (ds_swizzle, bit-mask
+// mode: and-mask 31, or-mask 0, xor-mask M).
+template <int M>
+__device__ __forceinline__ uint32_t swz_xor(uint32_t x) {
+  return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(x), 0x1F | (M << 10)));
+}
+
+// Inclusive add-scan inside aligned segments of S lanes (S = 16, 32, 64);
+// every lane of a segment must be active.
+template <int S>
+__device__ __forceinline__ int32_t seg_incl_scan(int32_t x) {
+  x += int32_t(dpp_mov<dpp::kRowShr + 1>(uint32_t(x)));
+  x += int32_t(dpp_mov<dpp::kRowShr + 2>(uint32_t(x)));
+  x += int32_t(dpp_mov<dpp::kRowShr + 4>(uint32_t(x)));
+  x += int32_t(dpp_mov<dpp::kRowShr + 8>(uint32_t(x)));
+  if (S >= 32) x += int32_t(dpp_mov<dpp::kRowBcast15, 0xA>(uint32_t(x)));
+  if (S >= 64) x += int32_t(dpp_mov<dpp::kRowBcast31, 0xC>(uint32_t(x)));
+  return x;
+}
+
+// Ascending sort of one uint32 per lane inside aligned segments of S lanes
+// (S = 16, 32, 64; every lane of a segment active): Batcher's bitonic network
+// with the mirror form of each merge (compare lane i with its mirror in the
+// merged block, then half-cleaners on lane ^ j), so every compare keeps the
+// minimum in the lower lane.  Partners inside a row come from DPP, lane ^ 16
+// and ^ 31 from ds_swizzle, lane ^ 63 from ds_bpermute.
+template <int S>
+__device__ __forceinline__ uint32_t seg_sort_u32(uint32_t x) {
+  const int l = lane_id();
+  const bool lo1 = !(l & 1), lo2 = !(l & 2), lo4 = !(l & 4), lo8 = !(l & 8), lo16 = !(l & 16), lo32 = !(l & 32);
+  auto cx = [&](uint32_t y, bool lower) { x = lower ? min(x, y) : max(x, y); };
+  auto xor4 = [&]() { return dpp_mov<dpp::kQuadXor3>(dpp_mov<dpp::kRowHalfMirror>(x)); };
+  cx(dpp_mov<dpp::kQuadXor1>(x), lo1);  // blocks of 2
+  cx(dpp_mov<dpp::kQuadXor3>(x), lo2);  // blocks of 4
+  cx(dpp_mov<dpp::kQuadXor1>(x), lo1);
+  cx(dpp_mov<dpp::kRowHalfMirror>(x), lo4);  // blocks of 8
+  cx(dpp_mov<dpp::kQuadXor2>(x), lo2);
+  cx(dpp_mov<dpp::kQuadXor1>(x), lo1);
+  cx(dpp_mov<dpp::kRowMirror>(x), lo8);  // blocks of 16
+  cx(xor4(), lo4);
+  cx(dpp_mov<dpp::kQuadXor2>(x), lo2);
+  cx(dpp_mov<dpp::kQuadXor1>(x), lo1);
+  if (S >= 32) {
+    cx(swz_xor<31>(x), lo16);  // blocks of 32
+    cx(dpp_mov<dpp::kRowRor8>(x), lo8);
+    cx(xor4(), lo4);
+    cx(dpp_mov<dpp::kQuadXor2>(x), lo2);
+    cx(dpp_mov<dpp::kQuadXor1>(x), lo1);
+  }
+  if (S >= 64) {
+    cx(static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), 63, 64)), lo32);  // the whole wave
+    cx(swz_xor<16>(x), lo16);
+    cx(dpp_mov<dpp::kRowRor8>(x), lo8);
+    cx(xor4(), lo4);
+    cx(dpp_mov<dpp::kQuadXor2>(x), lo2);
+    cx(dpp_mov<dpp::kQuadXor1>(x), lo1);
+  }
+  return x;
+}
+
+// Highest lane at or below `lane` whose bit is set in m (-1 if none).
+__device__ __forceinline__ int last_set_at_or_below(uint64_t m, int lane) {
+  const uint64_t v = m & (~0ull >> (63 - lane));
+  return v ? 63 - __clzll(static_cast<long long>(v)) : -1;
+}
+
 // Fill kernel used instead of hipMemsetAsync, so the same code path can be
 // captured into a HIP graph.
 template <typename T>

@@ -13,6 +13,7 @@
 // refinement round to read the counts that size the next launches.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -132,25 +133,33 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
                    const int32_t* cptr, const int32_t* order, Level& c) {
   Pool& P = ctx->pool;
   Buf<int32_t> ub, toff, lists, lcnt, scratch, tadj, tw, len, cnnzd;
+  Buf<int2> desc, mrow;
   GK_TRY(ub.alloc(P, nc));
   GK_TRY(toff.alloc(P, int64_t(nc) + 1));
   GK_TRY(lists.alloc(P, int64_t(4) * nc));
+  GK_TRY(desc.alloc(P, int64_t(4) * nc));
+  GK_TRY(mrow.alloc(P, g.n));
   GK_TRY(lcnt.alloc(P, 4));
   GK_TRY(scratch.alloc(P, gkd::scan_scratch_ints(int64_t(nc) + 1) + 64));
-  GK_STAGE(ctx, GK_ST_CONTRACT_AUX, 12LL * g.n + 28LL * nc,
-           gkd::launch_contract_bounds(g, nc, cptr, order, ub.p, toff.p, lists.p, lcnt.p, scratch.p, st));
+  GK_STAGE(ctx, GK_ST_CONTRACT_AUX, 20LL * g.n + 36LL * nc,
+           gkd::launch_contract_bounds(g, nc, cptr, order, ub.p, toff.p, lists.p, desc.p, mrow.p, lcnt.p,
+                                       scratch.p, st));
   int32_t hc[4];
   GK_TRY(fetch(ctx, st, {{hc, lcnt.p, sizeof hc}}));
   GK_TRY(tadj.alloc(P, g.nnz));
   GK_TRY(tw.alloc(P, g.nnz));
   GK_TRY(len.alloc(P, nc));
   const int lanes[3] = {16, 32, 64};
+  // GK_CONTRACT_WIDE=1 runs the row kernels that sort (id, weight) pairs,
+  // which otherwise serve only coarse graphs of 2^26 - 1 vertices or more.
+  const char* wide_env = std::getenv("GK_CONTRACT_WIDE");
+  const bool wide = wide_env && wide_env[0] == '1';
   auto row_classes = [&]() -> int {
     for (int cls = 0; cls < 3; ++cls) {
       const int32_t* rows = lists.p + int64_t(cls) * nc;
       auto run = [&](const Tuner::Shape& x) {
-        return gkd::launch_contract_rows(g, x.lanes, x.block, rows, hc[cls], cptr, order, cmap, toff.p, tadj.p,
-                                         tw.p, len.p, st);
+        return gkd::launch_contract_rows(g, x.lanes, x.block, rows, desc.p + int64_t(cls) * nc, hc[cls], nc,
+                                         mrow.p, cmap, toff.p, tadj.p, tw.p, len.p, wide, st);
       };
       Tuner::Shape shape{lanes[cls], 256};
       if (ctx->autotune && hc[cls] >= kTuneMinRows) {
@@ -211,7 +220,7 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
   GK_TRY(c.w.alloc(P, cnnz));
   // Row merge: member rows and the cmap gathers read once, merged rows
   // written once (known now that cnnz is).
-  if (ctx->cur) ctx->cur->stage_bytes[GK_ST_CONTRACT] += 12LL * g.n + 20LL * nc + 12LL * g.nnz + 8LL * cnnz;
+  if (ctx->cur) ctx->cur->stage_bytes[GK_ST_CONTRACT] += 8LL * g.n + 20LL * nc + 12LL * g.nnz + 8LL * cnnz;
   GK_STAGE(ctx, GK_ST_CONTRACT_AUX, 12LL * nc + 16LL * cnnz,
            gkd::launch_contract_copy(nc, toff.p, len.p, c.xadj.p, tadj.p, tw.p, c.adj.p, c.w.p, st));
   c.g.n = nc;

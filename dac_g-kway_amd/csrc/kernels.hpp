@@ -62,17 +62,20 @@ int launch_cluster(int32_t n, const uint64_t* skeys, const int32_t* sids, int sb
                    int32_t* cmap, int32_t* cptr, int32_t* cvwgt, int32_t* nc_dev,
                    int32_t* scratch, hipStream_t st);
 // Contraction, phase 1: per coarse row the upper bound of its entries
-// (sum of member degrees), its class (16/32/64-lane or big) and the
-// exclusive scan of the bounds (toff, nc+1 entries).
+// (sum of member degrees), its class (16/32/64-lane or big), appended to
+// lists[cls * nc ..] with its member range in desc; mrow[m] = (row start,
+// row length) of the member at order position m; the exclusive scan of the
+// bounds (toff, nc+1 entries).  lists and desc: 4 * nc entries, mrow: n.
 int launch_contract_bounds(const DGraph& g, int32_t nc, const int32_t* cptr, const int32_t* order,
-                           int32_t* ub, int32_t* toff, int32_t* lists, int32_t* list_cnt,
-                           int32_t* scratch, hipStream_t st);
+                           int32_t* ub, int32_t* toff, int32_t* lists, int2* desc, int2* mrow,
+                           int32_t* list_cnt, int32_t* scratch, hipStream_t st);
 // Phase 2 for the rows of one class (lanes in {16, 32, 64}); block size 64,
-// 128, 256 or 512 threads (launch shape only).
-int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, int32_t nrows,
-                         const int32_t* cptr, const int32_t* order, const int32_t* cmap,
-                         const int32_t* toff, int32_t* tadj, int32_t* tw, int32_t* len,
-                         hipStream_t st);
+// 128, 256 or 512 threads (launch shape only).  nc: coarse vertex count; the
+// sort packs the lane number beside the coarse id in 32 bits when nc allows,
+// else (or with force_wide) sorts (id, weight) pairs.
+int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, const int2* desc,
+                         int32_t nrows, int32_t nc, const int2* mrow, const int32_t* cmap, const int32_t* toff,
+                         int32_t* tadj, int32_t* tw, int32_t* len, bool force_wide, hipStream_t st);
 // Phase 2 for big rows: gather to (row-rank, neighbour) keys.
 int launch_contract_big_gather(const DGraph& g, const int32_t* rows, int32_t nrows,
                                const int32_t* boff, const int32_t* cptr, const int32_t* order,

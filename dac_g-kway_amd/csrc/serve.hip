@@ -138,9 +138,13 @@ void serve_loop(gk_server* s) {
   hipStream_t st = s->ctx->stream;
   gk_params wp{};
   wp.initpart_threads = s->cfg.workers;
-  auto make_engine = [s, st, &wp]() {
+  // Admitted and not finished (loop thread only).  A request leaves it under
+  // the server lock, before finish_locked makes it visible to its waiter, who
+  // may free it at once.
+  std::vector<Request*> running;
+  auto make_engine = [s, st, &wp, &running]() {
     auto E = std::make_unique<Engine>(s->ctx, st, batch_workers(wp));
-    E->done_cb_ = [s, st](Job& j) -> int {
+    E->done_cb_ = [s, st, &running](Job& j) -> int {
       Request* r = static_cast<Request*>(j.user);
       r->part.resize(r->n);
       if (r->n)
@@ -152,13 +156,13 @@ void serve_loop(gk_server* s) {
       s->foot_now -= r->foot;
       --s->inflight;
       if (r->alone) s->alone_running = false;
+      running.erase(std::remove(running.begin(), running.end(), r), running.end());
       finish_locked(s, r, j.rc, j.err);
       return GK_OK;
     };
     return E;
   };
   std::unique_ptr<Engine> E = make_engine();
-  std::vector<Request*> running;  // admitted and not finished (loop thread only)
   while (true) {
     std::vector<Request*> adm;
     {
@@ -195,8 +199,6 @@ void serve_loop(gk_server* s) {
       rc = E->step(&worked);
       E->drop_done();
     }
-    running.erase(std::remove_if(running.begin(), running.end(), [](Request* r) { return r->state == 2; }),
-                  running.end());
     {
       std::lock_guard<std::mutex> lk(s->m);
       ++s->steps;
@@ -290,8 +292,11 @@ int gk_serve_open(gk_ctx* ctx, const gk_serve_config* cfg, gk_server** out) {
   return GK_OK;
 }
 
-int gk_serve_submit(gk_server* s, const gk_csr* g, const gk_params* p, int64_t* request) {
-  if (!s || !g || !p || !request) { gkh::set_error("null argument"); return GK_ERR_ARG; }
+namespace {
+
+// Validates one request and copies its host graph (no server state touched).
+int make_request(const gk_csr* g, const gk_params* p, std::unique_ptr<Request>* out) {
+  if (!g || !p) { gkh::set_error("null argument"); return GK_ERR_ARG; }
   if (p->k < 1 || p->k > kMaxK || p->ubfactor < 0 || p->max_group < 1 || p->max_group > 64 ||
       p->max_moves < 1 || p->max_moves > 1024) {
     gkh::set_error("bad params");
@@ -309,30 +314,53 @@ int gk_serve_submit(gk_server* s, const gk_csr* g, const gk_params* p, int64_t* 
   for (int32_t x : r->v) r->T += x;
   if (r->T >= (int64_t(1) << 31)) { gkh::set_error("total vertex weight >= 2^31"); return GK_ERR_LIMIT; }
   r->foot = gk_serve_footprint(g->n, g->nnz, p);
+  *out = std::move(r);
+  return GK_OK;
+}
+
+// Queues validated requests in order under one hold of the server lock, so
+// that the loop sees all of them at its next admission.  All or none.
+int enqueue(gk_server* s, std::vector<std::unique_ptr<Request>>& rs, int64_t* ids) {
   std::lock_guard<std::mutex> lk(s->m);
   if (s->stop) { gkh::set_error("server is closing"); return GK_ERR_ARG; }
-  if (!s->have_shape) {
-    s->have_shape = true;
-    s->max_group = p->max_group;
-    s->max_moves = p->max_moves;
-  } else if (p->max_group != s->max_group || p->max_moves != s->max_moves) {
-    gkh::set_error("max_group and max_moves must match the server's first request");
-    return GK_ERR_ARG;
+  int32_t mg = s->have_shape ? s->max_group : rs[0]->prm.max_group;
+  int32_t mm = s->have_shape ? s->max_moves : rs[0]->prm.max_moves;
+  for (const auto& r : rs)
+    if (r->prm.max_group != mg || r->prm.max_moves != mm) {
+      gkh::set_error("max_group and max_moves must match the server's first request");
+      return GK_ERR_ARG;
+    }
+  s->have_shape = true;
+  s->max_group = mg;
+  s->max_moves = mm;
+  for (size_t i = 0; i < rs.size(); ++i) {
+    auto& r = rs[i];
+    r->id = s->next_id++;
+    r->t_submit = s->now();
+    if (s->first_submit < 0) s->first_submit = r->t_submit;
+    ++s->submitted;
+    ids[i] = r->id;
+    Request* rp = r.get();
+    s->reqs.emplace(r->id, std::move(r));
+    if (rp->n == 0) finish_locked(s, rp, GK_OK, "");
+    else s->queue.push_back(rp);
   }
-  r->id = s->next_id++;
-  r->t_submit = s->now();
-  if (s->first_submit < 0) s->first_submit = r->t_submit;
-  ++s->submitted;
-  *request = r->id;
-  Request* rp = r.get();
-  s->reqs.emplace(r->id, std::move(r));
-  if (rp->n == 0) {
-    finish_locked(s, rp, GK_OK, "");
-    return GK_OK;
-  }
-  s->queue.push_back(rp);
   s->cv_in.notify_one();
   return GK_OK;
+}
+
+}  // namespace
+
+int gk_serve_submit(gk_server* s, const gk_csr* g, const gk_params* p, int64_t* request) {
+  return gk_serve_submit_many(s, 1, g, p, request);
+}
+
+int gk_serve_submit_many(gk_server* s, int32_t count, const gk_csr* graphs, const gk_params* params,
+                         int64_t* requests) {
+  if (!s || !graphs || !params || !requests || count < 1) { gkh::set_error("null argument"); return GK_ERR_ARG; }
+  std::vector<std::unique_ptr<Request>> rs(static_cast<size_t>(count));
+  for (int32_t i = 0; i < count; ++i) GK_TRY(make_request(graphs + i, params + i, &rs[i]));
+  return enqueue(s, rs, requests);
 }
 
 int gk_serve_poll(gk_server* s, int64_t request, int32_t* part_out, gk_stats* stats) {

@@ -258,6 +258,13 @@ typedef struct gk_serve_stats {
 int gk_serve_open(gk_ctx* ctx, const gk_serve_config* cfg, gk_server** out);
 /* Copies the host graph; *request receives the request id. */
 int gk_serve_submit(gk_server* s, const gk_csr* graph, const gk_params* params, int64_t* request);
+/* count requests at once (graphs[i] at params[i]; ids to requests[i]): all
+ * are validated first and queued together, in order, so that the loop's next
+ * admission sees the whole burst; on an error none is queued.  No
+ * counterpart in the reference (no serving there): for clients that submit
+ * bursts. */
+int gk_serve_submit_many(gk_server* s, int32_t count, const gk_csr* graphs, const gk_params* params,
+                         int64_t* requests);
 /* 1 = finished (part_out[n] and *stats written, the request is released),
  * 0 = still queued or running, < 0 = the request failed (also released).
  * GK_ERR_ARG for an unknown id, a result already taken, or a request another

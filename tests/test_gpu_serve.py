@@ -100,9 +100,11 @@ def test_serve_rejects_other_shape(gk):
 
 # Failures stay with the request that causes them.  A device pool limit below
 # what the large graph needs makes its allocations fail (admission at 0.9 x
-# its CSR bytes, a later step at 1.3 x) while several requests share the
-# device; the loop then runs each of them again alone: the small ones finish
-# (equal to alone and the oracle), only the large one reports the error.
+# its CSR bytes, its first coarsening step at 1.3 x) while several requests
+# share the device (all are queued in one burst, so the loop admits them
+# together); the loop then runs each of them again alone: the small ones
+# finish (equal to alone and the oracle), only the large one reports the
+# error.
 @pytest.mark.parametrize("factor", [0.9, 1.3])
 def test_serve_failure_isolated(gk, ctx, factor):
     smalls, ks = mixed_set(gk, 6, 14, lo=2000, hi=8000, ks=(8, 16))
@@ -110,8 +112,7 @@ def test_serve_failure_isolated(gk, ctx, factor):
     limit = int(factor * 4 * (2 * big.n + 1 + 2 * big.nnz))
     print(f"pool limit {limit} bytes", flush=True)
     with gk.Server(pool_limit=limit) as srv:
-        ids = [srv.submit(g, k) for g, k in zip(smalls, ks)]
-        bid = srv.submit(big, 64)
+        *ids, bid = srv.submit_many(smalls + [big], list(ks) + [64])
         res = [srv.wait(r) for r in ids]
         with pytest.raises(RuntimeError, match="pool limit"):
             srv.wait(bid)

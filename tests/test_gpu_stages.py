@@ -130,6 +130,54 @@ def test_contract_big_rows(gk, oracle, ctx):
     assert np.array_equal(host(cw)[: len(ow)], ow)
 
 
+@pytest.mark.parametrize("wide", [False, True])
+def test_contract_row_classes(gk, oracle, ctx, wide, monkeypatch):
+    """Coarse rows of every lane class (16, 32, 64, big) from random groups
+    of a weighted mesh, rows whose members outnumber the lanes (isolated
+    members without entries), rows with no entries at all; both forms of the
+    row kernels (lane number packed beside the coarse id, and (id, weight)
+    pairs, forced by GK_CONTRACT_WIDE=1)."""
+    from _dev import DevGraph, dev, host
+    if wide:
+        monkeypatch.setenv("GK_CONTRACT_WIDE", "1")
+    L = gk.lib()
+    m = gk.delaunay(6000, 9)
+    iso = 400
+    rng = np.random.default_rng(21)
+    xadj = np.concatenate([m.xadj, np.full(iso, m.xadj[-1])]).astype(np.int64)
+    # symmetric weights 1..49: one draw per undirected edge
+    rows = np.repeat(np.arange(m.n), np.diff(m.xadj))
+    key = np.minimum(rows, m.adjncy) * np.int64(m.n) + np.maximum(rows, m.adjncy)
+    _, inv = np.unique(key, return_inverse=True)
+    w = rng.integers(1, 50, size=inv.max() + 1)[inv]
+    g = gk.Graph(xadj, m.adjncy, w, np.ones(m.n + iso, np.int64))
+    # mesh vertices: ids 0..1499 once each, then half spread evenly and half
+    # on ids 0..49 (big rows); isolated vertices: 100 on id 1400 and 100 on
+    # ids 1401..1404 (more members than lanes), 200 on ids 1500..1502 (no
+    # entries)
+    nm = 1500
+    rest = m.n - nm
+    cmap = np.concatenate([np.arange(nm), rng.integers(0, nm, size=rest // 2), rng.integers(0, 50, size=rest - rest // 2),
+                           np.full(100, 1400), rng.integers(1401, 1405, size=100),
+                           np.full(100, nm), np.full(50, nm + 1), np.full(50, nm + 2)]).astype(np.int32)
+    nc = nm + 3
+    dg = DevGraph(g)
+    cx = torch.zeros(nc + 1, dtype=torch.int32, device="cuda")
+    ca = torch.zeros(g.nnz, dtype=torch.int32, device="cuda")
+    cw = torch.zeros_like(ca)
+    cnnz = ctypes.c_int32(0)
+    assert L.gk_k_contract(ctx.handle, dg.ref(), _I(dev(cmap)), nc, _I(cx), _I(ca), _I(cw),
+                           ctypes.cast(ctypes.pointer(cnnz), gk.I32P), None) == 0
+    ox, oa, ow = oracle.contract(g, cmap, nc)
+    ub = np.zeros(nc, np.int64)
+    np.add.at(ub, cmap, np.diff(xadj))
+    assert (ub <= 16).any() and ((ub > 16) & (ub <= 32)).any() and ((ub > 32) & (ub <= 64)).any() and (ub > 64).any()
+    assert cnnz.value == len(oa)
+    assert np.array_equal(host(cx), ox)
+    assert np.array_equal(host(ca)[: len(oa)], oa)
+    assert np.array_equal(host(cw)[: len(ow)], ow)
+
+
 # (k, moves, seed, lowest slack below the cap): the oracle's prefix ends at
 # 0, 79, 9, 3, 0, 6, 1, 9 for the tight cases, 501, 1012, 729, 3, 1023 for
 # the roomier ones, and nowhere (-1) for (8192, 1024, 14, -12).
