@@ -89,11 +89,12 @@ __device__ inline void bitonic_lds(T* a, int P) {
     x0 = a[i0];
     x1 = a[i1];
   }
-  // Strides j2 = jstart .. 1 of merge size k2 on the wave's registers.
-  auto inwave = [&](int k2, int jstart) {
+  // Strides JS, JS / 2, ..., 1 of merge size k2 on the wave's registers.
+  auto inwave = [&](auto jsc, int k2) {
     if (!own) return;
-    for (int j2 = jstart; j2 > 0; j2 >>= 1) {
-      if (j2 == 64) {
+    for_pow2_down<decltype(jsc)::value>([&](auto jc) {
+      constexpr int J = decltype(jc)::value;
+      if constexpr (J == 64) {
         // i0 and i1 = i0 + 64 are partners; k2 >= 128 gives both the same direction
         const bool up = (i0 & k2) == 0;
         const T lo = x0 < x1 ? x0 : x1;
@@ -101,18 +102,24 @@ __device__ inline void bitonic_lds(T* a, int P) {
         x0 = up ? lo : hi;
         x1 = up ? hi : lo;
       } else {
-        const T y0 = __shfl_xor(x0, j2, 64);
-        const T y1 = __shfl_xor(x1, j2, 64);
-        const bool lower = (lane & j2) == 0;
+        const T y0 = lane_xor<J>(x0);
+        const T y1 = lane_xor<J>(x1);
+        const bool lower = (lane & J) == 0;
         // the lower element of an ascending pair keeps the minimum
         const bool mn0 = lower == ((i0 & k2) == 0);
         const bool mn1 = lower == ((i1 & k2) == 0);
         x0 = mn0 ? (y0 < x0 ? y0 : x0) : (y0 > x0 ? y0 : x0);
         x1 = mn1 ? (y1 < x1 ? y1 : x1) : (y1 > x1 ? y1 : x1);
       }
-    }
+    });
   };
-  for (int k2 = 2; k2 <= 128; k2 <<= 1) inwave(k2, k2 >> 1);
+  inwave(std::integral_constant<int, 1>{}, 2);
+  inwave(std::integral_constant<int, 2>{}, 4);
+  inwave(std::integral_constant<int, 4>{}, 8);
+  inwave(std::integral_constant<int, 8>{}, 16);
+  inwave(std::integral_constant<int, 16>{}, 32);
+  inwave(std::integral_constant<int, 32>{}, 64);
+  inwave(std::integral_constant<int, 64>{}, 128);
   for (int k2 = 256; k2 <= P; k2 <<= 1) {
     if (own) {
       a[i0] = x0;
@@ -136,7 +143,7 @@ __device__ inline void bitonic_lds(T* a, int P) {
       x0 = a[i0];
       x1 = a[i1];
     }
-    inwave(k2, 64);
+    inwave(std::integral_constant<int, 64>{}, k2);
   }
   if (own) {
     a[i0] = x0;
@@ -221,6 +228,15 @@ __device__ inline uint32_t* sort_events_by_part(uint32_t* a, uint32_t* b, int n,
   return a;
 }
 
+// Optional inputs and outputs of balance_prefix.
+struct BalanceOpt {
+  const int32_t* spw = nullptr;  // the part weights already in LDS: read instead of pw
+  int32_t* mdst = nullptr;       // when set, each position's destination (LDS, B entries)
+  bool gain_in_key = false;      // key bits 32..62 hold 0x7fffffff - gain: gain[] is not read
+  bool cut_given = false;        // thread 0's cut_now holds *cut: *cut is written, not read
+  long long cut_now = 0;
+};
+
 // Largest balanced prefix of the B sorted moves in L.sk (vertex id in the
 // bits of idmask): the last position pos such that, after moves 0..pos, no
 // part whose weight grew is above cap.  dpw: k ints of LDS (k parts, k <=
@@ -232,9 +248,11 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
                                      const int32_t* __restrict__ part, const int32_t* __restrict__ dest,
                                      const int32_t* __restrict__ gain, const int32_t* __restrict__ vw,
                                      int32_t* __restrict__ pw, long long cap, int32_t* __restrict__ moves,
-                                     int32_t* __restrict__ pos_out, long long* __restrict__ cut) {
+                                     int32_t* __restrict__ pos_out, long long* __restrict__ cut,
+                                     const BalanceOpt& opt = BalanceOpt{}) {
   const int t = threadIdx.x;
   const int lane = lane_id();
+  auto pw_of = [&](int32_t q) -> int32_t { return opt.spw ? opt.spw[q] : pw[q]; };
   const int wave = t >> 6;
   // 1. the moves, and each part's inflow over the whole sequence
   for (int32_t q = t; q < k; q += kBalThreads) dpw[q] = 0;
@@ -242,13 +260,15 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   L.cover[t] = 0;
   int32_t g = 0, src = 0, dst = 0, w = 0;
   if (t < B) {
-    const int32_t v = int32_t(L.sk[t] & idmask);
+    const uint64_t key = L.sk[t];
+    const int32_t v = int32_t(key & idmask);
     L.sv[t] = v;
     w = vw[v];
     L.sw[t] = w;
-    g = gain[v];
+    g = opt.gain_in_key ? int32_t(0x7fffffffu - uint32_t(key >> 32)) : gain[v];
     src = part[v];
     dst = dest[v];
+    if (opt.mdst) opt.mdst[t] = dst;
   }
   __syncthreads();
   if (t < B) atomicAdd(&dpw[dst], w);
@@ -261,7 +281,7 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
     bool r = false;
     if (q < k) {
       const int32_t in = dpw[q];
-      r = in > 0 && (long long)pw[q] + in > cap;
+      r = in > 0 && (long long)pw_of(q) + in > cap;
       dpw[q] = 0;
     }
     const uint64_t m = ballot(r);
@@ -310,7 +330,7 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   if (ha) {
     const int32_t sa = pa - dpw[qa];
     const int32_t na = (hb && qb == qa) ? jb : B;
-    if (ja < na && sa > 0 && (long long)pw[qa] + sa > cap) {
+    if (ja < na && sa > 0 && (long long)pw_of(qa) + sa > cap) {
       atomicAdd(&L.cover[ja], 1);
       atomicAdd(&L.cover[na], -1);
     }
@@ -322,7 +342,7 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
       const uint32_t ec = ev[2 * t + 2];
       if (int32_t(ec >> 11) == qb) nb = int32_t((ec >> 1) & 1023u);
     }
-    if (jb < nb && sb > 0 && (long long)pw[qb] + sb > cap) {
+    if (jb < nb && sb > 0 && (long long)pw_of(qb) + sb > cap) {
       atomicAdd(&L.cover[jb], 1);
       atomicAdd(&L.cover[nb], -1);
     }
@@ -352,12 +372,13 @@ __device__ inline int balance_prefix(BalanceLds& L, int32_t* dpw, int32_t k, int
   __syncthreads();
   for (int32_t q = t; q < k; q += kBalThreads) {
     const int32_t d = dpw[q];
-    if (d != 0) pw[q] += d;
+    if (d != 0) pw[q] = pw_of(q) + d;
   }
   if (t == 0) {
     long long s = 0;
     for (int w2 = 0; w2 < kBalThreads / 64; ++w2) s += L.sgain[w2];
-    *cut -= s;
+    if (opt.cut_given) *cut = opt.cut_now - s;
+    else *cut -= s;
     if (pos_out) *pos_out = pos;
   }
   return pos;

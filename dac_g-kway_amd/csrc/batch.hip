@@ -565,6 +565,8 @@ int launch_select_list(const DGraph& g, const int32_t* vjob, const int32_t* gain
 
 struct RoundLds {
   BalanceLds b;  // b.sk doubles as the 2048-bin radix histogram
+  int32_t mdst[kBalMax];  // each ordered move's destination
+  int32_t ht[1024];  // the apply's table of first occurrences (cleared at the start)
   int32_t ws[40];
   int32_t misc[4];
   int32_t lz[4];  // deferred tests: selected count; next threshold's digit; next threshold
@@ -575,21 +577,11 @@ struct RoundLds {
 constexpr int kLazyFactor = 2;
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t y = __shfl_xor(x, o, 64);
-    x = y < x ? y : x;
-  }
-  return x;
+  return wave_reduce(x, [](uint32_t a, uint32_t b) { return b < a ? b : a; });
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t y = __shfl_xor(x, o, 64);
-    x = y > x ? y : x;
-  }
-  return x;
+  return wave_reduce(x, [](uint32_t a, uint32_t b) { return b > a ? b : a; });
 }
 
 // Bits needed to hold values 0..x (at least 1).
@@ -615,7 +607,8 @@ __device__ __forceinline__ uint64_t move_key(const int32_t* __restrict__ gain, i
 // A job with nothing selected is left untouched, so extra rounds after the
 // last productive one change nothing.
 __global__ void __launch_bounds__(kBalThreads) k_round_b(
-    const int32_t* __restrict__ v_off, int32_t M, const int32_t* __restrict__ kk, const long long* __restrict__ cap,
+    const int32_t* __restrict__ v_off, int32_t M, int32_t kmax, const int32_t* __restrict__ kk,
+    const long long* __restrict__ cap,
     const int32_t* __restrict__ pw_off, const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj,
     int32_t* __restrict__ part, const int32_t* __restrict__ dest, const int32_t* __restrict__ gain,
     const int32_t* __restrict__ vw, int32_t* __restrict__ pw, long long* __restrict__ cut,
@@ -625,11 +618,25 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     int32_t* __restrict__ thr, long long* __restrict__ paths, int32_t* __restrict__ order,
     long long* __restrict__ abytes) {
   __shared__ RoundLds R;
-  extern __shared__ int32_t dpw[];
+  extern __shared__ int32_t dpw[];  // 2 * kmax ints: part weight changes, then the part weights
+  int32_t* spw = dpw + kmax;
   const int t = threadIdx.x;
   const int32_t j = blockIdx.x;
   const int32_t vb = v_off[j], ve = v_off[j + 1];
+  // Loads that the end of the round needs are issued now, off its critical
+  // path: the job's first kBalThreads part weights (stored to LDS before the
+  // balanced prefix) and its counters (thread 0).
+  const int32_t kj = kk[j];
+  const int32_t* pwj = pw + pw_off[j];
+  const int32_t pw_t = t < kj ? pwj[t] : 0;
+  R.ht[t] = -1;  // kBalThreads == table slots; the barriers before the apply order it
+  long long cut0 = 0, moved0 = 0, ab0 = 0;
+  int32_t rounds0 = 0;
   if (t == 0) {
+    cut0 = cut[j];
+    moved0 = moved[j];
+    rounds0 = rounds[j];
+    if (abytes) ab0 = abytes[j];
     R.misc[3] = sel_slot(sb.wpos, sb.selbits, vb);
     R.misc[1] = sel_slot(sb.wpos, sb.selbits, ve);
     R.misc[2] = lastcnt[j];
@@ -701,16 +708,49 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     // Exact top-B by radix select on a compressed key: (gain key - its
     // minimum) above (id - first id of the job); the key is monotone in
     // (gain desc, id asc) and has only as many bits as the ranges need.
-    // Pass 0 gathers the gain keys into the job's slice of hkey and finds
-    // their range; the passes then use 11-bit digits from the top.
+    // Pass 0 gathers the gain keys and finds their range; the passes then
+    // use 11-bit digits from the top.  Up to kKeep keys per thread stay in
+    // registers across the passes (with their vertices); more go through the
+    // job's slice of hkey.
+    constexpr int kKeep = 4;
+    const bool inreg = count <= kKeep * kBalThreads;
     uint32_t* hk = hkey + vb;
+    uint32_t hr[kKeep];
+    int32_t vr[kKeep];
     uint32_t lo = 0xffffffffu, hi = 0;
-    for (int32_t i = t; i < count; i += kBalThreads) {
-      const uint32_t h = uint32_t(0x7fffffff - gain[seg[i]]);
-      hk[i] = h;
-      lo = h < lo ? h : lo;
-      hi = h > hi ? h : hi;
+    if (inreg) {
+#pragma unroll
+      for (int r = 0; r < kKeep; ++r) {
+        const int32_t i = t + r * kBalThreads;
+        vr[r] = i < count ? seg[i] : vb;
+      }
+#pragma unroll
+      for (int r = 0; r < kKeep; ++r) {
+        const uint32_t h = uint32_t(0x7fffffff - gain[vr[r]]);
+        hr[r] = h;
+        if (t + r * kBalThreads < count) {
+          lo = h < lo ? h : lo;
+          hi = h > hi ? h : hi;
+        }
+      }
+    } else {
+      for (int32_t i = t; i < count; i += kBalThreads) {
+        const uint32_t h = uint32_t(0x7fffffff - gain[seg[i]]);
+        hk[i] = h;
+        lo = h < lo ? h : lo;
+        hi = h > hi ? h : hi;
+      }
     }
+    // fn(gain key, vertex) for every selected vertex this thread holds
+    auto each = [&](auto&& fn) {
+      if (inreg) {
+#pragma unroll
+        for (int r = 0; r < kKeep; ++r)
+          if (t + r * kBalThreads < count) fn(hr[r], vr[r]);
+      } else {
+        for (int32_t i = t; i < count; i += kBalThreads) fn(hk[i], seg[i]);
+      }
+    };
     lo = wave_min_u32(lo);
     hi = wave_max_u32(hi);
     if (lane_id() == 0) {
@@ -731,8 +771,8 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     const uint32_t hmin = uint32_t(R.ws[32]), hrange = uint32_t(R.ws[33]) - hmin;
     const int ib = bits_of(uint32_t(v_off[j + 1] - vb - 1));
     const int tb = bits_of(hrange) + ib;
-    auto ckey = [&](int32_t i) -> uint64_t {
-      return (uint64_t(hk[i] - hmin) << ib) | uint64_t(uint32_t(seg[i] - vb));
+    auto ckey = [&](uint32_t h, int32_t v) -> uint64_t {
+      return (uint64_t(h - hmin) << ib) | uint64_t(uint32_t(v - vb));
     };
     constexpr int kDigit = 11;
     int32_t* hist = reinterpret_cast<int32_t*>(R.b.sk);  // 2048 ints, free until the keys are collected
@@ -742,10 +782,10 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
       hist[2 * t] = 0;
       hist[2 * t + 1] = 0;
       __syncthreads();
-      for (int32_t i = t; i < count; i += kBalThreads) {
-        const uint64_t key = ckey(i);
+      each([&](uint32_t h, int32_t v) {
+        const uint64_t key = ckey(h, v);
         if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 2047], 1);
-      }
+      });
       __syncthreads();
       // The digit where the running count reaches `need`: a block scan over
       // the 2048 bins, two per thread.
@@ -794,14 +834,14 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
     if (narrow) nk[t] = 0xffffffffu;
     else R.b.sk[t] = ~0ull;
     __syncthreads();
-    for (int32_t i = t; i < count; i += kBalThreads) {
-      const uint64_t ck = ckey(i);
+    each([&](uint32_t h, int32_t v) {
+      const uint64_t ck = ckey(h, v);
       if ((ck & pmask) <= prefix) {
         const int slot = atomicAdd(&R.misc[2], 1);
         if (narrow) nk[slot] = uint32_t(ck);
-        else R.b.sk[slot] = (uint64_t(hk[i]) << 32) | uint64_t(uint32_t(seg[i]));
+        else R.b.sk[slot] = (uint64_t(h) << 32) | uint64_t(uint32_t(v));
       }
-    }
+    });
     if (narrow) {
       __syncthreads();
       int P = 128;
@@ -815,8 +855,16 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   __syncthreads();
   // 3. sort and balanced prefix
   if (!narrow) lds_bitonic(R.b, B);
-  const int pos = balance_prefix(R.b, dpw, kk[j], B, 0xffffffffull, part, dest, gain, vw, pw + pw_off[j], cap[j],
-                                 nullptr, nullptr, &cut[j]);
+  if (t < kj) spw[t] = pw_t;
+  for (int32_t q = t + kBalThreads; q < kj; q += kBalThreads) spw[q] = pwj[q];
+  BalanceOpt bo;
+  bo.spw = spw;
+  bo.mdst = R.mdst;
+  bo.gain_in_key = true;
+  bo.cut_given = true;
+  bo.cut_now = cut0;
+  const int pos = balance_prefix(R.b, dpw, kj, B, 0xffffffffull, part, dest, gain, vw, pw + pw_off[j], cap[j],
+                                 nullptr, nullptr, &cut[j], bo);
   // Optional trace of the round: which ordering path ran (top-M radix
   // select; compressed keys wider than 32 bits) and the ordered moves with B
   // and pos.  R.b.sv keeps the ordered vertices through the apply below.
@@ -835,85 +883,156 @@ __global__ void __launch_bounds__(kBalThreads) k_round_b(
   // 4. apply moves 0..pos (balance_prefix read part[] before its last barrier):
   // the movers and their neighbours are items [0, tot) -- mover m owns
   // [istart[m], istart[m] + deg + 1), first itself, then its row -- dealt out
-  // to the block in passes of kApplyPer items per thread.  An item's vertex
-  // joins the gain worklist if its update flag was clear; the winners of a
-  // pass are appended with one global atomic for the whole block.
-  const bool mv = t <= pos;
-  int32_t es = 0, deg = 0;
-  if (mv) {
-    const int32_t v = R.b.sv[t];
-    part[v] = dest[v];
-    es = xadj[v];
-    deg = xadj[v + 1] - es;
-  }
-  int tot;
-  const int ib = block_excl_scan_1024(mv ? deg + 1 : 0, &tot, R.ws);
+  // in passes of kApplyPer items per thread.  An item's vertex joins the gain
+  // worklist if its update flag was clear.  One workgroup (one job) owns the
+  // worklist count, which the selection zeroed: it appends at a running
+  // offset and stores the count at the end; several jobs append with one
+  // global atomic per pass.
+  constexpr int kApplyPer = 8;
+  const bool sole = gridDim.x == 1;
   int32_t* istart = R.b.sw;     // free after balance_prefix
   int32_t* rstart = R.b.cover;  // likewise
-  if (mv) {
-    istart[t] = ib;
-    rstart[t] = es;
-  }
-  __syncthreads();
-  constexpr int kApplyPer = 8;
-  // One workgroup (one job) owns the worklist count, which the selection
-  // zeroed: it appends at a running offset and stores the count at the end.
-  // Several jobs append with one global atomic per pass.
-  const bool sole = gridDim.x == 1;
-  int32_t wrun = 0;
-  for (int pb = 0; pb < tot; pb += kApplyPer * kBalThreads) {
-    const int a0 = pb + kApplyPer * t;
-    int32_t us[kApplyPer];
+  int tot = 0;
+  // Items [a0, a0 + kApplyPer) of [0, tot): their vertices (-1 past the end)
+  // and the bit mask of the first occurrences.  With a table ht (LDS, kHt
+  // slots, all -1, more than twice the items): first occurrences within the
+  // round by insertion into ht, the update flag then set by a plain store --
+  // every flag is clear when a round starts (the gain pass of the previous
+  // round cleared those it set).  Without: by the update flag's atomic swap.
+  constexpr int kHt = sizeof(R.ht) / sizeof(int32_t);
+  static_assert(kHt == 1024, "10-bit table index");
+  auto take = [&](int a0, int32_t (&us)[kApplyPer], int32_t* ht) -> uint32_t {
     uint32_t win = 0;
-    if (a0 < tot) {
-      int lo = 0, hi = pos;  // last mover m with istart[m] <= a0
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (istart[mid] <= a0) lo = mid; else hi = mid - 1;
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) us[i] = -1;
+    if (a0 >= tot) return 0;
+    int lo = 0, hi = pos;  // last mover m with istart[m] <= a0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (istart[mid] <= a0) lo = mid; else hi = mid - 1;
+    }
+    int m = lo;
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+      const int e2 = a0 + i;
+      if (e2 < tot) {
+        while (m < pos && istart[m + 1] <= e2) ++m;
+        const int off = e2 - istart[m];
+        us[i] = off == 0 ? R.b.sv[m] : adj[rstart[m] + off - 1];
       }
-      int m = lo;
+    }
+    if (ht) {
 #pragma unroll
       for (int i = 0; i < kApplyPer; ++i) {
-        const int e2 = a0 + i;
-        us[i] = -1;
-        if (e2 < tot) {
-          while (m < pos && istart[m + 1] <= e2) ++m;
-          const int off = e2 - istart[m];
-          us[i] = off == 0 ? R.b.sv[m] : adj[rstart[m] + off - 1];
+        if (us[i] < 0) continue;
+        uint32_t sl = (uint32_t(us[i]) * 0x9E3779B1u) >> 22;  // 10 bits: kHt slots
+        while (true) {
+          const int32_t o = atomicCAS(&ht[sl], -1, us[i]);
+          if (o == -1) {
+            win |= 1u << i;
+            upd[us[i]] = 1;
+            break;
+          }
+          if (o == us[i]) break;
+          sl = (sl + 1) & (kHt - 1);
         }
       }
+    } else {
 #pragma unroll
       for (int i = 0; i < kApplyPer; ++i)
         if (us[i] >= 0 && atomicExch(&upd[us[i]], 1) == 0) win |= 1u << i;
     }
-    int wtot;
-    const int wex = block_excl_scan_1024(__popc(win), &wtot, R.ws);
-    if (t == 0) {
-      if (sole) {
-        R.misc[0] = wrun;
+    return win;
+  };
+  int32_t wrun = 0;
+  if (pos < 64) {
+    // At most 64 movers (the usual case once the parts sit at the cap): wave
+    // 0 alone, with wave-level scans and no block barriers.
+    if (t < 64) {
+      const bool mv = t <= pos;
+      int32_t es = 0, cnt = 0;
+      if (mv) {
+        const int32_t v = R.b.sv[t];
+        part[v] = R.mdst[t];
+        es = xadj[v];
+        cnt = xadj[v + 1] - es + 1;
+      }
+      const int32_t incl = seg_incl_scan<64>(cnt);
+      tot = __shfl(incl, 63, 64);
+      if (mv) {
+        istart[t] = incl - cnt;
+        rstart[t] = es;
+      }
+      // first occurrences in LDS when the items fit the table
+      int32_t* ht = tot <= kHt / 2 ? R.ht : nullptr;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int pb = 0; pb < tot; pb += kApplyPer * 64) {
+        int32_t us[kApplyPer];
+        const uint32_t win = take(pb + kApplyPer * t, us, ht);
+        const int c = __popc(win);
+        const int wincl = seg_incl_scan<64>(c);
+        const int wtot = __shfl(wincl, 63, 64);
+        int base = wrun;
+        if (!sole) {
+          int b0 = 0;
+          if (t == 0 && wtot > 0) b0 = atomicAdd(&wlc[0], wtot);
+          base = __shfl(b0, 0, 64);
+        }
+        int r = base + wincl - c;
+#pragma unroll
+        for (int i = 0; i < kApplyPer; ++i)
+          if ((win >> i) & 1u) wl[r++] = us[i];
         wrun += wtot;
-      } else if (wtot > 0) {
-        R.misc[0] = atomicAdd(&wlc[0], wtot);
       }
     }
+  } else {
+    const bool mv = t <= pos;
+    int32_t es = 0, deg = 0;
+    if (mv) {
+      const int32_t v = R.b.sv[t];
+      part[v] = R.mdst[t];
+      es = xadj[v];
+      deg = xadj[v + 1] - es;
+    }
+    const int ib = block_excl_scan_1024(mv ? deg + 1 : 0, &tot, R.ws);
+    if (mv) {
+      istart[t] = ib;
+      rstart[t] = es;
+    }
     __syncthreads();
-    const int wbase = R.misc[0] + wex;
-    int r = 0;
+    for (int pb = 0; pb < tot; pb += kApplyPer * kBalThreads) {
+      int32_t us[kApplyPer];
+      const uint32_t win = take(pb + kApplyPer * t, us, nullptr);
+      int wtot;
+      const int wex = block_excl_scan_1024(__popc(win), &wtot, R.ws);
+      if (t == 0) {
+        if (sole) {
+          R.misc[0] = wrun;
+          wrun += wtot;
+        } else if (wtot > 0) {
+          R.misc[0] = atomicAdd(&wlc[0], wtot);
+        }
+      }
+      __syncthreads();
+      const int wbase = R.misc[0] + wex;
+      int r = 0;
 #pragma unroll
-    for (int i = 0; i < kApplyPer; ++i)
-      if ((win >> i) & 1u) wl[wbase + r++] = us[i];
-    __syncthreads();  // R.misc[0] is reused by the next pass
+      for (int i = 0; i < kApplyPer; ++i)
+        if ((win >> i) & 1u) wl[wbase + r++] = us[i];
+      __syncthreads();  // R.misc[0] is reused by the next pass
+    }
   }
   if (t == 0) {
     if (sole) wlc[0] = wrun;
     thr[j] = R.lz[2];
-    rounds[j] += 1;
-    moved[j] += pos + 1;
+    rounds[j] = rounds0 + 1;
+    moved[j] = moved0 + pos + 1;
     // Algorithmic bytes of the round: each selected vertex's id and gain,
     // each ordered move's weight, gain, part and destination, the job's part
     // weights, and per applied item (a mover or a neighbour) its id, update
     // flag and worklist slot.
-    if (abytes) abytes[j] += 8LL * count + 16LL * B + 4LL * kk[j] + 12LL * tot;
+    if (abytes) abytes[j] = ab0 + 8LL * count + 16LL * B + 4LL * kj + 12LL * tot;
   }
 }
 
@@ -925,7 +1044,8 @@ int launch_round_b(int32_t nj, int32_t kmax, const int32_t* v_off, int32_t M, co
                    long long* abytes, hipStream_t st) {
   if (nj <= 0) return 0;
   if (M > kBalMax) return GK_ERR_LIMIT;
-  gk_launch(k_round_b, dim3(nj), dim3(kBalThreads), uint32_t(sizeof(int32_t) * kmax), st, v_off, M, kk, cap, pw_off,
+  gk_launch(k_round_b, dim3(nj), dim3(kBalThreads), uint32_t(2 * sizeof(int32_t) * kmax), st, v_off, M, kmax, kk, cap,
+            pw_off,
             g.xadj, g.adj, part, dest, gain, g.vw, pw, cut, rounds, moved, sb, xlist, hkey, upd, wl, wlc, lastcnt,
             active, cand, thr, paths, order, abytes);
   return hip_status();

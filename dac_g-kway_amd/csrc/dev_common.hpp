@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #define GK_WAVE 64
 
@@ -20,35 +21,6 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
-
-// Wave-level inclusive add-scan (int32), whole wave active.
-__device__ __forceinline__ int wave_incl_scan(int x) {
-  const int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(x, o, 64);
-    if (l >= o) x += y;
-  }
-  return x;
-}
-
-__device__ __forceinline__ int wave_sum(int x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-
-__device__ __forceinline__ long long wave_sum64(long long x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-
-__device__ __forceinline__ int wave_max(int x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
-  return x;
-}
 
 // ---------------------------------------------------------------- DPP moves
 // Lane permutations and shifts as DPP source modifiers (gfx9 encoding): a
@@ -132,6 +104,56 @@ __device__ __forceinline__ uint32_t seg_sort_u32(uint32_t x) {
     cx(dpp_mov<dpp::kQuadXor1>(x), lo1);
   }
   return x;
+}
+
+// x from lane ^ J (J a power of two below 64): DPP moves inside a row,
+// ds_swizzle for 16, ds_bpermute for 32.
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x) {
+  static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "lane distance");
+  if constexpr (J == 1) return dpp_mov<dpp::kQuadXor1>(x);
+  else if constexpr (J == 2) return dpp_mov<dpp::kQuadXor2>(x);
+  else if constexpr (J == 4) return dpp_mov<dpp::kQuadXor3>(dpp_mov<dpp::kRowHalfMirror>(x));
+  else if constexpr (J == 8) return dpp_mov<dpp::kRowRor8>(x);
+  else if constexpr (J == 16) return swz_xor<16>(x);
+  else return static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), J, 64));
+}
+template <int J>
+__device__ __forceinline__ uint64_t lane_xor(uint64_t x) {
+  const uint64_t lo = lane_xor<J>(static_cast<uint32_t>(x)), hi = lane_xor<J>(static_cast<uint32_t>(x >> 32));
+  return (hi << 32) | lo;
+}
+
+// f(std::integral_constant<int, J>) for J = J0, J0 / 2, ..., 1.
+template <int J0, class F>
+__device__ __forceinline__ void for_pow2_down(F&& f) {
+  f(std::integral_constant<int, J0>{});
+  if constexpr (J0 > 1) for_pow2_down<J0 / 2>(f);
+}
+
+// Wave-wide op-reduction by lane ^ J exchanges (whole wave active); every
+// lane gets the result.
+template <class T, class Op>
+__device__ __forceinline__ T wave_reduce(T x, Op op) {
+  for_pow2_down<32>([&](auto jc) { x = op(x, static_cast<T>(lane_xor<decltype(jc)::value>(x))); });
+  return x;
+}
+
+// Wave-level inclusive add-scan (int32), whole wave active.
+__device__ __forceinline__ int wave_incl_scan(int x) { return seg_incl_scan<64>(x); }
+
+__device__ __forceinline__ int wave_sum(int x) {
+  return static_cast<int>(wave_reduce(static_cast<uint32_t>(x), [](uint32_t a, uint32_t b) { return a + b; }));
+}
+
+__device__ __forceinline__ long long wave_sum64(long long x) {
+  return static_cast<long long>(wave_reduce(static_cast<uint64_t>(x), [](uint64_t a, uint64_t b) { return a + b; }));
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+  return static_cast<int>(wave_reduce(static_cast<uint32_t>(x), [](uint32_t a, uint32_t b) {
+    return static_cast<uint32_t>(max(static_cast<int>(a), static_cast<int>(b)));
+  }));
 }
 
 // Highest lane at or below `lane` whose bit is set in m (-1 if none).
