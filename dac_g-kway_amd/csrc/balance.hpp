@@ -47,27 +47,24 @@ struct BalanceLds {
 };
 
 // Exclusive scan of one int per thread over a kBalThreads block; *tot gets
-// the block total.  ws: >= kBalThreads / 64 + 1 ints of LDS.
+// the block total.  ws: >= kBalThreads / 64 ints of LDS.  One barrier: each
+// wave adds up the wave totals below it itself.  ws is read after that
+// barrier, so the next scan on the same ws must follow another barrier.
 __device__ __forceinline__ int block_excl_scan_1024(int x, int* tot, int32_t* ws) {
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int incl = wave_incl_scan(x);
   if (lane == 63) ws[wave] = incl;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int w = 0; w < kBalThreads / 64; ++w) {
-      const int q = ws[w];
-      ws[w] = run;
-      run += q;
-    }
-    ws[kBalThreads / 64] = run;
+  int below = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kBalThreads / 64; ++w) {
+    const int q = ws[w];
+    below += w < wave ? q : 0;
+    all += q;
   }
-  __syncthreads();
-  const int r = incl - x + ws[wave];
-  *tot = ws[kBalThreads / 64];
-  __syncthreads();
-  return r;
+  *tot = all;
+  return incl - x + below;
 }
 
 // Ascending bitonic sort of a[0, P) in LDS, P a power of two in
