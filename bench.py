@@ -465,18 +465,20 @@ def run_serve(args, gk, torch):
     srv.close()
     m_done = sum(pool[p].m for p in picks)
     # Alone vs in the mix: the sampled requests, each partitioned alone.
-    exact = True
+    exact_alone = True
     with gk.Context(0) as ctx:
         for i, part in results.items():
             p1, _ = ctx.partition(pool[picks[i]], int(ks[i]))
-            exact &= bool(np.array_equal(p1, part))
+            exact_alone &= bool(np.array_equal(p1, part))
+    exact = exact_alone
     log(f"serve: {len(arrivals)} requests in {wall:.1f}s; p50 {s.p50_ms:.1f} ms p99 {s.p99_ms:.1f} ms; "
         f"{s.edges_per_s / 1e6:.1f} M edges/s; {len(results)} checked alone-vs-mix: {exact}")
     cpu = None
+    exact_oracle, ng = None, 0
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle
-        tc, mc, ng = 0.0, 0, 0
+        tc, mc = 0.0, 0
         exact_oracle = True
         for i in sorted(results):
             g = pool[picks[i]]
@@ -516,7 +518,8 @@ def run_serve(args, gk, torch):
         "engine_steps": int(s.steps),
         "footprint_peak_bytes": int(s.footprint_peak),
         "device_peak_bytes": int(s.device_peak),
-        "bit_exact_alone_vs_mix": exact,
+        "bit_exact": {"alone_vs_mix": exact_alone, "alone_checked": len(results), "vs_oracle_sample": exact_oracle,
+                      "oracle_sample_requests": ng},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
