@@ -95,18 +95,21 @@ def test_match_group_cluster_contract(gk, oracle, ctx, graph):
         assert np.array_equal(host(cmap), ocmap)
         assert np.array_equal(host(cvw)[:onc], ocvw)
 
-    _, ocmap, _, onc = oracle.cluster(olabel, ostamp, g.vwgt, 6)
-    cx = torch.zeros(onc + 1, dtype=torch.int32, device="cuda")
-    ca = torch.zeros(max(g.nnz, 1), dtype=torch.int32, device="cuda")
-    cw = torch.zeros_like(ca)
-    cnnz = ctypes.c_int32(0)
-    assert L.gk_k_contract(ctx.handle, dg.ref(), _I(dev(ocmap)), onc, _I(cx), _I(ca), _I(cw),
-                           ctypes.cast(ctypes.pointer(cnnz), gk.I32P), None) == 0
-    ox, oa, ow = oracle.contract(g, ocmap, onc)
-    assert cnnz.value == len(oa)
-    assert np.array_equal(host(cx), ox)
-    assert np.array_equal(host(ca)[: len(oa)], oa)
-    assert np.array_equal(host(cw)[: len(ow)], ow)
+    # contraction of the reference's groups of at most 6, and of groups of up
+    # to 64 members (max_group = 64: up to 64 members per coarse row)
+    for G in (6, 64):
+        _, ocmap, _, onc = oracle.cluster(olabel, ostamp, g.vwgt, G)
+        cx = torch.zeros(onc + 1, dtype=torch.int32, device="cuda")
+        ca = torch.zeros(max(g.nnz, 1), dtype=torch.int32, device="cuda")
+        cw = torch.zeros_like(ca)
+        cnnz = ctypes.c_int32(0)
+        assert L.gk_k_contract(ctx.handle, dg.ref(), _I(dev(ocmap)), onc, _I(cx), _I(ca), _I(cw),
+                               ctypes.cast(ctypes.pointer(cnnz), gk.I32P), None) == 0
+        ox, oa, ow = oracle.contract(g, ocmap, onc)
+        assert cnnz.value == len(oa), G
+        assert np.array_equal(host(cx), ox), G
+        assert np.array_equal(host(ca)[: len(oa)], oa), G
+        assert np.array_equal(host(cw)[: len(ow)], ow), G
 
 
 def test_contract_big_rows(gk, oracle, ctx):
