@@ -14,11 +14,14 @@ packed CSR to device-resident partitions.  Beside it, on the same graphs:
     bounded sample (which is also the CPU baseline, with the host named);
   * nested, the plain path on the 2-D Delaunay triangulation of 2^22 points
     (k = 64), with its host-to-host window and an oracle check, once with
-    the points numbered in random order and once along a Hilbert curve.
+    the points numbered in random order and once along a Hilbert curve;
+  * nested, the serving form under Poisson arrivals at 200 requests/s for
+    60 s (500 distinct graphs, n log-uniform in [1e3, 2e6]): p50/p99 latency
+    and sustained edges/s, sampled requests checked alone and by the oracle.
 `value` is 0 unless every result is bit-exact, deterministic and within the
 3% cap.
 
-  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-c2]
+  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-c2] [--no-serve]
   python bench.py --form plain [--workload delaunay|dag|road] [--n N] [--k K]
   python bench.py --form serve [--rate R] [--duration S] [--pool P]
 """
@@ -373,6 +376,12 @@ def run_batch(args, gk, torch, nested_c2=True):
         # about n / 3 for random numbering).
         c2.workload = "delaunay_hilbert"
         out["plain_config2_hilbert"] = run_plain(c2, gk, torch, emit=False)
+    if not args.no_serve:
+        # Config 5: the serving form under Poisson arrivals at 200 requests/s
+        # for 60 s, requests drawn from 500 distinct graphs.
+        c5 = argparse.Namespace(**vars(args))
+        c5.seed, c5.rate, c5.duration, c5.pool, c5.check = 5, 200.0, 60.0, 500, 24
+        out["serve_config5"] = run_serve(c5, gk, torch, emit=False)
     print(json.dumps(out), flush=True)
 
 
@@ -412,7 +421,7 @@ def serve_pool(gk, count, seed, lo=1000, hi=2_000_000):
         return list(ex.map(make, range(count)))
 
 
-def run_serve(args, gk, torch):
+def run_serve(args, gk, torch, emit=True):
     """Serving form: open-loop Poisson arrivals into gk_serve_*; each request
     is a graph drawn from a pool of distinct graphs with its own k."""
     import threading
@@ -522,7 +531,9 @@ def run_serve(args, gk, torch):
                       "oracle_sample_requests": ng},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def run_plain(args, gk, torch, emit=True):
@@ -676,6 +687,7 @@ def main():
     ap.add_argument("--plain-reps", type=int, default=3, help="batch form: timed plain-path calls per graph")
     ap.add_argument("--plain-sample", type=int, default=256, help="batch form: graphs offered to the oracle")
     ap.add_argument("--no-c2", action="store_true", help="batch form: skip the nested 2^22-point Delaunay runs")
+    ap.add_argument("--no-serve", action="store_true", help="batch form: skip the nested 60 s serving run")
     ap.add_argument("--c2-steps", type=int, default=10, help="batch form: timed steps of the nested plain run")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="batch/serve forms: seconds of oracle work")
     ap.add_argument("--rate", type=float, default=200.0, help="serve form: Poisson arrival rate (requests/s)")
