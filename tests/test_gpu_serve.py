@@ -98,6 +98,24 @@ def test_serve_rejects_other_shape(gk):
             srv.submit(g, 4, max_moves=256)
 
 
+def test_serve_submit_many_all_or_none(gk, ctx):
+    """A burst is validated before anything is queued: one bad request (k = 0)
+    rejects the burst and queues none of it; a good burst is served in
+    order, each result equal to its graph partitioned alone."""
+    gs = [gk.delaunay(n, 40 + i) for i, n in enumerate((1500, 3000, 2200))]
+    with gk.Server() as srv:
+        with pytest.raises(RuntimeError, match="bad params"):
+            srv.submit_many(gs, [8, 0, 16])
+        assert srv.stats().submitted == 0
+        ids = srv.submit_many(gs, [8, 4, 16])
+        assert ids == sorted(ids) and len(set(ids)) == 3
+        res = [srv.wait(r) for r in ids]
+        s = srv.stats()
+    assert s.submitted == 3 and s.completed == 3 and s.failed == 0
+    for (part, st), g, k in zip(res, gs, (8, 4, 16)):
+        check_alone(ctx, g, k, part, st)
+
+
 # Failures stay with the request that causes them.  A device pool limit below
 # what the large graph needs makes its allocations fail (admission at 0.9 x
 # its CSR bytes, its first coarsening step at 1.3 x) while several requests
