@@ -142,3 +142,13 @@ def test_dag_2m_k256(gk, oracle, ctx):
     print(f"round paths (top-M select, 64-bit keys, deferred tests): {paths}", flush=True)
     assert paths[0] > 1000
     assert st.max_part_wgt <= st.cap
+
+
+# Group sizes other than the reference's 6: pairs only, and groups of up to 64
+# members (coarse rows gathering up to 64 member rows, many of them over 64
+# entries), each level checked against the oracle.
+@pytest.mark.parametrize("family,n,k,group", [("delaunay", 30000, 16, 2), ("delaunay", 30000, 16, 64),
+                                              ("dag", 30000, 32, 64), ("road", 40000, 8, 17)])
+def test_group_size(gk, oracle, ctx, family, n, k, group):
+    g = {"delaunay": lambda: gk.delaunay(n, 8), "dag": lambda: gk.dag(n, 40, 4), "road": lambda: gk.road(n, 3)}[family]()
+    compare(gk, oracle, ctx, g, k, max_group=group)
