@@ -420,15 +420,6 @@ inline int64_t overload(const int64_t pw[2], const Bal& b) {
   return std::max<int64_t>(0, pw[0] - b.max[0]) + std::max<int64_t>(0, pw[1] - b.max[1]);
 }
 
-// Cut of a 2-way partition.
-int64_t cut2(const G& g, const std::vector<uint8_t>& side) {
-  int64_t c = 0;
-  for (int32_t v = 0; v < g.n; ++v)
-    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
-      if (side[v] != side[g.adj[j]]) c += g.w[j];
-  return c / 2;
-}
-
 // Boundary FM refinement.  Moves that keep both sides within their maxima
 // (or reduce the overload) are taken in max-gain order; the best prefix of a
 // pass (lowest (overload, cut)) is kept.
@@ -441,7 +432,10 @@ struct FmWork {
   Heap q[2];
 };
 
-void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int64_t& cut) {
+// conn0 / inc (optional): every vertex's edge weight to side 0 and in all,
+// as a growing run leaves them; the degrees then come in O(n).
+void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int64_t& cut,
+         const int64_t* conn0 = nullptr, const int64_t* inc = nullptr) {
   const int32_t n = g.n;
   thread_local FmWork W;
   std::vector<int64_t>& id = W.id;
@@ -457,16 +451,24 @@ void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int
   const int32_t limit = std::max<int32_t>(25, std::min<int32_t>(n / 20, 300));
   // Internal / external degrees: computed once, then kept exact through the
   // moves of a pass and their rollback.
-  for (int32_t v = 0; v < n; ++v) {
-    const uint8_t sv = side[v];
-    int64_t a = 0, t = 0;
-    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
-      const int32_t wj = g.w[j];
-      t += wj;
-      a += side[g.adj[j]] == sv ? wj : 0;
+  if (conn0) {
+    for (int32_t v = 0; v < n; ++v) {
+      const int64_t a = side[v] == 0 ? conn0[v] : inc[v] - conn0[v];
+      id[v] = a;
+      ed[v] = inc[v] - a;
     }
-    id[v] = a;
-    ed[v] = t - a;
+  } else {
+    for (int32_t v = 0; v < n; ++v) {
+      const uint8_t sv = side[v];
+      int64_t a = 0, t = 0;
+      for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+        const int32_t wj = g.w[j];
+        t += wj;
+        a += side[g.adj[j]] == sv ? wj : 0;
+      }
+      id[v] = a;
+      ed[v] = t - a;
+    }
   }
   for (int pass = 0; pass < passes; ++pass) {
     q[0].reset(n);
@@ -571,7 +573,8 @@ struct GrowWork {
 
 // Greedy graph growing from `seedv`: side 0 grows by max gain until it
 // reaches its target weight.  Returns the cut of the result (each added
-// vertex turns its edges to side 0 internal and the rest cut).
+// vertex turns its edges to side 0 internal and the rest cut).  gw.conn0
+// ends as every vertex's edge weight to side 0.
 int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vector<uint8_t>& side) {
   const int32_t n = g.n;
   side.assign(n, 1);
@@ -588,8 +591,8 @@ int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vect
     cut += inc[v] - 2 * conn0[v];
     for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
       int32_t u = g.adj[j];
-      if (side[u] == 0) continue;
       conn0[u] += g.w[j];
+      if (side[u] == 0) continue;
       // gain of moving u to side 0: 2*conn0 - total incident weight
       int64_t gain = 2 * conn0[u] - inc[u];
       if (q.contains(u)) q.update(u, gain); else q.push(u, gain);
@@ -612,8 +615,9 @@ int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vect
   return cut;
 }
 
-// Multilevel bisection of g.  target0 = desired weight of side 0.
-void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vector<uint8_t>& side, bool par) {
+// Multilevel bisection of g.  target0 = desired weight of side 0.  Returns
+// the cut of the result (tracked exactly through growth, FM and projection).
+int64_t ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vector<uint8_t>& side, bool par) {
   Rng rng(seed);
   const int64_t t1 = g.tw - target0;
   Bal bal;
@@ -641,7 +645,7 @@ void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vect
   for (int t = 0; t < trials; ++t) {
     int32_t s = static_cast<int32_t>(rng.below(static_cast<uint32_t>(std::max(1, cg.n))));
     int64_t c = grow(cg, gw, s, target0, trial);
-    fm2(cg, trial, bal, 4, c);
+    fm2(cg, trial, bal, 4, c, gw.conn0.data(), gw.inc.data());
     int64_t pw[2] = {0, 0};
     for (int32_t v = 0; v < cg.n; ++v) pw[trial[v]] += cg.vw[v];
     int64_t ol = overload(pw, bal);
@@ -659,6 +663,7 @@ void ml_bisect(const G& g, int64_t target0, double tol, uint64_t seed, std::vect
     fm2(fine, side, bal, GKIP_FMPASS, c);
     best_cut = c;
   }
+  return best_cut;
 }
 
 // Induced subgraph of the vertices with side[v] == s.
@@ -721,14 +726,13 @@ void rb(const G& g, int32_t k, int32_t part0, int depth, RBCtx& ctx, int32_t* ou
   std::vector<std::vector<uint8_t>> trial(ncuts);
   std::vector<int64_t> key_ol(ncuts), key_cut(ncuts);
   auto run_trial = [&](int c) {
-    ml_bisect(g, target0, ctx.tol, s + 0x51ED2701ull * uint64_t(c), trial[c], ctx.parallel);
+    key_cut[c] = ml_bisect(g, target0, ctx.tol, s + 0x51ED2701ull * uint64_t(c), trial[c], ctx.parallel);
     int64_t pw0 = 0;
     for (int32_t v = 0; v < g.n; ++v) if (trial[c][v] == 0) pw0 += g.vw[v];
     const int64_t t1 = g.tw - target0;
     const int64_t m0 = static_cast<int64_t>(std::floor(double(target0) * (1.0 + ctx.tol)));
     const int64_t m1 = static_cast<int64_t>(std::floor(double(t1) * (1.0 + ctx.tol)));
     key_ol[c] = std::max<int64_t>(0, pw0 - m0) + std::max<int64_t>(0, (g.tw - pw0) - m1);
-    key_cut[c] = cut2(g, trial[c]);
   };
   if (ctx.parallel && g.n > GKIP_SPAWN_MIN) {
     std::vector<std::shared_ptr<ForkJoin::Task>> ts;
