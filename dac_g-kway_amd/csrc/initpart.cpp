@@ -561,13 +561,19 @@ void fm2(const G& g, std::vector<uint8_t>& side, const Bal& bal, int passes, int
 }
 
 // Buffers of the growing trials of one bisection: the incident edge weight
-// of every vertex (computed once), side-0 connectivity and the heap.
+// of every vertex (computed once), side-0 connectivity, room for one row's
+// side-1 neighbours and the queue.
 struct GrowWork {
   std::vector<int64_t> inc, conn0;
+  std::vector<int32_t> front;  // side-1 neighbours of the vertex being added
   Heap q;
   explicit GrowWork(const G& g) : inc(g.n, 0) {
-    for (int32_t u = 0; u < g.n; ++u)
+    int32_t maxdeg = 1;
+    for (int32_t u = 0; u < g.n; ++u) {
       for (int32_t k = g.xadj[u]; k < g.xadj[u + 1]; ++k) inc[u] += g.w[k];
+      maxdeg = std::max(maxdeg, g.xadj[u + 1] - g.xadj[u]);
+    }
+    front.resize(maxdeg);
   }
 };
 
@@ -577,11 +583,12 @@ struct GrowWork {
 // ends as every vertex's edge weight to side 0.
 int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vector<uint8_t>& side) {
   const int32_t n = g.n;
+  Heap& q = gw.q;
   side.assign(n, 1);
   std::vector<int64_t>& conn0 = gw.conn0;
   conn0.assign(n, 0);
   const std::vector<int64_t>& inc = gw.inc;
-  Heap& q = gw.q;
+  int32_t* front = gw.front.data();
   q.reset(n);
   int64_t w0 = 0, cut = 0;
   int32_t next_unreached = 0;
@@ -589,12 +596,19 @@ int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vect
     side[v] = 0;
     w0 += g.vw[v];
     cut += inc[v] - 2 * conn0[v];
+    // connectivity of every neighbour; the side-1 ones are listed without a
+    // branch and their queue keys raised after
+    int32_t nf = 0;
     for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
-      int32_t u = g.adj[j];
+      const int32_t u = g.adj[j];
       conn0[u] += g.w[j];
-      if (side[u] == 0) continue;
+      front[nf] = u;
+      nf += side[u] != 0;
+    }
+    for (int32_t i = 0; i < nf; ++i) {
+      const int32_t u = front[i];
       // gain of moving u to side 0: 2*conn0 - total incident weight
-      int64_t gain = 2 * conn0[u] - inc[u];
+      const int64_t gain = 2 * conn0[u] - inc[u];
       if (q.contains(u)) q.update(u, gain); else q.push(u, gain);
     }
   };
@@ -614,6 +628,7 @@ int64_t grow(const G& g, GrowWork& gw, int32_t seedv, int64_t target0, std::vect
   }
   return cut;
 }
+
 
 // Multilevel bisection of g.  target0 = desired weight of side 0.  Returns
 // the cut of the result (tracked exactly through growth, FM and projection).
