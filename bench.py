@@ -707,8 +707,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(graph-level sharding across GPUs is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
+                          "(graph-level sharding across GPUs is not implemented)"}))
         sys.exit(2)
 
     gk = load_pkg()

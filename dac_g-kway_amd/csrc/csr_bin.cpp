@@ -1,7 +1,7 @@
 // Binary CSR files: the input step without text parsing.  The reference reads
 // only METIS text (gkway/graph.hpp:70-152); parsing the config-2 graph takes
-// seconds, and a server re-reading graphs should not pay it.  This format is
-// this build's own.
+// seconds, and a server re-reading graphs should not pay it.  This format has
+// no counterpart in the reference.
 //
 // Layout (little-endian):
 //   bytes 0..63   header: magic "GKCSR\x01\0\0", n (int64), nnz (int64),

@@ -2,8 +2,8 @@
 //
 // The reference hands its coarsest graph to METIS_PartGraphKway
 // (gkway/metis_partition.hpp:46-47).  The METIS it uses exists only as a
-// prebuilt binary, so this build supplies its own deterministic partitioner
-// for that step:
+// prebuilt binary, so this library supplies its own deterministic
+// partitioner for that step:
 //   multilevel recursive bisection
 //     - coarsening by sequential heavy-edge matching in a seeded random order,
 //     - initial bisection by greedy graph growing from several seeds,

@@ -290,7 +290,7 @@ int gk_graph_read(const char* path, gk_csr* out);
 void gk_graph_free(gk_csr* g);
 int gk_graph_write(const char* path, const gk_csr* g);
 int gk_graph_check(const gk_csr* g, int check_symmetry);
-/* Binary CSR files (this build's format, no counterpart in the reference,
+/* Binary CSR files (this library's own format, no counterpart in the reference,
    whose only input is METIS text, gkway/graph.hpp:70-152): a 64-byte header
    and 64-byte-aligned int32 sections, so a graph is mapped instead of parsed.
    gk_graph_write_bin writes one; gk_graph_map_bin maps one read-only and

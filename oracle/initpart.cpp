@@ -3,7 +3,7 @@
 //
 // The reference hands its coarsest graph to METIS_PartGraphKway
 // (gkway/metis_partition.hpp:46-47), which exists there only as a prebuilt
-// library that this build may not load.  The product therefore supplies its
+// library that may not be loaded here.  The product therefore supplies its
 // own partitioner for that step (libgkway.so, gk_initpart), and this file
 // restates its rules one at a time, single-threaded and without the
 // product's data structures (ordered sets instead of indexed heaps, no

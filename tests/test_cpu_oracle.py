@@ -299,8 +299,9 @@ def test_partition_invariants_and_determinism(gk, oracle):
 
 
 # Cuts of METIS 5.2.1 (METIS_PartGraphKway, defaults) on the reference's
-# sample graph simple_test/delaunay_n11.graph, as recorded in the survey of
-# the reference; the partitioner must stay within 5% of them.
+# sample graph simple_test/delaunay_n11.graph (figures from a one-time probe
+# of that library, which is not run here); the partitioner must stay within 5%
+# of them.
 METIS_N11 = {2: 91, 4: 205, 8: 349, 64: 2083}
 
 
