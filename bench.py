@@ -79,8 +79,10 @@ def stage_table(gk, stats_list, steps, log_it=True):
 
 def launch_flags(gk, args):
     """Flags that choose launch mechanics only (results are identical): rounds
-    replayed from a hipGraph, launch shapes picked by the autotuner."""
-    return (gk.GK_FLAG_GRAPH_ROUNDS if args.graph_rounds else 0) | (gk.GK_FLAG_AUTOTUNE if args.autotune else 0)
+    replayed from a hipGraph, launch shapes picked by the autotuner,
+    persistent work-stealing launches."""
+    return ((gk.GK_FLAG_GRAPH_ROUNDS if args.graph_rounds else 0) | (gk.GK_FLAG_AUTOTUNE if args.autotune else 0)
+            | (gk.GK_FLAG_PERSISTENT if args.persistent else 0))
 
 
 def dominant_stage(gk, L, ctx, stage_step, off, steps=2):
@@ -352,7 +354,7 @@ def run_batch(args, gk, torch, nested_c2=True):
                    "n_total": int(pg.n), "m_total": int(m_tot), "eps": 0.03,
                    "window": "device packed CSR -> device partitions",
                    "rounds": "hipGraph replay" if args.graph_rounds else "stream launches",
-                   "autotune": bool(args.autotune)},
+                   "autotune": bool(args.autotune), "persistent": bool(args.persistent)},
         "plain": plain,
         "speedup_vs_plain": round(value / plain["edges_per_s"], 2) if plain else None,
         "host_to_host_edges_per_s": round(m_tot / min(h2h), 1),
@@ -653,7 +655,7 @@ def run_plain(args, gk, torch, emit=True):
         "config": {"workload": f"{args.workload} n={g.n} m={m} seed={args.seed}", "k": args.k,
                    "eps": 0.03, "window": "device CSR -> device partition",
                    "rounds": "hipGraph replay" if args.graph_rounds else "stream launches",
-                   "autotune": bool(args.autotune)},
+                   "autotune": bool(args.autotune), "persistent": bool(args.persistent)},
         "host_to_host_ms": round(min(h2h) * 1e3, 3),
         "host_to_host_edges_per_s": round(m / min(h2h), 1),
         "cut": int(st.cut),
@@ -700,6 +702,8 @@ def main():
                     help="replay each level's refinement rounds from a captured hipGraph")
     ap.add_argument("--autotune", action="store_true",
                     help="time candidate launch shapes of matching and contraction on first use")
+    ap.add_argument("--persistent", action="store_true",
+                    help="run matching, contraction rows and the gain pass as persistent work-stealing launches")
     ap.add_argument("--stage-steps", type=int, default=3,
                     help="extra steps with per-stage HIP events for the stage table and roofline")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -54,6 +54,7 @@ class LevelStats(Structure):
 GK_FLAG_STAGE_TIME = 4
 GK_FLAG_GRAPH_ROUNDS = 8
 GK_FLAG_AUTOTUNE = 16
+GK_FLAG_PERSISTENT = 32
 GK_ST_COUNT = 16
 STAGE_NAMES = ["match", "group", "sort", "cluster", "contract", "contract_big", "project", "gain",
                "select", "msort", "balance", "apply", "cut", "gain_heavy", "contract_aux", "pack"]

@@ -7,6 +7,7 @@
 // discarded by a later sort.
 #include "kernels.hpp"
 #include "prim.hpp"
+#include "steal.hpp"
 #include "../../include/gkway.h"
 
 namespace gkd {
@@ -28,11 +29,11 @@ static inline int hip_status() { return hipGetLastError() == hipSuccess ? 0 : GK
 // matching_with_score (gkway/coarsen.hpp:39-78) with S lanes per vertex: the
 // lanes stride over the row, then reduce the packed preference
 // (weight desc, neighbour degree asc, neighbour id asc) by xor-shuffles.
-template <int S, int B>
-__global__ void __launch_bounds__(B) k_match(int32_t n, const int32_t* __restrict__ xadj,
-                                             const int32_t* __restrict__ adj,
-                                             const int32_t* __restrict__ w, int32_t* __restrict__ cand) {
-  const int64_t gt = int64_t(blockIdx.x) * B + threadIdx.x;
+// Global thread slot gt serves vertex gt / S as lane gt % S of its group.
+template <int S>
+__device__ __forceinline__ void match_slot(int64_t gt, int32_t n, const int32_t* __restrict__ xadj,
+                                           const int32_t* __restrict__ adj, const int32_t* __restrict__ w,
+                                           int32_t* __restrict__ cand) {
   const int32_t v = static_cast<int32_t>(gt / S);
   const int sub = static_cast<int>(gt % S);
   if (v >= n) return;
@@ -52,26 +53,56 @@ __global__ void __launch_bounds__(B) k_match(int32_t n, const int32_t* __restric
   if (sub == 0) cand[v] = bu;
 }
 
+template <int S, int B>
+__global__ void __launch_bounds__(B) k_match(int32_t n, const int32_t* __restrict__ xadj,
+                                             const int32_t* __restrict__ adj,
+                                             const int32_t* __restrict__ w, int32_t* __restrict__ cand) {
+  match_slot<S>(int64_t(blockIdx.x) * B + threadIdx.x, n, xadj, adj, w, cand);
+}
+
+// The same slots, taken from the work-stealing queues.
+template <int S, int B>
+__global__ void __launch_bounds__(B) k_match_ws(int32_t n, const int32_t* __restrict__ xadj,
+                                                const int32_t* __restrict__ adj,
+                                                const int32_t* __restrict__ w, int32_t* __restrict__ cand,
+                                                unsigned long long* __restrict__ ctr, StealShape sh) {
+  const int64_t sweeps = (int64_t(n) * S + 63) / 64;
+  steal_sweeps(ctr, sweeps, sh, [&](int64_t sw) { match_slot<S>(sw * 64 + lane_id(), n, xadj, adj, w, cand); });
+}
+
+template <int S, int B>
+static void match_go(const DGraph& g, int32_t* cand, const Steal* ws, hipStream_t st) {
+  const unsigned grid = grid_for(int64_t(g.n) * S, B);
+  if (!ws) {
+    gk_launch(k_match<S, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand);
+    return;
+  }
+  const unsigned pg = steal_grid(reinterpret_cast<const void*>(&k_match_ws<S, B>), B, *ws, grid);
+  const StealShape sh =
+      steal_shape((int64_t(g.n) * S + 63) / 64, int64_t(pg) * (B / 64));
+  fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
+  gk_launch(k_match_ws<S, B>, dim3(pg), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand, ws->ctr, sh);
+}
+
 template <int B>
-static void match_block(const DGraph& g, int lanes, int32_t* cand, hipStream_t st) {
-  const unsigned grid = grid_for(int64_t(g.n) * lanes, B);
+static void match_block(const DGraph& g, int lanes, int32_t* cand, const Steal* ws, hipStream_t st) {
   switch (lanes) {
-    case 1: gk_launch(k_match<1, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
-    case 2: gk_launch(k_match<2, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
-    case 4: gk_launch(k_match<4, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
-    case 8: gk_launch(k_match<8, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
-    case 16: gk_launch(k_match<16, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
-    case 32: gk_launch(k_match<32, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
-    default: gk_launch(k_match<64, B>, dim3(grid), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand); break;
+    case 1: match_go<1, B>(g, cand, ws, st); break;
+    case 2: match_go<2, B>(g, cand, ws, st); break;
+    case 4: match_go<4, B>(g, cand, ws, st); break;
+    case 8: match_go<8, B>(g, cand, ws, st); break;
+    case 16: match_go<16, B>(g, cand, ws, st); break;
+    case 32: match_go<32, B>(g, cand, ws, st); break;
+    default: match_go<64, B>(g, cand, ws, st); break;
   }
 }
 
-int launch_match(const DGraph& g, int lanes, int block, int32_t* cand, hipStream_t st) {
+int launch_match(const DGraph& g, int lanes, int block, int32_t* cand, hipStream_t st, const Steal* ws) {
   if (g.n <= 0) return 0;
   switch (block) {
-    case 128: match_block<128>(g, lanes, cand, st); break;
-    case 512: match_block<512>(g, lanes, cand, st); break;
-    default: match_block<256>(g, lanes, cand, st); break;
+    case 128: match_block<128>(g, lanes, cand, ws, st); break;
+    case 512: match_block<512>(g, lanes, cand, ws, st); break;
+    default: match_block<256>(g, lanes, cand, ws, st); break;
   }
   return hip_status();
 }
@@ -424,22 +455,19 @@ __global__ void __launch_bounds__(B) k_contract_rows_wide(const int32_t* __restr
 //      compacted with a ballot.
 // The dependent loads per entry: the row's list slot (id and member range),
 // the members' row bounds, the entry, its coarse id.
-template <int S, int B>
-__global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__ rows,
-                                                       const int2* __restrict__ desc, int32_t nrows,
-                                                       const int2* __restrict__ mrow,
-                                                       const int32_t* __restrict__ adj,
-                                                       const int32_t* __restrict__ w,
-                                                       const int32_t* __restrict__ cmap,
-                                                       const int32_t* __restrict__ toff,
-                                                       int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
-                                                       int32_t* __restrict__ len) {
+// Global thread slot gt serves row gt / S as lane gt % S of its segment; wm
+// is the wave's 64-slot LDS mark array.  The whole wave calls it together.
+template <int S>
+__device__ __forceinline__ void contract_row_slot(int64_t gt, int32_t* wm, const int32_t* __restrict__ rows,
+                                                  const int2* __restrict__ desc, int32_t nrows,
+                                                  const int2* __restrict__ mrow, const int32_t* __restrict__ adj,
+                                                  const int32_t* __restrict__ w, const int32_t* __restrict__ cmap,
+                                                  const int32_t* __restrict__ toff, int32_t* __restrict__ tadj,
+                                                  int32_t* __restrict__ tw, int32_t* __restrict__ len) {
   static_assert(S == 16 || S == 32 || S == 64, "segment of 16, 32 or 64 lanes");
-  static_assert(B % 64 == 0, "whole waves");
   constexpr int LB = S == 16 ? 4 : (S == 32 ? 5 : 6);
   constexpr uint32_t kNone = 0xffffffffu;
-  __shared__ int32_t mark[B];
-  const int64_t ri = (int64_t(blockIdx.x) * B + threadIdx.x) >> LB;
+  const int64_t ri = gt >> LB;
   const int lane = lane_id();
   const int sub = lane & (S - 1);
   const int base = lane - sub;
@@ -465,7 +493,6 @@ __global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__
     const int32_t inc = seg_incl_scan<S>(md);
     const int32_t e = inc - md;
     const int32_t tot = __shfl(inc, base + S - 1, 64);
-    int32_t* wm = mark + (threadIdx.x - lane);  // this wave's 64 slots
     wm[lane] = INT32_MIN;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -513,18 +540,73 @@ __global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__
   if (live && sub == 0) len[c] = popc64(tails & segm);
 }
 
+template <int S, int B>
+__global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__ rows,
+                                                       const int2* __restrict__ desc, int32_t nrows,
+                                                       const int2* __restrict__ mrow,
+                                                       const int32_t* __restrict__ adj,
+                                                       const int32_t* __restrict__ w,
+                                                       const int32_t* __restrict__ cmap,
+                                                       const int32_t* __restrict__ toff,
+                                                       int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
+                                                       int32_t* __restrict__ len) {
+  static_assert(B % 64 == 0, "whole waves");
+  __shared__ int32_t mark[B];
+  contract_row_slot<S>(int64_t(blockIdx.x) * B + threadIdx.x, mark + (threadIdx.x & ~63u), rows, desc, nrows, mrow,
+                       adj, w, cmap, toff, tadj, tw, len);
+}
+
+// The same slots, taken from the work-stealing queues.
+template <int S, int B>
+__global__ void __launch_bounds__(B) k_contract_rows_ws(const int32_t* __restrict__ rows,
+                                                          const int2* __restrict__ desc, int32_t nrows,
+                                                          const int2* __restrict__ mrow,
+                                                          const int32_t* __restrict__ adj,
+                                                          const int32_t* __restrict__ w,
+                                                          const int32_t* __restrict__ cmap,
+                                                          const int32_t* __restrict__ toff,
+                                                          int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
+                                                          int32_t* __restrict__ len,
+                                                          unsigned long long* __restrict__ ctr, StealShape sh) {
+  static_assert(B % 64 == 0, "whole waves");
+  __shared__ int32_t mark[B];
+  int32_t* wm = mark + (threadIdx.x & ~63u);
+  const int64_t sweeps = (int64_t(nrows) * S + 63) / 64;
+  steal_sweeps(ctr, sweeps, sh, [&](int64_t sw) {
+    contract_row_slot<S>(sw * 64 + lane_id(), wm, rows, desc, nrows, mrow, adj, w, cmap, toff, tadj, tw, len);
+  });
+}
+
 // Largest coarse id count for k_contract_rows<64, B> (ids below 2^26 - 1);
 // the 16- and 32-lane kernels allow more, one limit keeps the choice simple.
 constexpr int32_t kNarrowIdsMax = (1 << 26) - 1;
 
+// Wide rows (coarse graphs of 2^26 - 1 vertices or more) always take the
+// direct launch; the narrow ones may take the work-stealing form.
 template <int B, bool Wide>
 static void contract_rows_block(const DGraph& g, int lanes, const int32_t* rows, const int2* desc, int32_t nrows,
                                 const int2* mrow, const int32_t* cmap, const int32_t* toff, int32_t* tadj,
-                                int32_t* tw, int32_t* len, hipStream_t st) {
+                                int32_t* tw, int32_t* len, const Steal* ws, hipStream_t st) {
   const unsigned grid = grid_for(int64_t(nrows) * lanes, B);
   auto go = [&](auto kern) {
     gk_launch(kern, dim3(grid), dim3(B), 0, st, rows, desc, nrows, mrow, g.adj, g.w, cmap, toff, tadj, tw, len);
   };
+  auto go_ws = [&](auto kern, int S) {
+    const unsigned pg = steal_grid(reinterpret_cast<const void*>(kern), B, *ws, grid);
+    const StealShape sh =
+        steal_shape((int64_t(nrows) * S + 63) / 64, int64_t(pg) * (B / 64));
+    fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
+    gk_launch(kern, dim3(pg), dim3(B), 0, st, rows, desc, nrows, mrow, g.adj, g.w, cmap, toff, tadj, tw, len,
+              ws->ctr, sh);
+  };
+  if (!Wide && ws) {
+    switch (lanes) {
+      case 16: go_ws(k_contract_rows_ws<16, B>, 16); break;
+      case 32: go_ws(k_contract_rows_ws<32, B>, 32); break;
+      default: go_ws(k_contract_rows_ws<64, B>, 64); break;
+    }
+    return;
+  }
   switch (lanes) {
     case 16: Wide ? go(k_contract_rows_wide<16, B>) : go(k_contract_rows<16, B>); break;
     case 32: Wide ? go(k_contract_rows_wide<32, B>) : go(k_contract_rows<32, B>); break;
@@ -535,23 +617,24 @@ static void contract_rows_block(const DGraph& g, int lanes, const int32_t* rows,
 template <bool Wide>
 static void contract_rows_any(const DGraph& g, int lanes, int block, const int32_t* rows, const int2* desc,
                               int32_t nrows, const int2* mrow, const int32_t* cmap, const int32_t* toff,
-                              int32_t* tadj, int32_t* tw, int32_t* len, hipStream_t st) {
+                              int32_t* tadj, int32_t* tw, int32_t* len, const Steal* ws, hipStream_t st) {
   switch (block) {
-    case 64: contract_rows_block<64, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
-    case 128: contract_rows_block<128, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
-    case 512: contract_rows_block<512, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
-    default: contract_rows_block<256, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st); break;
+    case 64: contract_rows_block<64, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, ws, st); break;
+    case 128: contract_rows_block<128, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, ws, st); break;
+    case 512: contract_rows_block<512, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, ws, st); break;
+    default: contract_rows_block<256, Wide>(g, lanes, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, ws, st); break;
   }
 }
 
 int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, const int2* desc,
                          int32_t nrows, int32_t nc, const int2* mrow, const int32_t* cmap, const int32_t* toff,
-                         int32_t* tadj, int32_t* tw, int32_t* len, bool force_wide, hipStream_t st) {
+                         int32_t* tadj, int32_t* tw, int32_t* len, bool force_wide, hipStream_t st,
+                         const Steal* ws) {
   if (nrows <= 0) return 0;
   if (force_wide || nc > kNarrowIdsMax)
-    contract_rows_any<true>(g, lanes, block, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st);
+    contract_rows_any<true>(g, lanes, block, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, nullptr, st);
   else
-    contract_rows_any<false>(g, lanes, block, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, st);
+    contract_rows_any<false>(g, lanes, block, rows, desc, nrows, mrow, cmap, toff, tadj, tw, len, ws, st);
   return hip_status();
 }
 

@@ -159,7 +159,7 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
       const int32_t* rows = lists.p + int64_t(cls) * nc;
       auto run = [&](const Tuner::Shape& x) {
         return gkd::launch_contract_rows(g, x.lanes, x.block, rows, desc.p + int64_t(cls) * nc, hc[cls], nc,
-                                         mrow.p, cmap, toff.p, tadj.p, tw.p, len.p, wide, st);
+                                         mrow.p, cmap, toff.p, tadj.p, tw.p, len.p, wide, st, steal_of(ctx));
       };
       Tuner::Shape shape{lanes[cls], 256};
       if (ctx->autotune && hc[cls] >= kTuneMinRows) {
@@ -234,7 +234,7 @@ int contract_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t nc, con
 int match_stage(gk_ctx* ctx, hipStream_t st, const DGraph& g, int32_t* cand) {
   const int def = gkd::lanes_for_degree(g.n ? double(g.nnz) / g.n : 0.0);
   Tuner::Shape shape{def, 256};
-  auto run = [&](const Tuner::Shape& x) { return gkd::launch_match(g, x.lanes, x.block, cand, st); };
+  auto run = [&](const Tuner::Shape& x) { return gkd::launch_match(g, x.lanes, x.block, cand, st, steal_of(ctx)); };
   if (ctx->autotune && g.nnz >= kTuneMinEntries) {
     const uint64_t key = Tuner::key(0, g.n, g.nnz);
     if (const Tuner::Shape* s = ctx->tuner.find(key)) {
@@ -274,15 +274,15 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
   std::memset(&S, 0, sizeof S);
   ctx->cur = &S;
   ctx->clock.start((prm.flags & GK_FLAG_STAGE_TIME) != 0, st);
-  ctx->autotune = (prm.flags & GK_FLAG_AUTOTUNE) != 0;
   struct Reset {
     gk_ctx* c;
     ~Reset() {
       c->cur = nullptr;
       c->clock.start(false, nullptr);
-      c->autotune = false;
+      gkrt::clear_launch_options(c);
     }
   } reset{ctx};
+  GK_TRY(gkrt::set_launch_options(ctx, prm.flags));
 
   std::vector<Level> L;
   L.reserve(GK_MAX_LEVELS + 1);
@@ -455,7 +455,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
     gkd::fill<long long>(dpaths.p, 0, 3, st);
     // gain statistics: a (vertices, entries) pair per block of the gain
     // launches (grids of at most kGainSlots blocks), summed at the end
-    constexpr int kGainSlots = 2048;
+    constexpr int kGainSlots = gkd::kGainMaxBlocks;
     GK_TRY(work.alloc(P, 2 * kGainSlots));
     gkd::fill<unsigned long long>(work.p, 0ull, 2 * kGainSlots, st);
     // the selection kernels' algorithmic bytes, likewise per block slot
@@ -527,7 +527,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
       GK_STAGE(ctx, GK_ST_PROJECT, 4LL * g.n, gkd::launch_compact(g.n, upd.p, wl.p, hcnt.p, st));
       GK_STAGE(ctx, GK_ST_GAIN, 0,
                gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
-                                   work.p, g.n, cand.p, st));
+                                   work.p, g.n, cand.p, st, steal_of(ctx)));
       if (heavy_rows)
         GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
                  gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p,
@@ -546,7 +546,7 @@ int run_vcycle(gk_ctx* ctx, hipStream_t st, const DGraph& g0in, const gk_params&
                                      dabytes.p + l, st));
         GK_STAGE(ctx, GK_ST_GAIN, 0,
                  gkd::launch_gain_wl(g, k, lanes, wl.p, hcnt.p, heavy.p, cur_part, upd.p, cur_bnd, gain.p, dest.p,
-                                     work.p, wl_items, cand.p, st));
+                                     work.p, wl_items, cand.p, st, steal_of(ctx)));
         if (heavy_rows)
           GK_STAGE(ctx, GK_ST_GAIN_HEAVY, 0,
                    gkd::launch_gain_heavy(g, k, heavy.p, hcnt.p, cur_part, upd.p, cur_bnd, gain.p, dest.p, work.p,

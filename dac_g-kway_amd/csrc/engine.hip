@@ -624,7 +624,7 @@ int Engine::refine_step(bool* worked) {
   // Gain-pass statistics: a (vertices, entries) pair per block of the gain
   // launches (grids of at most kGainSlots blocks), summed after the rounds
   // into the stage's algorithmic bytes.
-  constexpr int kGainSlots = 2048;
+  constexpr int kGainSlots = gkd::kGainMaxBlocks;
   Buf<unsigned long long> gwork;
   GK_TRY(gwork.alloc(P_, 2 * kGainSlots));
   gkd::fill<unsigned long long>(gwork.p, 0ull, 2 * kGainSlots, st_);
@@ -648,7 +648,7 @@ int Engine::refine_step(bool* worked) {
   GK_STAGE(ctx_, GK_ST_PROJECT, 4LL * N, gkd::launch_compact(N, upd.p, wl.p, hcnt.p, st_));
   GK_STAGE(ctx_, GK_ST_GAIN, 0,
            gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                               gwork.p, N, cand.p, st_));
+                               gwork.p, N, cand.p, st_, steal_of(ctx_)));
   if (heavy_rows)
     GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
              gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, gwork.p,
@@ -677,7 +677,7 @@ int Engine::refine_step(bool* worked) {
                                  lastcnt.p, active.p, cand.p, dthr.p, dpaths.p, nullptr, dab.p, st_));
     GK_STAGE(ctx_, GK_ST_GAIN, 0,
              gkd::launch_gain_wl(g, kmax, lanes, wl.p, hcnt.p, heavy.p, part->p, upd.p, bnd->p, gain.p, dest.p,
-                                 gwork.p, wl_items, cand.p, st_));
+                                 gwork.p, wl_items, cand.p, st_, steal_of(ctx_)));
     if (heavy_rows)
       GK_STAGE(ctx_, GK_ST_GAIN_HEAVY, 0,
                gkd::launch_gain_heavy(g, kmax, heavy.p, hcnt.p, part->p, upd.p, bnd->p, gain.p, dest.p, gwork.p,
@@ -924,15 +924,15 @@ int run_packed(gk_ctx* ctx, hipStream_t st, const gk_packed_csr* pk, const gk_pa
   std::memset(&SB, 0, sizeof SB);
   ctx->cur = &SB;
   ctx->clock.start((prm[0].flags & GK_FLAG_STAGE_TIME) != 0, st);
-  ctx->autotune = (prm[0].flags & GK_FLAG_AUTOTUNE) != 0;
   struct Reset {
     gk_ctx* c;
     ~Reset() {
       c->cur = nullptr;
       c->clock.start(false, nullptr);
-      c->autotune = false;
+      gkrt::clear_launch_options(c);
     }
   } reset{ctx};
+  GK_TRY(gkrt::set_launch_options(ctx, prm[0].flags));
   // Level 0 of every graph is a slice of the caller's pack.
   auto S0 = std::make_shared<Store>();
   S0->nj = G;

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "steal.hpp"
+
 namespace gkd {
 
 // Kernel-level stage timing.  While the host times a stage kernel by kernel
@@ -42,7 +44,9 @@ int lanes_for_degree(double avg_deg);
 // ---- coarsening (coarsen.hip)
 // lanes per vertex (1..64) and block size (128, 256 or 512 threads) change
 // only the launch shape, never the result.
-int launch_match(const DGraph& g, int lanes, int block, int32_t* cand, hipStream_t st);
+// ws (may be null): run as a persistent work-stealing launch (same result).
+int launch_match(const DGraph& g, int lanes, int block, int32_t* cand, hipStream_t st,
+                 const Steal* ws = nullptr);
 // Pointer jumping over the candidate pseudo-forest.
 int launch_jump_init(int32_t n, const int32_t* cand, int32_t* next, int32_t* dist, hipStream_t st);
 // One Jacobi round; sets flags[round] = 1 when any pointer moved.  Skips all
@@ -72,10 +76,12 @@ int launch_contract_bounds(const DGraph& g, int32_t nc, const int32_t* cptr, con
 // Phase 2 for the rows of one class (lanes in {16, 32, 64}); block size 64,
 // 128, 256 or 512 threads (launch shape only).  nc: coarse vertex count; the
 // sort packs the lane number beside the coarse id in 32 bits when nc allows,
-// else (or with force_wide) sorts (id, weight) pairs.
+// else (or with force_wide) sorts (id, weight) pairs.  ws (may be null):
+// the 32-bit-key rows run as a persistent work-stealing launch.
 int launch_contract_rows(const DGraph& g, int lanes, int block, const int32_t* rows, const int2* desc,
                          int32_t nrows, int32_t nc, const int2* mrow, const int32_t* cmap, const int32_t* toff,
-                         int32_t* tadj, int32_t* tw, int32_t* len, bool force_wide, hipStream_t st);
+                         int32_t* tadj, int32_t* tw, int32_t* len, bool force_wide, hipStream_t st,
+                         const Steal* ws = nullptr);
 // Phase 2 for big rows: gather to (row-rank, neighbour) keys.
 int launch_contract_big_gather(const DGraph& g, const int32_t* rows, int32_t nrows,
                                const int32_t* boff, const int32_t* cptr, const int32_t* order,
@@ -110,9 +116,13 @@ int launch_compact(int32_t n, const int32_t* flags, int32_t* list, int32_t* cnt,
 // and launch_gain_heavy.
 constexpr int32_t kGainInlineK = 256;
 inline bool gain_heavy_separate(int32_t k) { return k > kGainInlineK; }
+// Workgroups of a gain launch at most: work holds 2 * kGainMaxBlocks slots.
+constexpr unsigned kGainMaxBlocks = 2048;
+// ws (may be null): run as a persistent work-stealing launch (same result).
 int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
                    const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
-                   unsigned long long* work, int64_t max_items, uint64_t* cand, hipStream_t st);
+                   unsigned long long* work, int64_t max_items, uint64_t* cand, hipStream_t st,
+                   const Steal* ws = nullptr);
 // Rows collected in heavy[0..wlc[1]) by launch_gain_wl: one wave each, LDS
 // histogram of the k part connectivities.  Both gain kernels keep the optional
 // bitmask cand (bit v = boundary vertex with positive gain) in step.

@@ -106,31 +106,30 @@ __device__ __forceinline__ void set_cand(uint64_t* cand, int32_t v, bool on) {
 // histogram of the k connectivities (as k_gain_heavy); with larger k they are
 // appended to the heavy list (wlc[1] entries) for k_gain_heavy.  Processed
 // vertices get upd[v] = 0.
-template <int S>
-__global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
-                                                 int32_t k, const int32_t* __restrict__ xadj,
-                                                 const int32_t* __restrict__ adj,
-                                                 const int32_t* __restrict__ w,
-                                                 const int32_t* __restrict__ part,
-                                                 int32_t* __restrict__ upd, int32_t* __restrict__ bnd,
-                                                 int32_t* __restrict__ gain, int32_t* __restrict__ dest,
-                                                 int32_t* __restrict__ heavy,
-                                                 unsigned long long* __restrict__ work,
-                                                 uint64_t* __restrict__ cand) {
-  __shared__ int32_t hh[4][kGainInlineK];
+//
+// Direct form (Steal = false): group g of the grid (S lanes) takes worklist
+// entries g, g + nsg, ...; the loop runs while any group of the wave has an
+// entry, so the wave can take its long rows together.  Work-stealing form
+// (Steal = true, k_gain_wl_ws): the wave's passes of 64 / S entries are
+// taken from the queues at ctr.  Each entry is processed once
+// either way and the statistics are sums, so both give the same results.
+template <int S, bool Steal>
+__device__ __forceinline__ void gain_wl_body(const int32_t* __restrict__ wl, int32_t* __restrict__ wlc, int32_t k,
+                                             const int32_t* __restrict__ xadj, const int32_t* __restrict__ adj,
+                                             const int32_t* __restrict__ w, const int32_t* __restrict__ part,
+                                             int32_t* __restrict__ upd, int32_t* __restrict__ bnd,
+                                             int32_t* __restrict__ gain, int32_t* __restrict__ dest,
+                                             int32_t* __restrict__ heavy, unsigned long long* __restrict__ work,
+                                             uint64_t* __restrict__ cand, unsigned long long* __restrict__ ctr,
+                                             int guide, int tail, int32_t (*hh)[kGainInlineK], int (*ws)[4]) {
   const int32_t cnt = wlc[0];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
   const int sub = threadIdx.x & (S - 1);
   const bool inline_heavy = k <= kGainInlineK;
-  // Group g of the grid (S lanes) takes worklist entries g, g + nsg, ...;
-  // the loop runs while any group of the wave has an entry, so the wave can
-  // take its long rows together.
-  const int64_t nsg = int64_t(gridDim.x) * (256 / S);
-  const int64_t wave_g0 = (int64_t(blockIdx.x) * 4 + wv) * (64 / S);
   const int grp = lane / S;
   int done_v = 0, done_e = 0;
-  for (int64_t eb = wave_g0; eb < cnt; eb += nsg) {
+  auto pass = [&](int64_t eb) {
     const int64_t e = eb + grp;
     int32_t v = 0, s = 0, d = 0;
     if (e < cnt) {
@@ -180,7 +179,7 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
         done_e += d;
       }
     }
-    if (!inline_heavy) continue;
+    if (!inline_heavy) return;
     // The pass's long rows, one at a time, by the whole wave.
     int32_t* hist = hh[wv];
     uint64_t lm = ballot(long_row && sub == 0);
@@ -232,11 +231,19 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
+  };
+  if constexpr (Steal) {
+    const int64_t sweeps = (int64_t(cnt) + (64 / S) - 1) / (64 / S);
+    steal_sweeps(ctr, sweeps, steal_shape(sweeps, int64_t(gridDim.x) * 4, guide, tail),
+                 [&](int64_t sw) { pass(sw * (64 / S)); });
+  } else {
+    const int64_t nsg = int64_t(gridDim.x) * (256 / S);
+    const int64_t wave_g0 = (int64_t(blockIdx.x) * 4 + wv) * (64 / S);
+    for (int64_t eb = wave_g0; eb < cnt; eb += nsg) pass(eb);
   }
   if (work) {
     // Statistics: one slot pair per block (no two blocks of a launch share
     // one, launches are stream-ordered), summed on the host.
-    __shared__ int ws[2][4];
     const int a2 = wave_sum(done_v), b2 = wave_sum(done_e);
     if (lane_id() == 0) {
       ws[0][threadIdx.x >> 6] = a2;
@@ -248,6 +255,41 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
       work[2 * blockIdx.x + 1] += (unsigned long long)(ws[1][0] + ws[1][1] + ws[1][2] + ws[1][3]);
     }
   }
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
+                                                 int32_t k, const int32_t* __restrict__ xadj,
+                                                 const int32_t* __restrict__ adj,
+                                                 const int32_t* __restrict__ w,
+                                                 const int32_t* __restrict__ part,
+                                                 int32_t* __restrict__ upd, int32_t* __restrict__ bnd,
+                                                 int32_t* __restrict__ gain, int32_t* __restrict__ dest,
+                                                 int32_t* __restrict__ heavy,
+                                                 unsigned long long* __restrict__ work,
+                                                 uint64_t* __restrict__ cand) {
+  __shared__ int32_t hh[4][kGainInlineK];
+  __shared__ int ws[2][4];
+  gain_wl_body<S, false>(wl, wlc, k, xadj, adj, w, part, upd, bnd, gain, dest, heavy, work, cand, nullptr, 0, 0, hh,
+                         ws);
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) k_gain_wl_ws(const int32_t* __restrict__ wl, int32_t* __restrict__ wlc,
+                                                    int32_t k, const int32_t* __restrict__ xadj,
+                                                    const int32_t* __restrict__ adj,
+                                                    const int32_t* __restrict__ w,
+                                                    const int32_t* __restrict__ part,
+                                                    int32_t* __restrict__ upd, int32_t* __restrict__ bnd,
+                                                    int32_t* __restrict__ gain, int32_t* __restrict__ dest,
+                                                    int32_t* __restrict__ heavy,
+                                                    unsigned long long* __restrict__ work,
+                                                    uint64_t* __restrict__ cand,
+                                                    unsigned long long* __restrict__ ctr, int guide, int tail) {
+  __shared__ int32_t hh[4][kGainInlineK];
+  __shared__ int ws[2][4];
+  gain_wl_body<S, true>(wl, wlc, k, xadj, adj, w, part, upd, bnd, gain, dest, heavy, work, cand, ctr, guide, tail, hh,
+                        ws);
 }
 
 // Rows above S entries: one wave per vertex with a dense LDS histogram of the
@@ -312,31 +354,39 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
   }
 }
 
+template <int S>
+static void gain_go(const DGraph& g, int32_t k, const int32_t* wl, int32_t* wlc, int32_t* heavy, const int32_t* part,
+                    int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest, unsigned long long* work,
+                    unsigned grid, uint64_t* cand, const Steal* ws, hipStream_t st) {
+  if (!ws) {
+    gk_launch(k_gain_wl<S>, dim3(grid), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest,
+              heavy, work, cand);
+    return;
+  }
+  // Every workgroup slot of the device, at most kGainMaxBlocks (the
+  // statistics slots); the grab sizes follow the count on the device.
+  const unsigned pg = steal_grid(reinterpret_cast<const void*>(&k_gain_wl_ws<S>), 256, *ws, kGainMaxBlocks);
+  fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
+  gk_launch(k_gain_wl_ws<S>, dim3(pg), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest,
+            heavy, work, cand, ws->ctr, kStealGuide, kStealTail);
+}
+
 int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
                    const int32_t* part, int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest,
-                   unsigned long long* work, int64_t max_items, uint64_t* cand, hipStream_t st) {
+                   unsigned long long* work, int64_t max_items, uint64_t* cand, hipStream_t st, const Steal* ws) {
   if (g.n <= 0) return 0;
   // The grid covers at most max_items worklist entries (a bound the caller
   // knows without reading the count); the kernel grid-strides beyond it.
   const int64_t items = std::min<int64_t>(g.n, std::max<int64_t>(max_items, 1));
   unsigned grid = grid_for(items * lanes, 256);
-  if (grid > 2048) grid = 2048;
-#define GK_GAIN_CASE(S)                                                                            \
-  case S:                                                                                           \
-    gk_launch(k_gain_wl<S>, dim3(grid), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest, \
-              heavy, work, cand);                                                                 \
-    break;
+  if (grid > kGainMaxBlocks) grid = kGainMaxBlocks;
   switch (lanes) {
-    GK_GAIN_CASE(4)
-    GK_GAIN_CASE(8)
-    GK_GAIN_CASE(16)
-    GK_GAIN_CASE(32)
-    default:
-      gk_launch(k_gain_wl<64>, dim3(grid), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain,
-                dest, heavy, work, cand);
-      break;
+    case 4: gain_go<4>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
+    case 8: gain_go<8>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
+    case 16: gain_go<16>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
+    case 32: gain_go<32>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
+    default: gain_go<64>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
   }
-#undef GK_GAIN_CASE
   return hip_status();
 }
 

@@ -375,7 +375,38 @@ struct gk_ctx {
   gkrt::Pinned up, dl;      // pinned staging of the batched form (uploads / read-backs)
   gkrt::Tuner tuner;        // launch shapes chosen by timing (GK_FLAG_AUTOTUNE)
   bool autotune = false;    // set per call from GK_FLAG_AUTOTUNE
+  bool persistent = false;  // set per call from GK_FLAG_PERSISTENT
+  gkrt::Buf<unsigned long long> steal_ctr;  // queue counters of the work-stealing launches
+  gkd::Steal steal;              // steal_ctr + the device's multiprocessor count
 };
+
+namespace gkrt {
+
+// Per-call launch options from the flags.  GK_FLAG_PERSISTENT makes the
+// matching, contraction-row and gain launches persistent work-stealing ones
+// (the counters are allocated on first use and kept in the ctx).
+inline int set_launch_options(gk_ctx* ctx, int32_t flags) {
+  ctx->autotune = (flags & GK_FLAG_AUTOTUNE) != 0;
+  ctx->persistent = (flags & GK_FLAG_PERSISTENT) != 0;
+  if (ctx->persistent && !ctx->steal_ctr.p) {
+    GK_TRY(ctx->steal_ctr.alloc(ctx->pool, gkd::kStealWords));
+    int cus = 0;
+    GK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    ctx->steal.ctr = ctx->steal_ctr.p;
+    ctx->steal.cus = cus;
+  }
+  return GK_OK;
+}
+
+inline void clear_launch_options(gk_ctx* ctx) {
+  ctx->autotune = false;
+  ctx->persistent = false;
+}
+
+// The work-stealing handle for a launch, or null for a direct launch.
+inline const gkd::Steal* steal_of(const gk_ctx* ctx) { return ctx->persistent ? &ctx->steal : nullptr; }
+
+}  // namespace gkrt
 
 namespace gkrt {
 

@@ -98,6 +98,11 @@ typedef struct gk_params {
                                 (block size) by timing candidates on first
                                 use of a degree profile; kept in the ctx.
                                 Results are identical.                      */
+#define GK_FLAG_PERSISTENT 32 /* run matching, the contraction rows and the
+                                gain pass as persistent work-stealing
+                                launches: a grid sized to the device whose
+                                waves take tiles from per-XCD queues.
+                                Results are identical.                      */
 
 /* Stage ids of gk_stats.stage_ms / stage_launches / stage_bytes. */
 #define GK_ST_MATCH 0      /* candidate matching                           */

@@ -38,7 +38,7 @@ def test_header_constants_match_binding(gk):
     txt = open(os.path.join(ROOT, "include", "gkway.h")).read()
     defs = dict(re.findall(r"#define\s+(GK_(?:FLAG|ST|COARSEN)_[A-Z_]+)\s+(-?\d+)", txt))
     assert {"GK_FLAG_CHECK", "GK_FLAG_TRACE", "GK_FLAG_STAGE_TIME", "GK_FLAG_GRAPH_ROUNDS",
-            "GK_FLAG_AUTOTUNE"} <= set(defs)
+            "GK_FLAG_AUTOTUNE", "GK_FLAG_PERSISTENT"} <= set(defs)
     checked = 0
     for name, val in defs.items():
         if hasattr(gk, name):

@@ -157,6 +157,15 @@ def test_batch_graph_rounds(gk, ctx):
         check_same(ctx, graphs, ks, parts, stats, max_moves=64)
 
 
+def test_batch_persistent(gk, ctx):
+    # Persistent work-stealing launches over the pack: each graph's result
+    # equals its plain run (direct launches).
+    graphs, ks = mixed_set(gk, 24, 17, lo=2000, hi=60000)
+    print(f"persistent batch: n={[g.n for g in graphs]}", flush=True)
+    parts, stats = ctx.partition_batch(graphs, ks, flags=gk.GK_FLAG_PERSISTENT)
+    check_same(ctx, graphs, ks, parts, stats)
+
+
 def test_batch_autotune(gk, ctx):
     # A pack of over 1 Mi entries is tuned on its first coarsening level; each
     # graph's result still equals its plain run.

@@ -130,6 +130,29 @@ def test_autotune_same_result(gk, oracle, ctx):
     assert np.array_equal(base, opart)
 
 
+# Persistent work-stealing launches of matching, the contraction rows and the
+# gain pass: the schedule changes, the results do not.  Every level's hashes
+# equal the direct launches' and the partition equals the oracle's, on a mesh,
+# a DAG with heavy rows (k > 256: the heavy list) and a road network, twice in
+# a row (the queue counters are cleared per launch).
+@pytest.mark.parametrize("kind,k", [("delaunay", 64), ("dag", 300), ("road", 16)])
+def test_persistent_same_result(gk, oracle, ctx, kind, k):
+    g = {"delaunay": lambda: gk.delaunay(200000, 31), "dag": lambda: gk.dag(60000, 100, 32),
+         "road": lambda: gk.road(150000, 33)}[kind]()
+    print(f"persistent {kind} n={g.n} nnz={g.nnz} k={k}", flush=True)
+    base, sb = ctx.partition(g, k, flags=gk.GK_FLAG_TRACE)
+    for rep in range(2):
+        part, st = ctx.partition(g, k, flags=gk.GK_FLAG_TRACE | gk.GK_FLAG_PERSISTENT)
+        assert np.array_equal(part, base), rep
+        assert (st.cut, st.nlevels) == (sb.cut, sb.nlevels)
+        for l in range(st.nlevels):
+            a, b = st.levels[l], sb.levels[l]
+            assert (a.h_cand, a.h_cmap, a.h_coarse, a.h_part, a.rounds, a.moves) == \
+                (b.h_cand, b.h_cmap, b.h_coarse, b.h_part, b.rounds, b.moves), (rep, l)
+    opart, _ = oracle.partition(g, k)
+    assert np.array_equal(base, opart)
+
+
 # A circuit-timing-shaped DAG at 2^21 vertices (200 layers, edge weights
 # 1..100, vertex weights 1..4, k = 256, 1024 moves per round): thousands of
 # rounds, most selecting more than 1024 moves, so the top-1024 radix select of
