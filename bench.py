@@ -371,6 +371,7 @@ def run_batch(args, gk, torch, nested_c2=True):
         c2 = argparse.Namespace(**vars(args))
         c2.workload, c2.n, c2.k, c2.seed = "delaunay", 1 << 22, 64, 1
         c2.steps, c2.warmup, c2.stage_steps = args.c2_steps, 2, 1
+        c2.graph_rounds, c2.autotune, c2.persistent = args.plain_launch
         out["plain_config2"] = run_plain(c2, gk, torch, emit=False)
         # The same mesh with its vertices numbered along a Hilbert curve:
         # neighbours get nearby ids, as in the DIMACS10 Delaunay inputs
@@ -704,6 +705,9 @@ def main():
                     help="time candidate launch shapes of matching and contraction on first use")
     ap.add_argument("--persistent", action="store_true",
                     help="run matching, contraction rows and the gain pass as persistent work-stealing launches")
+    ap.add_argument("--direct", action="store_true",
+                    help="batch form: direct launches and eager rounds (by default the batched form runs "
+                         "persistent work-stealing launches, hipGraph-replayed rounds and autotuned launch shapes)")
     ap.add_argument("--stage-steps", type=int, default=3,
                     help="extra steps with per-stage HIP events for the stage table and roofline")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -719,6 +723,13 @@ def main():
     import torch
 
     if args.form == "batch":
+        # The batched form's launch mechanics (results identical): persistent
+        # work-stealing launches, rounds replayed from captured hipGraphs and
+        # autotuned launch shapes, unless --direct.  The nested plain runs
+        # keep the plain path's own (direct unless asked for).
+        args.plain_launch = (args.graph_rounds, args.autotune, args.persistent)
+        if not args.direct:
+            args.graph_rounds = args.autotune = args.persistent = True
         args.seed = 4 if args.seed is None else args.seed
         args.steps = 8 if args.steps is None else args.steps
         args.warmup = 2 if args.warmup is None else args.warmup

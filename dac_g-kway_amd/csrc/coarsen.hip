@@ -65,9 +65,9 @@ template <int S, int B>
 __global__ void __launch_bounds__(B) k_match_ws(int32_t n, const int32_t* __restrict__ xadj,
                                                 const int32_t* __restrict__ adj,
                                                 const int32_t* __restrict__ w, int32_t* __restrict__ cand,
-                                                unsigned long long* __restrict__ ctr, StealShape sh) {
+                                                unsigned long long* __restrict__ ctr) {
   const int64_t sweeps = (int64_t(n) * S + 63) / 64;
-  steal_sweeps(ctr, sweeps, sh, [&](int64_t sw) { match_slot<S>(sw * 64 + lane_id(), n, xadj, adj, w, cand); });
+  steal_sweeps(ctr, sweeps, [&](int64_t sw) { match_slot<S>(sw * 64 + lane_id(), n, xadj, adj, w, cand); });
 }
 
 template <int S, int B>
@@ -78,10 +78,12 @@ static void match_go(const DGraph& g, int32_t* cand, const Steal* ws, hipStream_
     return;
   }
   const unsigned pg = steal_grid(reinterpret_cast<const void*>(&k_match_ws<S, B>), B, *ws, grid);
-  const StealShape sh =
-      steal_shape((int64_t(g.n) * S + 63) / 64, int64_t(pg) * (B / 64));
-  fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
-  gk_launch(k_match_ws<S, B>, dim3(pg), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand, ws->ctr, sh);
+  unsigned long long* ctr = nullptr;  // queues cleared and used only when the work has a tail
+  if (steal_tail((int64_t(g.n) * S + 63) / 64, int64_t(pg) * (B / 64)) > 0) {
+    fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
+    ctr = ws->ctr;
+  }
+  gk_launch(k_match_ws<S, B>, dim3(pg), dim3(B), 0, st, g.n, g.xadj, g.adj, g.w, cand, ctr);
 }
 
 template <int B>
@@ -567,12 +569,12 @@ __global__ void __launch_bounds__(B) k_contract_rows_ws(const int32_t* __restric
                                                           const int32_t* __restrict__ toff,
                                                           int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
                                                           int32_t* __restrict__ len,
-                                                          unsigned long long* __restrict__ ctr, StealShape sh) {
+                                                          unsigned long long* __restrict__ ctr) {
   static_assert(B % 64 == 0, "whole waves");
   __shared__ int32_t mark[B];
   int32_t* wm = mark + (threadIdx.x & ~63u);
   const int64_t sweeps = (int64_t(nrows) * S + 63) / 64;
-  steal_sweeps(ctr, sweeps, sh, [&](int64_t sw) {
+  steal_sweeps(ctr, sweeps, [&](int64_t sw) {
     contract_row_slot<S>(sw * 64 + lane_id(), wm, rows, desc, nrows, mrow, adj, w, cmap, toff, tadj, tw, len);
   });
 }
@@ -593,11 +595,12 @@ static void contract_rows_block(const DGraph& g, int lanes, const int32_t* rows,
   };
   auto go_ws = [&](auto kern, int S) {
     const unsigned pg = steal_grid(reinterpret_cast<const void*>(kern), B, *ws, grid);
-    const StealShape sh =
-        steal_shape((int64_t(nrows) * S + 63) / 64, int64_t(pg) * (B / 64));
-    fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
-    gk_launch(kern, dim3(pg), dim3(B), 0, st, rows, desc, nrows, mrow, g.adj, g.w, cmap, toff, tadj, tw, len,
-              ws->ctr, sh);
+    unsigned long long* ctr = nullptr;  // queues cleared and used only when the work has a tail
+    if (steal_tail((int64_t(nrows) * S + 63) / 64, int64_t(pg) * (B / 64)) > 0) {
+      fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
+      ctr = ws->ctr;
+    }
+    gk_launch(kern, dim3(pg), dim3(B), 0, st, rows, desc, nrows, mrow, g.adj, g.w, cmap, toff, tadj, tw, len, ctr);
   };
   if (!Wide && ws) {
     switch (lanes) {

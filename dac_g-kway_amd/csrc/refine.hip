@@ -121,7 +121,7 @@ __device__ __forceinline__ void gain_wl_body(const int32_t* __restrict__ wl, int
                                              int32_t* __restrict__ gain, int32_t* __restrict__ dest,
                                              int32_t* __restrict__ heavy, unsigned long long* __restrict__ work,
                                              uint64_t* __restrict__ cand, unsigned long long* __restrict__ ctr,
-                                             int guide, int tail, int32_t (*hh)[kGainInlineK], int (*ws)[4]) {
+                                             int32_t (*hh)[kGainInlineK], int (*ws)[4]) {
   const int32_t cnt = wlc[0];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
@@ -234,8 +234,7 @@ __device__ __forceinline__ void gain_wl_body(const int32_t* __restrict__ wl, int
   };
   if constexpr (Steal) {
     const int64_t sweeps = (int64_t(cnt) + (64 / S) - 1) / (64 / S);
-    steal_sweeps(ctr, sweeps, steal_shape(sweeps, int64_t(gridDim.x) * 4, guide, tail),
-                 [&](int64_t sw) { pass(sw * (64 / S)); });
+    steal_sweeps(ctr, sweeps, [&](int64_t sw) { pass(sw * (64 / S)); });
   } else {
     const int64_t nsg = int64_t(gridDim.x) * (256 / S);
     const int64_t wave_g0 = (int64_t(blockIdx.x) * 4 + wv) * (64 / S);
@@ -270,8 +269,7 @@ __global__ void __launch_bounds__(256) k_gain_wl(const int32_t* __restrict__ wl,
                                                  uint64_t* __restrict__ cand) {
   __shared__ int32_t hh[4][kGainInlineK];
   __shared__ int ws[2][4];
-  gain_wl_body<S, false>(wl, wlc, k, xadj, adj, w, part, upd, bnd, gain, dest, heavy, work, cand, nullptr, 0, 0, hh,
-                         ws);
+  gain_wl_body<S, false>(wl, wlc, k, xadj, adj, w, part, upd, bnd, gain, dest, heavy, work, cand, nullptr, hh, ws);
 }
 
 template <int S>
@@ -285,11 +283,10 @@ __global__ void __launch_bounds__(256) k_gain_wl_ws(const int32_t* __restrict__ 
                                                     int32_t* __restrict__ heavy,
                                                     unsigned long long* __restrict__ work,
                                                     uint64_t* __restrict__ cand,
-                                                    unsigned long long* __restrict__ ctr, int guide, int tail) {
+                                                    unsigned long long* __restrict__ ctr) {
   __shared__ int32_t hh[4][kGainInlineK];
   __shared__ int ws[2][4];
-  gain_wl_body<S, true>(wl, wlc, k, xadj, adj, w, part, upd, bnd, gain, dest, heavy, work, cand, ctr, guide, tail, hh,
-                        ws);
+  gain_wl_body<S, true>(wl, wlc, k, xadj, adj, w, part, upd, bnd, gain, dest, heavy, work, cand, ctr, hh, ws);
 }
 
 // Rows above S entries: one wave per vertex with a dense LDS histogram of the
@@ -357,18 +354,24 @@ __global__ void __launch_bounds__(64) k_gain_heavy(const int32_t* __restrict__ h
 template <int S>
 static void gain_go(const DGraph& g, int32_t k, const int32_t* wl, int32_t* wlc, int32_t* heavy, const int32_t* part,
                     int32_t* upd, int32_t* bnd, int32_t* gain, int32_t* dest, unsigned long long* work,
-                    unsigned grid, uint64_t* cand, const Steal* ws, hipStream_t st) {
+                    unsigned grid, int64_t items, uint64_t* cand, const Steal* ws, hipStream_t st) {
   if (!ws) {
     gk_launch(k_gain_wl<S>, dim3(grid), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest,
               heavy, work, cand);
     return;
   }
   // Every workgroup slot of the device, at most kGainMaxBlocks (the
-  // statistics slots); the grab sizes follow the count on the device.
+  // statistics slots).  The count is on the device; the queues are cleared
+  // and offered only when the caller's expected count would reach the tail
+  // (a round's worklist rarely does), else the static schedule takes all.
   const unsigned pg = steal_grid(reinterpret_cast<const void*>(&k_gain_wl_ws<S>), 256, *ws, kGainMaxBlocks);
-  fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
+  unsigned long long* ctr = nullptr;
+  if (steal_tail((items + (64 / S) - 1) / (64 / S), int64_t(pg) * 4) > 0) {
+    fill<unsigned long long>(ws->ctr, 0ull, kStealWords, st);
+    ctr = ws->ctr;
+  }
   gk_launch(k_gain_wl_ws<S>, dim3(pg), dim3(256), 0, st, wl, wlc, k, g.xadj, g.adj, g.w, part, upd, bnd, gain, dest,
-            heavy, work, cand, ws->ctr, kStealGuide, kStealTail);
+            heavy, work, cand, ctr);
 }
 
 int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int32_t* wlc, int32_t* heavy,
@@ -381,11 +384,11 @@ int launch_gain_wl(const DGraph& g, int32_t k, int lanes, const int32_t* wl, int
   unsigned grid = grid_for(items * lanes, 256);
   if (grid > kGainMaxBlocks) grid = kGainMaxBlocks;
   switch (lanes) {
-    case 4: gain_go<4>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
-    case 8: gain_go<8>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
-    case 16: gain_go<16>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
-    case 32: gain_go<32>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
-    default: gain_go<64>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, cand, ws, st); break;
+    case 4: gain_go<4>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, items, cand, ws, st); break;
+    case 8: gain_go<8>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, items, cand, ws, st); break;
+    case 16: gain_go<16>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, items, cand, ws, st); break;
+    case 32: gain_go<32>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, items, cand, ws, st); break;
+    default: gain_go<64>(g, k, wl, wlc, heavy, part, upd, bnd, gain, dest, work, grid, items, cand, ws, st); break;
   }
   return hip_status();
 }

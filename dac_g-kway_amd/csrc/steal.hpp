@@ -2,18 +2,32 @@
 //
 // A persistent launch is sized to the device -- its multiprocessor count
 // times the workgroups of the kernel that fit on one -- instead of to the
-// work.  Its waves take work from kStealQueues queues until every queue is
-// empty.  The work is a run of "sweeps" (64 consecutive global thread slots,
-// the unit one wave covers in the kernel's direct launch), cut into
-// kStealQueues contiguous ranges; queue q hands out range q through a 64-bit
-// counter in its own 128-byte line, a grab at a time.  A wave starts at
-// queue blockIdx.x % kStealQueues (the dispatcher places consecutive
-// workgroups on the eight XCDs in turn, so a range is mostly read through
-// one XCD's L2) and moves on to the next queue when its own is empty; it
-// reads a queue's counter before taking from it, so an empty queue costs a
-// load, not an atomic.  Which wave takes which sweep
-// depends on timing: the kernels that use this write results that do not
-// depend on the schedule.
+// work, and its waves loop over the work.  The work is a run of "sweeps" (64
+// consecutive global thread slots, the unit one wave covers in the kernel's
+// direct launch).  Each wave first takes its static share, sweeps w, w + W,
+// w + 2W, ... (W waves in the grid) of all but the last `tail` sweeps, with
+// no shared state.  The tail -- one sweep per wave of the grid, when there
+// are more than kStealStaticWaves sweeps per wave -- is handed out by
+// kStealQueues queues, a workgroup at a time (one sweep per wave of the
+// workgroup per grab): a workgroup whose static share ran long leaves the
+// tail to the others.  The tail is cut into kStealQueues ranges, one per
+// queue, each with a 64-bit counter in its own 128-byte line (cleared by the
+// launcher before the launch); a workgroup drains its home queue
+// (blockIdx.x % kStealQueues) and then steals from the next one (from all of
+// them when the grid has fewer workgroups than queues, so that every range
+// has a workgroup that drains it).
+//
+// Why so little stealing: same-address atomics serialise on MI355X, and the
+// queue operations, not imbalance, set the cost of finer schedules.
+// Measured on a 2^22-point Delaunay graph (k = 64; coarsening / refinement
+// per V-cycle): direct launches 3.40 / 1.72 ms; this grid with the static
+// schedule alone 3.55 / 1.58 ms; every wave grabbing 1/(its queue's waves)
+// of what is left (guided self-scheduling) from all 8 queues 5.20 / 2.45 ms;
+// fixed per-wave grabs from two queues 4.18 / 2.40 ms; this schedule 3.41 /
+// 1.49 ms.
+//
+// Which wave takes which sweep depends on timing: the kernels that use this
+// write results that do not depend on the schedule.
 #pragma once
 #include <mutex>
 #include <unordered_map>
@@ -25,69 +39,49 @@ namespace gkd {
 constexpr int kStealQueues = 8;
 constexpr int kStealLine = 16;  // 64-bit counters from one queue to the next (128 bytes)
 constexpr int kStealWords = kStealQueues * kStealLine;
+constexpr int kStealStaticWaves = 2;  // up to this many sweeps per wave: static schedule only
 
-// Host-side handle: the counters (kStealWords 64-bit words of device memory,
-// cleared by the launcher before every launch) and the device's
-// multiprocessor count.
+// Host-side handle: the counters (kStealWords 64-bit words of device memory)
+// and the device's multiprocessor count.
 struct Steal {
   unsigned long long* ctr = nullptr;
   int cus = 0;
 };
 
-// Grab sizes (guided self-scheduling): a wave takes about 1 / div of the
-// sweeps its queue has left, and at least min_chunk, so grabs are large while
-// much remains and small at the end, where the last ones finish together.
-// div = the queue's waves x kStealGuide; min_chunk = range / (div x
-// kStealTail).  The grabs per queue stay near div x (1 + ln kStealTail).
-// Same-address atomics serialise, and the grabs are what this schedule costs
-// over a direct launch: measured on MI355X (2^22-point Delaunay, k = 64),
-// coarsening took 5.8 ms with guide 1 / tail 16, 8.0 ms with guide 2, 9.8 ms
-// with guide 4 and 5.3 ms with guide 1 / tail 4, against 3.4 ms for the
-// direct launches.
-constexpr int kStealGuide = 1;
-constexpr int kStealTail = 4;
-
-struct StealShape {
-  int64_t div;        // remaining / div sweeps per grab
-  int64_t min_chunk;  // sweeps per grab at least
-};
-
-__host__ __device__ inline StealShape steal_shape(int64_t sweeps, int64_t waves, int guide = kStealGuide,
-                                                  int tail = kStealTail) {
-  int64_t div = (waves / kStealQueues) * guide;
-  if (div < 1) div = 1;
-  const int64_t range = sweeps / kStealQueues + 1;
-  int64_t mc = range / (div * tail);
-  return {div, mc < 1 ? 1 : mc};
+// Sweeps of [0, sweeps) that go through the queues, for a grid of W waves.
+__host__ __device__ inline int64_t steal_tail(int64_t sweeps, int64_t W) {
+  return sweeps > int64_t(kStealStaticWaves) * W ? W : 0;
 }
 
-// Calls f(sw) for every sweep sw of [0, sweeps) that this wave takes.  The
-// whole wave runs it together (wave-uniform control flow).
+// Calls f(sw) for every sweep sw of [0, sweeps) that this wave takes.  Every
+// thread of the workgroup calls it (the tail grabs are workgroup-wide); f is
+// called by whole waves.  ctr == nullptr (counters not cleared for this
+// launch): the static schedule takes everything.
 template <class F>
-__device__ __forceinline__ void steal_sweeps(unsigned long long* ctr, int64_t sweeps, StealShape sh, F&& f) {
+__device__ __forceinline__ void steal_sweeps(unsigned long long* ctr, int64_t sweeps, F&& f) {
+  __shared__ unsigned long long grab;
+  const int wpb = static_cast<int>(blockDim.x >> 6);
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+  const int64_t W = int64_t(gridDim.x) * wpb;
+  const int64_t tail = ctr ? steal_tail(sweeps, W) : 0;
+  const int64_t s0 = sweeps - tail;
+  for (int64_t sw = int64_t(blockIdx.x) * wpb + wave; sw < s0; sw += W) f(sw);
+  if (tail == 0) return;
   const int home = static_cast<int>(blockIdx.x % kStealQueues);
-  for (int i = 0; i < kStealQueues; ++i) {
+  const int visits = gridDim.x < kStealQueues ? kStealQueues : 2;
+  for (int i = 0; i < visits; ++i) {
     const int q = (home + i) % kStealQueues;
-    const int64_t lo = sweeps * q / kStealQueues, hi = sweeps * (q + 1) / kStealQueues;
+    const int64_t lo = s0 + tail * q / kStealQueues, hi = s0 + tail * (q + 1) / kStealQueues;
     const unsigned long long len = static_cast<unsigned long long>(hi - lo);
     unsigned long long* c = ctr + q * kStealLine;
     while (true) {
-      unsigned long long start = ~0ull, chunk = 0;
-      if (lane_id() == 0) {
-        const unsigned long long seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (seen < len) {
-          chunk = (len - seen) / static_cast<unsigned long long>(sh.div);
-          if (chunk < static_cast<unsigned long long>(sh.min_chunk)) chunk = sh.min_chunk;
-          start = atomicAdd(c, chunk);
-        }
-      }
-      start = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(start)) |
-              (static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(start >> 32))) << 32);
-      chunk = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(chunk));
+      __syncthreads();  // every wave has read the previous grab
+      if (threadIdx.x == 0) grab = atomicAdd(c, static_cast<unsigned long long>(wpb));
+      __syncthreads();
+      const unsigned long long start = grab;
       if (start >= len) break;
-      const int64_t s0 = lo + static_cast<int64_t>(start);
-      const int64_t s1 = s0 + static_cast<int64_t>(chunk) < hi ? s0 + static_cast<int64_t>(chunk) : hi;
-      for (int64_t sw = s0; sw < s1; ++sw) f(sw);
+      const int64_t sw = lo + static_cast<int64_t>(start) + wave;
+      if (sw < hi) f(sw);
     }
   }
 }

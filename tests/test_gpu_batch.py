@@ -166,6 +166,18 @@ def test_batch_persistent(gk, ctx):
     check_same(ctx, graphs, ks, parts, stats)
 
 
+def test_batch_all_launch_options(gk, ctx):
+    # Rounds replayed from captured hipGraphs, autotuned launch shapes and
+    # persistent work-stealing launches together, twice in a row: each
+    # graph's result equals its plain run.
+    graphs, ks = mixed_set(gk, 24, 18, lo=5000, hi=60000)
+    flags = gk.GK_FLAG_GRAPH_ROUNDS | gk.GK_FLAG_AUTOTUNE | gk.GK_FLAG_PERSISTENT
+    for rep in range(2):
+        print(f"all launch options rep {rep}: n={[g.n for g in graphs]}", flush=True)
+        parts, stats = ctx.partition_batch(graphs, ks, flags=flags)
+        check_same(ctx, graphs, ks, parts, stats)
+
+
 def test_batch_autotune(gk, ctx):
     # A pack of over 1 Mi entries is tuned on its first coarsening level; each
     # graph's result still equals its plain run.

@@ -93,14 +93,17 @@ def test_any_k(gk, oracle, ctx, family, n, k):
 
 # Refinement rounds replayed from a captured hipGraph: the same trace and
 # partition as the oracle, on levels with many replays (small move cap) and
-# across calls that re-capture the graph, with eager calls in between.
+# across calls that re-capture the graph, with eager calls in between; also
+# with persistent work-stealing launches, whose queue-clearing fill kernel
+# and gain launch are then captured nodes.
+@pytest.mark.parametrize("persistent", [False, True])
 @pytest.mark.parametrize("family,n,k,moves", [("delaunay", 60000, 64, 1024), ("dag", 20000, 32, 16),
                                               ("road", 30000, 8, 64)])
-def test_graph_rounds(gk, oracle, ctx, family, n, k, moves):
+def test_graph_rounds(gk, oracle, ctx, family, n, k, moves, persistent):
     g = {"delaunay": lambda: gk.delaunay(n, 3), "dag": lambda: gk.dag(n, 40, 3), "road": lambda: gk.road(n, 2)}[family]()
-    flags = gk.GK_FLAG_TRACE | gk.GK_FLAG_GRAPH_ROUNDS
+    flags = gk.GK_FLAG_TRACE | gk.GK_FLAG_GRAPH_ROUNDS | (gk.GK_FLAG_PERSISTENT if persistent else 0)
     for rep in range(2):
-        print(f"graph rounds rep {rep} n={g.n} k={k} moves={moves}", flush=True)
+        print(f"graph rounds rep {rep} n={g.n} k={k} moves={moves} persistent={persistent}", flush=True)
         part, st = ctx.partition(g, k, flags=flags, max_moves=moves)
         eager, se = ctx.partition(g, k, flags=gk.GK_FLAG_TRACE, max_moves=moves)
         assert np.array_equal(part, eager)
@@ -132,12 +135,13 @@ def test_autotune_same_result(gk, oracle, ctx):
 
 # Persistent work-stealing launches of matching, the contraction rows and the
 # gain pass: the schedule changes, the results do not.  Every level's hashes
-# equal the direct launches' and the partition equals the oracle's, on a mesh,
-# a DAG with heavy rows (k > 256: the heavy list) and a road network, twice in
-# a row (the queue counters are cleared per launch).
+# equal the direct launches' and the partition equals the oracle's, on a mesh
+# large enough that its first levels hand out a tail through the queues, a
+# DAG with heavy rows (k > 256: the heavy list) and a road network (static
+# schedule only), twice in a row (the queue counters are cleared per launch).
 @pytest.mark.parametrize("kind,k", [("delaunay", 64), ("dag", 300), ("road", 16)])
 def test_persistent_same_result(gk, oracle, ctx, kind, k):
-    g = {"delaunay": lambda: gk.delaunay(200000, 31), "dag": lambda: gk.dag(60000, 100, 32),
+    g = {"delaunay": lambda: gk.delaunay(600000, 31), "dag": lambda: gk.dag(60000, 100, 32),
          "road": lambda: gk.road(150000, 33)}[kind]()
     print(f"persistent {kind} n={g.n} nnz={g.nnz} k={k}", flush=True)
     base, sb = ctx.partition(g, k, flags=gk.GK_FLAG_TRACE)
