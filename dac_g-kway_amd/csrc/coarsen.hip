@@ -57,7 +57,7 @@ template <int S, int B>
 __global__ void __launch_bounds__(B) k_match(int32_t n, const int32_t* __restrict__ xadj,
                                              const int32_t* __restrict__ adj,
                                              const int32_t* __restrict__ w, int32_t* __restrict__ cand) {
-  match_slot<S>(int64_t(blockIdx.x) * B + threadIdx.x, n, xadj, adj, w, cand);
+  match_slot<S>(int64_t(xcd_block()) * B + threadIdx.x, n, xadj, adj, w, cand);
 }
 
 // The same slots, taken from the work-stealing queues.
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(256) k_row_bounds(int32_t nc, const int32_t* _
                                                     int32_t* __restrict__ ub, int32_t* __restrict__ lists,
                                                     int2* __restrict__ desc, int2* __restrict__ mrow,
                                                     int32_t* __restrict__ cnt) {
-  const int64_t c = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t c = int64_t(xcd_block()) * 256 + threadIdx.x;
   int cls = -1;
   int32_t m0 = 0, m1 = 0;
   if (c < nc) {
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(B) k_contract_rows_wide(const int32_t* __restr
                                                        const int32_t* __restrict__ toff,
                                                        int32_t* __restrict__ tadj, int32_t* __restrict__ tw,
                                                        int32_t* __restrict__ len) {
-  const int64_t gt = int64_t(blockIdx.x) * B + threadIdx.x;
+  const int64_t gt = int64_t(xcd_block()) * B + threadIdx.x;
   const int64_t ri = gt / S;
   const int sub = static_cast<int>(gt % S);
   if (ri >= nrows) return;
@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(B) k_contract_rows(const int32_t* __restrict__
                                                        int32_t* __restrict__ len) {
   static_assert(B % 64 == 0, "whole waves");
   __shared__ int32_t mark[B];
-  contract_row_slot<S>(int64_t(blockIdx.x) * B + threadIdx.x, mark + (threadIdx.x & ~63u), rows, desc, nrows, mrow,
+  contract_row_slot<S>(int64_t(xcd_block()) * B + threadIdx.x, mark + (threadIdx.x & ~63u), rows, desc, nrows, mrow,
                        adj, w, cmap, toff, tadj, tw, len);
 }
 

@@ -162,6 +162,24 @@ __device__ __forceinline__ int last_set_at_or_below(uint64_t m, int lane) {
   return v ? 63 - __clzll(static_cast<long long>(v)) : -1;
 }
 
+// XCD-aware workgroup order.  The dispatcher deals the workgroups of a
+// launch to the eight XCDs in turn (workgroup b to XCD b % 8), and each XCD
+// has its own L2.  xcd_tile renumbers them so that XCD x takes a contiguous
+// run of tiles -- [x * per, (x + 1) * per) with per = G / 8, the first G % 8
+// XCDs one tile more -- still in increasing order over time, so the rows
+// that neighbouring tiles share, and the neighbour lookups of graphs
+// numbered with locality, are fetched into one L2 instead of into all
+// eight.  It is a bijection on [0, G), so no result changes.  Measured on
+// MI355X (pack of 4096 graphs, level 0, HBM bytes fetched per launch):
+// matching 2048 -> 1201 MB, contraction rows (16 / 32 / 64 lanes) 1521 /
+// 2208 / 1744 -> 808 / 1372 / 1216 MB.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
+  const uint32_t per = G / 8, rem = G % 8;
+  const uint32_t x = b % 8, i = b / 8;
+  return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+__device__ __forceinline__ uint32_t xcd_block() { return xcd_tile(blockIdx.x, gridDim.x); }
+
 // Fill kernel used instead of hipMemsetAsync, so the same code path can be
 // captured into a HIP graph.
 template <typename T>

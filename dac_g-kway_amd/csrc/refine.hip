@@ -237,7 +237,7 @@ __device__ __forceinline__ void gain_wl_body(const int32_t* __restrict__ wl, int
     steal_sweeps(ctr, sweeps, [&](int64_t sw) { pass(sw * (64 / S)); });
   } else {
     const int64_t nsg = int64_t(gridDim.x) * (256 / S);
-    const int64_t wave_g0 = (int64_t(blockIdx.x) * 4 + wv) * (64 / S);
+    const int64_t wave_g0 = (int64_t(xcd_block()) * 4 + wv) * (64 / S);
     for (int64_t eb = wave_g0; eb < cnt; eb += nsg) pass(eb);
   }
   if (work) {
@@ -603,7 +603,7 @@ __global__ void __launch_bounds__(256) k_cut(int32_t n, int32_t k, const int32_t
   __syncthreads();
   long long c = 0;
   const int sub = threadIdx.x & 7;
-  for (int64_t v = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 8; v < n; v += int64_t(gridDim.x) * 32) {
+  for (int64_t v = (int64_t(xcd_block()) * 256 + threadIdx.x) / 8; v < n; v += int64_t(gridDim.x) * 32) {
     const int32_t p = part[v];
     for (int32_t j = xadj[v] + sub; j < xadj[v + 1]; j += 8) {
       const int32_t u = adj[j];

@@ -4,9 +4,10 @@
 // times the workgroups of the kernel that fit on one -- instead of to the
 // work, and its waves loop over the work.  The work is a run of "sweeps" (64
 // consecutive global thread slots, the unit one wave covers in the kernel's
-// direct launch).  Each wave first takes its static share, sweeps w, w + W,
-// w + 2W, ... (W waves in the grid) of all but the last `tail` sweeps, with
-// no shared state.  The tail -- one sweep per wave of the grid, when there
+// direct launch).  Each wave first takes its static share of all but the
+// last `tail` sweeps, with no shared state: the part is cut into eight
+// contiguous ranges, one per XCD (workgroup b runs on XCD b % 8), and the
+// waves of an XCD's workgroups stride over its range.  The tail -- one sweep per wave of the grid, when there
 // are more than kStealStaticWaves sweeps per wave -- is handed out by
 // kStealQueues queues, a workgroup at a time (one sweep per wave of the
 // workgroup per grab): a workgroup whose static share ran long leaves the
@@ -65,7 +66,16 @@ __device__ __forceinline__ void steal_sweeps(unsigned long long* ctr, int64_t sw
   const int64_t W = int64_t(gridDim.x) * wpb;
   const int64_t tail = ctr ? steal_tail(sweeps, W) : 0;
   const int64_t s0 = sweeps - tail;
-  for (int64_t sw = int64_t(blockIdx.x) * wpb + wave; sw < s0; sw += W) f(sw);
+  if (gridDim.x >= kStealQueues) {
+    // By XCD: the workgroups of XCD x (blockIdx.x % 8 == x) stride over the
+    // x-th eighth of the static part, so it is read through one L2.
+    const uint32_t x = blockIdx.x % kStealQueues, i = blockIdx.x / kStealQueues;
+    const int64_t gx = int64_t(gridDim.x / kStealQueues) + (x < gridDim.x % kStealQueues ? 1 : 0);
+    const int64_t lo = s0 * x / kStealQueues, hi = s0 * (x + 1) / kStealQueues;
+    for (int64_t sw = lo + int64_t(i) * wpb + wave; sw < hi; sw += gx * wpb) f(sw);
+  } else {
+    for (int64_t sw = int64_t(blockIdx.x) * wpb + wave; sw < s0; sw += W) f(sw);
+  }
   if (tail == 0) return;
   const int home = static_cast<int>(blockIdx.x % kStealQueues);
   const int visits = gridDim.x < kStealQueues ? kStealQueues : 2;
