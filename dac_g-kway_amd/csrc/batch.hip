@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_project_b(int32_t n, const int32_t* __r
                                                    const int32_t* __restrict__ cdest, int32_t* __restrict__ part,
                                                    int32_t* __restrict__ bnd, int32_t* __restrict__ gain,
                                                    int32_t* __restrict__ dest, int32_t* __restrict__ upd) {
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t v = int64_t(xcd_block()) * 256 + threadIdx.x;
   if (v >= n) return;
   const int32_t c = cmap[v];
   const int32_t p = cpart[c];
@@ -1093,7 +1093,7 @@ __global__ void __launch_bounds__(256) k_cut_chunks(const int32_t* __restrict__ 
                                                     unsigned long long* __restrict__ pw64) {
   extern __shared__ unsigned int hist[];
   __shared__ long long wsum[4];
-  const int32_t c = blockIdx.x;
+  const int32_t c = static_cast<int32_t>(xcd_block());
   const int32_t j = cj[c], k = kk[j];
   for (int32_t q = threadIdx.x; q < k; q += 256) hist[q] = 0;
   __syncthreads();

@@ -115,7 +115,7 @@ int launch_match(const DGraph& g, int lanes, int block, int32_t* cand, hipStream
 // finds for every vertex its root pair and the hop distance to it.
 __global__ void __launch_bounds__(256) k_jump_init(int32_t n, const int32_t* __restrict__ cand,
                                                    int32_t* __restrict__ next, int32_t* __restrict__ dist) {
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t v = int64_t(xcd_block()) * 256 + threadIdx.x;
   if (v >= n) return;
   const int32_t c = cand[v];
   const bool root = cand[c] == v;
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(256) k_jump_round(int32_t n, const int32_t* __
                                                     int32_t* __restrict__ dist_out,
                                                     int32_t* __restrict__ flags, int32_t round) {
   if (round > 0 && flags[round - 1] == 0) return;  // converged in an earlier round
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t v = int64_t(xcd_block()) * 256 + threadIdx.x;
   bool moved = false;
   if (v < n) {
     const int32_t nx = next_in[v];
@@ -164,7 +164,7 @@ int launch_jump_round(int32_t n, const int32_t* cand, const int32_t* next_in, co
 __global__ void __launch_bounds__(256) k_label_max(int32_t n, const int32_t* __restrict__ cand,
                                                    const int32_t* __restrict__ next,
                                                    int32_t* __restrict__ compmax) {
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t v = int64_t(xcd_block()) * 256 + threadIdx.x;
   if (v >= n) return;
   const int32_t r = next[v];
   const int32_t key = max(r, cand[r]);
@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(256) k_label_get(int32_t n, const int32_t* __r
                                                    const int32_t* __restrict__ next,
                                                    const int32_t* __restrict__ compmax,
                                                    int32_t* __restrict__ label) {
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t v = int64_t(xcd_block()) * 256 + threadIdx.x;
   if (v >= n) return;
   const int32_t r = next[v];
   label[v] = compmax[max(r, cand[r])];
@@ -267,7 +267,7 @@ int launch_cluster(int32_t n, const uint64_t* skeys, const int32_t* sids, int sb
 __global__ void __launch_bounds__(256) k_cvwgt(int32_t nc, const int32_t* __restrict__ cptr,
                                                const int32_t* __restrict__ order,
                                                const int32_t* __restrict__ vw, int32_t* __restrict__ cvw) {
-  const int64_t c = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t c = int64_t(xcd_block()) * 256 + threadIdx.x;
   if (c >= nc) return;
   int32_t s = 0;
   for (int32_t j = cptr[c]; j < cptr[c + 1]; ++j) s += vw[order[j]];

@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(256) k_project(int32_t n, const int32_t* __res
                                                  int32_t* __restrict__ part, int32_t* __restrict__ bnd,
                                                  int32_t* __restrict__ gain, int32_t* __restrict__ dest,
                                                  int32_t* __restrict__ upd) {
-  const int64_t v = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t v = int64_t(xcd_block()) * 256 + threadIdx.x;
   if (v >= n) return;
   const int32_t c = cmap[v];
   const int32_t p = cpart[c];
