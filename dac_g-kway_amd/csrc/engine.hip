@@ -256,15 +256,21 @@ void Engine::mark(Job& j, int phase) {
 // ------------------------------------------------------------ coarsening
 int Engine::coarsen_step(bool* worked) {
   std::vector<Job*> ids;
+  bool ready = false;
   for (auto& jp : jobs_) {
     Job& j = *jp;
     if (j.phase != COARSEN) continue;
     if (static_cast<int32_t>(j.L.size()) >= j.maxlv || !(j.L.back().n > j.floor_n)) {
       mark(j, INIT_READY);
+      ready = true;
       continue;
     }
     ids.push_back(&j);
   }
+  // Jobs whose coarsening ended go to the host workers before this step's
+  // device work, not after it (measured on the 4096-graph batch: 866-872
+  // against 877-882 ms per call, alternating on one box).
+  if (ready) GK_TRY(init_step(worked));
   if (ids.empty()) return GK_OK;
   *worked = true;
   // Jobs of one coarsening pack share the stage parameters G (max_group).
