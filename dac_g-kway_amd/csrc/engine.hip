@@ -54,11 +54,10 @@ WorkerPool::WorkerPool(int n) {
           std::unique_lock<std::mutex> lk(m_);
           cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
           if (stop_ && q_.empty()) return;
-          f = std::move(const_cast<Item&>(q_.top()).f);
+          f = std::move(q_.front());
           q_.pop();
         }
         f();
-        if (hook_) hook_();
       }
     });
 }
@@ -72,12 +71,12 @@ WorkerPool::~WorkerPool() {
   for (auto& t : th_) t.join();
 }
 
-std::future<int> WorkerPool::submit(std::function<int()> f, int64_t prio) {
+std::future<int> WorkerPool::submit(std::function<int()> f) {
   auto task = std::make_shared<std::packaged_task<int()>>(std::move(f));
   std::future<int> fut = task->get_future();
   {
     std::lock_guard<std::mutex> lk(m_);
-    q_.push(Item{prio, seq_++, [task] { (*task)(); }});
+    q_.emplace([task] { (*task)(); });
   }
   cv_.notify_one();
   return fut;
@@ -118,15 +117,7 @@ double Engine::now_ms() const {
 
 Engine::Engine(gk_ctx* ctx, hipStream_t st, int workers)
     : ctx_(ctx), st_(st), P_(ctx->pool), t0_(std::chrono::steady_clock::now()), up_(ctx->up), dl_(ctx->dl),
-      pool_(workers) {
-  pool_.set_done_hook([this] {
-    {
-      std::lock_guard<std::mutex> lk(dmu_);
-      ++inits_done_;
-    }
-    dcv_.notify_all();
-  });
-}
+      pool_(workers) {}
 
 Engine::~Engine() {
   // GK_ENGINE_PROFILE=1: one line per engine with the host wall time of its
@@ -403,9 +394,6 @@ int Engine::init_step(bool* worked) {
     j.S.nlevels = j.nl;
     j.S.coarsest_n = nt;
     Job* jp = &j;
-    // Deeper hierarchies first: their refinement, which follows on the
-    // device, is the call's tail.
-    const int64_t prio = int64_t(j.L.size());
     j.fut = pool_.submit([this, jp]() -> int {
       Job& J2 = *jp;
       timespec c0{}, c1{};
@@ -439,7 +427,7 @@ int Engine::init_step(bool* worked) {
       clock_gettime(CLOCK_THREAD_CPUTIME_ID, &c1);
       init_cpu_us_ += (c1.tv_sec - c0.tv_sec) * 1000000LL + (c1.tv_nsec - c0.tv_nsec) / 1000;
       return GK_OK;
-    }, prio);
+    });
     mark(j, INIT_RUNNING);
   }
   return GK_OK;
@@ -453,16 +441,9 @@ int Engine::collect_inits(bool block, bool* worked) {
     if (!first) first = jp.get();
     if (jp->fut.wait_for(std::chrono::seconds(0)) == std::future_status::ready) ready.push_back(jp.get());
   }
-  // Blocking: wait for whichever running initial partition finishes first.
-  while (ready.empty() && block && first) {
-    {
-      std::unique_lock<std::mutex> lk(dmu_);
-      dcv_.wait(lk, [this] { return inits_done_ > inits_seen_; });
-      inits_seen_ = inits_done_;
-    }
-    for (auto& jp : jobs_)
-      if (jp->phase == INIT_RUNNING && jp->fut.wait_for(std::chrono::seconds(0)) == std::future_status::ready)
-        ready.push_back(jp.get());
+  if (ready.empty() && block && first) {
+    first->fut.wait();
+    ready.push_back(first);
   }
   if (ready.empty()) return GK_OK;
   *worked = true;

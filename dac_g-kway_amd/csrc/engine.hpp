@@ -28,23 +28,11 @@ class WorkerPool {
  public:
   explicit WorkerPool(int n);
   ~WorkerPool();
-  // Tasks run highest priority first, in submission order within a priority.
-  std::future<int> submit(std::function<int()> f, int64_t prio = 0);
-  // Called on the worker after each task, once its future is ready (set
-  // before the first submit).
-  void set_done_hook(std::function<void()> h) { hook_ = std::move(h); }
+  std::future<int> submit(std::function<int()> f);
 
  private:
-  std::function<void()> hook_;
-  struct Item {
-    int64_t prio;
-    uint64_t seq;
-    std::function<void()> f;
-    bool operator<(const Item& o) const { return prio != o.prio ? prio < o.prio : seq > o.seq; }
-  };
   std::vector<std::thread> th_;
-  std::priority_queue<Item> q_;
-  uint64_t seq_ = 0;
+  std::queue<std::function<void()>> q_;
   std::mutex m_;
   std::condition_variable cv_;
   bool stop_ = false;
@@ -184,9 +172,6 @@ class Engine {
   int64_t steps_ = 0, rounds_ = 0;
   double prof_ms_[6] = {0, 0, 0, 0, 0, 0};  // host wall time per step phase
   std::atomic<long long> init_cpu_us_{0};   // worker CPU time in initial partitions
-  std::mutex dmu_;                          // initial partitions finished so far, for
-  std::condition_variable dcv_;             // a blocking collect that waits for any
-  uint64_t inits_done_ = 0, inits_seen_ = 0;
   std::vector<std::unique_ptr<Job>> jobs_;
   WorkerPool pool_;  // declared last: joined before the jobs it works on go away
 };
